@@ -1,0 +1,340 @@
+/*
+ * pfhip.h - C ABI of pathfinder_amd, an MI355X (gfx950) implementation of
+ * Pathfinder's D3D11-level compute rasterization path:
+ *   bound -> dice -> bin -> propagate -> sort -> fill -> tile.
+ *
+ * Two levels, mirroring the reference:
+ *
+ *  1. The renderer seam. The reference drives its D3D11 level through
+ *     RendererD3D11 (renderer/src/gpu/d3d11/renderer.rs) with four calls made
+ *     by Renderer::render_command / end_scene (renderer/src/gpu/renderer.rs:
+ *     350-460): upload_scene, prepare_tiles, prepare_and_draw_tiles,
+ *     end_frame. pfh_context_* replaces RendererD3D11 plus the RendererCore
+ *     state it uses (area LUT, mask pages, texture metadata, stats).
+ *
+ *  2. The scene front-end. pfh_scene_* mirrors Scene / SceneBuilder
+ *     (renderer/src/scene.rs, renderer/src/builder.rs) and produces the same
+ *     RenderCommand stream (renderer/src/gpu_data.rs:37-105) that the seam
+ *     consumes, so a caller can hand over a whole scene.
+ *
+ * Conventions follow the reference's own C API (c/src/lib.rs): opaque
+ * handles created by a create call and released by the matching destroy call, row-major
+ * POD structs, caller-owned input arrays borrowed for the duration of a call.
+ * Unlike the reference (which panics), every fallible call returns a
+ * pfh_status; pfh_last_error() gives the message. A context is used from one
+ * thread at a time, like Renderer (&mut self).
+ *
+ * All wire structs below are byte-identical to the #[repr(C)] structs of
+ * renderer/src/gpu_data.rs, so a Rust binding can pass `slice.as_ptr()`.
+ */
+#ifndef PFHIP_H
+#define PFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFHIP_ABI_VERSION 1
+
+typedef enum pfh_status {
+    PFH_OK = 0,
+    PFH_ERROR_INVALID_ARGUMENT = 1,
+    PFH_ERROR_HIP = 2,          /* a HIP runtime call failed */
+    PFH_ERROR_OUT_OF_MEMORY = 3,
+    PFH_ERROR_STATE = 4,        /* call out of order (e.g. draw before prepare) */
+    PFH_ERROR_NO_DEVICE = 5,    /* no gfx950 device visible */
+    PFH_ERROR_LIMIT = 6         /* a reference format limit would be exceeded */
+} pfh_status;
+
+/* ------------------------------------------------------------ geometry */
+
+/* c/src/lib.rs:134-177 */
+typedef struct pfh_vector2f { float x, y; } pfh_vector2f;
+typedef struct pfh_rect_f { pfh_vector2f origin, lower_right; } pfh_rect_f;
+typedef struct pfh_color_u { uint8_t r, g, b, a; } pfh_color_u;
+typedef struct pfh_color_f { float r, g, b, a; } pfh_color_f;
+/* PFTransform2F: row-major 2x2 matrix + translation. */
+typedef struct pfh_transform2f {
+    float m00, m01, m10, m11;
+    float tx, ty;
+} pfh_transform2f;
+
+/* ------------------------------------------------------------ wire formats
+ * renderer/src/gpu_data.rs */
+
+/* SegmentIndicesD3D11 (gpu_data.rs:178-181); flags builder.rs:47-48 */
+#define PFH_CURVE_IS_QUADRATIC 0x80000000u
+#define PFH_CURVE_IS_CUBIC 0x40000000u
+typedef struct pfh_segment_indices { uint32_t first_point_index, flags; } pfh_segment_indices;
+
+/* DiceMetadataD3D11 (gpu_data.rs:329-334) */
+typedef struct pfh_dice_metadata {
+    uint32_t global_path_id, first_global_segment_index, first_batch_segment_index, pad;
+} pfh_dice_metadata;
+
+/* TilePathInfoD3D11 (gpu_data.rs:299-309) */
+typedef struct pfh_tile_path_info {
+    int16_t tile_min_x, tile_min_y, tile_max_x, tile_max_y;
+    uint32_t first_tile_index;
+    uint16_t color;
+    uint8_t ctrl;
+    int8_t backdrop;
+} pfh_tile_path_info;
+
+/* PropagateMetadataD3D11 (gpu_data.rs:314-325); clip_path_index ~0 = none */
+typedef struct pfh_propagate_metadata {
+    int32_t tile_rect[4]; /* min_x, min_y, max_x, max_y (RectI) */
+    uint32_t tile_offset, path_index, z_write, clip_path_index;
+    uint32_t backdrop_offset, pad0, pad1, pad2;
+} pfh_propagate_metadata;
+
+/* BackdropInfoD3D11 (gpu_data.rs:409-414) */
+typedef struct pfh_backdrop_info {
+    int32_t initial_backdrop, tile_x_offset;
+    uint32_t path_index;
+} pfh_backdrop_info;
+
+/* TileD3D11 (gpu_data.rs:279-288) */
+typedef struct pfh_tile {
+    int32_t next_tile_id, first_fill_id;
+    int16_t alpha_tile_id_lo;
+    int8_t alpha_tile_id_hi, backdrop_delta;
+    uint16_t color;
+    uint8_t ctrl;
+    int8_t backdrop;
+} pfh_tile;
+
+/* Fill (gpu_data.rs:356-363): 8.8 fixed point within the tile + list link */
+typedef struct pfh_fill { uint16_t from_x, from_y, to_x, to_y; uint32_t link; } pfh_fill;
+
+/* MicrolineD3D11 (gpu_data.rs:418-428) */
+typedef struct pfh_microline {
+    int16_t from_x_px, from_y_px, to_x_px, to_y_px;
+    uint8_t from_x_subpx, from_y_subpx, to_x_subpx, to_y_subpx;
+    uint32_t path_index;
+} pfh_microline;
+
+/* AlphaTileD3D11 (gpu_data.rs:292-295) */
+typedef struct pfh_alpha_tile { uint32_t alpha_tile_index, clip_tile_index; } pfh_alpha_tile;
+
+/* SegmentsD3D11 (gpu_data.rs:171-174) */
+typedef struct pfh_segments {
+    const pfh_vector2f *points;
+    uint32_t point_count;
+    const pfh_segment_indices *indices;
+    uint32_t index_count;
+} pfh_segments;
+
+/* PathSource (gpu_data.rs:143-148) */
+#define PFH_PATH_SOURCE_DRAW 0u
+#define PFH_PATH_SOURCE_CLIP 1u
+
+/* TileBatchDataD3D11 + PrepareTilesInfoD3D11 (gpu_data.rs:123-168) and the
+ * clip batch id of ClippedPathInfo. Arrays have path_count entries except
+ * backdrops (backdrop_count = sum of tile-rect widths). */
+typedef struct pfh_tile_batch {
+    uint32_t batch_id;
+    uint32_t path_source;
+    uint32_t path_count, tile_count, segment_count;
+    pfh_transform2f transform;
+    const pfh_dice_metadata *dice_metadata;
+    const pfh_tile_path_info *tile_path_info;
+    const pfh_propagate_metadata *propagate_metadata;
+    const pfh_backdrop_info *backdrops;
+    uint32_t backdrop_count;
+    int32_t clip_batch_id; /* -1: no clipped paths */
+} pfh_tile_batch;
+
+/* ------------------------------------------------------------ context */
+
+typedef struct pfh_context pfh_context;
+
+/* RendererLevel (gpu/options.rs:41-46); only D3D11 is implemented. */
+#define PFH_RENDERER_LEVEL_D3D11 2
+
+typedef struct pfh_context_options {
+    int32_t device;               /* HIP device ordinal */
+    int32_t level;                /* PFH_RENDERER_LEVEL_D3D11 */
+    const uint8_t *area_lut_rgba; /* 256x256 RGBA8 area LUT; NULL = built-in copy */
+    uint32_t initial_microline_capacity; /* 0 = 16384 (d3d11/renderer.rs:43) */
+    uint32_t initial_fill_capacity;      /* 0 = 16384 (d3d11/renderer.rs:44) */
+    uint32_t enable_timing;       /* record per-stage device times */
+} pfh_context_options;
+
+/* Renderer::new (gpu/renderer.rs:181-340) + RendererD3D11::new. */
+pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **out);
+void pfh_context_destroy(pfh_context *ctx);
+const char *pfh_last_error(const pfh_context *ctx);
+
+/* RendererOptions {dest, background_color} (gpu/options.rs:25-33) for one frame.
+ * Renderer::begin_scene (gpu/renderer.rs:350-358). */
+typedef struct pfh_frame_options {
+    int32_t width, height;        /* destination framebuffer in pixels */
+    int32_t has_background;       /* 1: first draw clears to background_color */
+    pfh_color_f background_color;
+} pfh_frame_options;
+pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame);
+
+/* UploadTextureMetadata: one solid base colour per paint (PaintMetadata
+ * base_color, paint.rs:323-331; encoded as f16 per gpu/renderer.rs:702-775). */
+pfh_status pfh_upload_texture_metadata(pfh_context *ctx, const pfh_color_u *base_colors,
+                                       uint32_t paint_count);
+
+/* RendererD3D11::upload_scene (d3d11/renderer.rs:236-241). */
+pfh_status pfh_upload_scene(pfh_context *ctx, const pfh_segments *draw_segments,
+                            const pfh_segments *clip_segments);
+
+/* RendererD3D11::prepare_tiles (d3d11/renderer.rs:427-542): PrepareClipTilesD3D11. */
+pfh_status pfh_prepare_tiles(pfh_context *ctx, const pfh_tile_batch *batch);
+
+/* RendererD3D11::prepare_and_draw_tiles (d3d11/renderer.rs:414-424): DrawTilesD3D11. */
+pfh_status pfh_prepare_and_draw_tiles(pfh_context *ctx, const pfh_tile_batch *batch);
+
+/* Renderer::end_scene (gpu/renderer.rs:426-460). Waits for the frame, grows
+ * any buffer that overflowed and re-renders the frame if one did. Batch
+ * buffers stay readable until the next pfh_begin_scene (the reference frees
+ * them here, d3d11/renderer.rs:784-795). */
+pfh_status pfh_end_scene(pfh_context *ctx);
+
+/* Re-runs every batch of the last scene from device-resident inputs (no host
+ * uploads, no host synchronisation). Used to time frames. */
+pfh_status pfh_rerender_scene(pfh_context *ctx);
+pfh_status pfh_synchronize(pfh_context *ctx);
+
+/* Reads the destination framebuffer, RGBA8, top row = scene row 0
+ * (tile.cs.glsl:67-69 writes scene row y to image row H - y; this call
+ * returns scene rows). stride in bytes (0 = width * 4). */
+pfh_status pfh_read_framebuffer(pfh_context *ctx, uint8_t *rgba, size_t stride);
+
+/* Debug readbacks of one batch of the last scene, in the reference layouts. */
+typedef enum pfh_debug_kind {
+    PFH_DEBUG_MICROLINES = 0,     /* pfh_microline[microline count] */
+    PFH_DEBUG_FILLS = 1,          /* pfh_fill[fill count] */
+    PFH_DEBUG_TILES = 2,          /* pfh_tile[tile count] */
+    PFH_DEBUG_Z_BUFFER = 3,       /* int32[8 + fb tiles] (8-word header) */
+    PFH_DEBUG_FIRST_TILE_MAP = 4, /* int32[fb tiles] */
+    PFH_DEBUG_ALPHA_TILES = 5,    /* pfh_alpha_tile[batch alpha tile count] */
+    PFH_DEBUG_BACKDROPS = 6,      /* pfh_backdrop_info[backdrop count], after bin */
+    PFH_DEBUG_ALPHA_RANGE = 7     /* uint32[2]: first and end alpha tile id */
+} pfh_debug_kind;
+pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind kind, void *dst,
+                          size_t capacity, size_t *out_len);
+/* The frame's alpha mask: 64 RGBA8 texels per alpha tile, texel (x, y) of
+ * tile `id` at word id*64 + y*16 + x (the reference stores the same texel at
+ * (16*(id & 255) + x, 4*(id >> 8) + y) of a 4096-wide RGBA8 texture). */
+pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
+                         size_t *out_words);
+
+/* RenderStats (gpu/perf.rs:20-44) plus path-specific counts and device times. */
+typedef struct pfh_render_stats {
+    uint64_t path_count, fill_count, alpha_tile_count, total_tile_count, drawcall_count;
+    uint64_t microline_count, segment_count, column_count;
+    uint64_t gpu_bytes_allocated;
+    uint32_t overflow_replays;
+    /* device milliseconds of the last frame, by stage (valid when timing is on) */
+    float ms_dice, ms_bound, ms_bin, ms_propagate, ms_sort, ms_fill, ms_tile, ms_frame;
+} pfh_render_stats;
+pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats);
+
+/* ------------------------------------------------------------ scene front-end */
+
+typedef struct pfh_scene pfh_scene;
+typedef struct pfh_command_list pfh_command_list;
+
+/* FillRule (content/src/fill.rs:18-23) */
+#define PFH_FILL_RULE_WINDING 0
+#define PFH_FILL_RULE_EVEN_ODD 1
+/* BlendMode: SrcOver is the only mode on this path. */
+#define PFH_BLEND_MODE_SRC_OVER 0
+/* PointFlags (content/src/outline.rs:50-60) */
+#define PFH_POINT_CONTROL_0 0x01u
+#define PFH_POINT_CONTROL_1 0x02u
+
+/* Outline (content/src/outline.rs:34-60) as flat arrays: contour k spans
+ * points [contour_ends[k-1], contour_ends[k]). Bounds are the union of every
+ * point, control points included (outline.rs:560-568). */
+typedef struct pfh_outline {
+    const pfh_vector2f *points;
+    const uint8_t *flags;
+    uint32_t point_count;
+    const uint32_t *contour_ends;
+    uint32_t contour_count;
+} pfh_outline;
+
+pfh_scene *pfh_scene_create(void);
+void pfh_scene_destroy(pfh_scene *scene);
+/* Scene::set_view_box (scene.rs:222). */
+void pfh_scene_set_view_box(pfh_scene *scene, const pfh_rect_f *view_box);
+/* Scene::push_paint (scene.rs:191, palette dedup paint.rs:381-390). Returns the PaintId. */
+uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color);
+/* Scene::push_clip_path (scene.rs:98). clip_path = -1 for none. Returns ClipPathId. */
+uint32_t pfh_scene_push_clip_path(pfh_scene *scene, const pfh_outline *outline, int32_t fill_rule,
+                                  int32_t clip_path);
+/* Scene::push_draw_path (scene.rs:77) with DrawPath::{set_clip_path,set_fill_rule,
+ * set_blend_mode}. Returns DrawPathId. */
+uint32_t pfh_scene_push_draw_path(pfh_scene *scene, const pfh_outline *outline, uint32_t paint,
+                                  int32_t fill_rule, int32_t clip_path, int32_t blend_mode);
+/* Bulk variants: path i owns contours [path_contour_ends[i-1], path_contour_ends[i]);
+ * contour_ends index the shared points array. Per-path arrays may be NULL
+ * (defaults: winding, no clip, src-over). Returns the first id. */
+uint32_t pfh_scene_push_draw_paths(pfh_scene *scene, const pfh_vector2f *points,
+                                   const uint8_t *flags, uint32_t point_count,
+                                   const uint32_t *contour_ends, uint32_t contour_count,
+                                   const uint32_t *path_contour_ends, uint32_t path_count,
+                                   const uint32_t *paints, const int32_t *fill_rules,
+                                   const int32_t *clip_paths, const int32_t *blend_modes);
+uint32_t pfh_scene_push_clip_paths(pfh_scene *scene, const pfh_vector2f *points,
+                                   const uint8_t *flags, uint32_t point_count,
+                                   const uint32_t *contour_ends, uint32_t contour_count,
+                                   const uint32_t *path_contour_ends, uint32_t path_count,
+                                   const int32_t *fill_rules, const int32_t *clip_paths);
+uint32_t pfh_scene_draw_path_count(const pfh_scene *scene);
+
+/* BuildOptions (options.rs:53-61): a 2D transform only. */
+typedef struct pfh_build_options {
+    pfh_transform2f transform;
+} pfh_build_options;
+
+/* SceneBuilder::build (builder.rs:148-222) into a command list. */
+pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
+                           pfh_command_list **out);
+void pfh_command_list_destroy(pfh_command_list *list);
+
+/* RenderCommand kinds (gpu_data.rs:37-105) produced for the D3D11 level. */
+typedef enum pfh_command_kind {
+    PFH_COMMAND_START = 0,
+    PFH_COMMAND_UPLOAD_TEXTURE_METADATA = 1,
+    PFH_COMMAND_UPLOAD_SCENE_D3D11 = 2,
+    PFH_COMMAND_PREPARE_CLIP_TILES_D3D11 = 3,
+    PFH_COMMAND_DRAW_TILES_D3D11 = 4,
+    PFH_COMMAND_FINISH = 5
+} pfh_command_kind;
+uint32_t pfh_command_list_len(const pfh_command_list *list);
+int32_t pfh_command_kind_at(const pfh_command_list *list, uint32_t index);
+/* For PREPARE_CLIP_TILES_D3D11 / DRAW_TILES_D3D11: the batch (pointers stay
+ * valid while the list lives). */
+pfh_status pfh_command_batch(const pfh_command_list *list, uint32_t index, pfh_tile_batch *out);
+/* For UPLOAD_SCENE_D3D11. */
+pfh_status pfh_command_segments(const pfh_command_list *list, uint32_t index,
+                                pfh_segments *draw, pfh_segments *clip);
+/* For UPLOAD_TEXTURE_METADATA: base colours, one per paint. */
+pfh_status pfh_command_paints(const pfh_command_list *list, uint32_t index,
+                              const pfh_color_u **colors, uint32_t *count);
+
+/* Renderer::render_command for every command of the list, between
+ * pfh_begin_scene and pfh_end_scene (the caller brackets it). */
+pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list);
+
+/* Scene::build_and_render (scene.rs:369-378): begin, build, render, end. */
+pfh_status pfh_scene_build_and_render(pfh_scene *scene, pfh_context *ctx,
+                                      const pfh_frame_options *frame,
+                                      const pfh_build_options *options);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFHIP_H */

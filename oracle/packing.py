@@ -1,0 +1,197 @@
+"""Restatement of the reference's scene packing for the D3D11 level.
+
+TEST INFRASTRUCTURE ONLY: this is the checker for pathfinder_amd's C++
+SceneBuilder (pathfinder_amd/csrc/pfh_scene.cpp); the product never imports it.
+
+Follows renderer/src/builder.rs:
+  SceneBuilder::build                               :148-222
+  BuiltSegments::from_scene, SegmentsD3D11::add_path :777-840
+  build_tile_batches_for_draw_path_display_item     :885-1056
+  prepare_draw_path_for_gpu_binning                 :1058-1095
+  add_clip_path_to_batch, prepare_clip_path_...     :1124-1220
+  TileBatchDataD3D11::push, init_backdrops          :653-768
+  send_to                                           :1097-1113
+with geometry from geometry/src/rect.rs (intersection :122-137, round_out
+:171-173), geometry/src/transform2d.rs (Transform2F * RectF :328-338) and
+renderer/src/tiles.rs:64-66, all in binary32 like the reference's F32x4 code.
+"""
+import numpy as np
+
+F = np.float32
+CONTROL = 0x03
+CUBIC = 0x40000000
+QUAD = 0x80000000
+MAX_CLIP_BATCHES = 32
+PREPARE_CLIP = 3
+DRAW = 4
+
+
+def _bounds(points, contour_ends):
+    """Outline::bounds: union of all points of non-empty contours, or zero."""
+    rect = None
+    start = 0
+    for end in contour_ends:
+        end = int(end)
+        if end > start:
+            p = points[start:end]
+            cb = [p[:, 0].min(), p[:, 1].min(), p[:, 0].max(), p[:, 1].max()]
+            if rect is None:
+                rect = cb
+            else:
+                rect = [min(rect[0], cb[0]), min(rect[1], cb[1]),
+                        max(rect[2], cb[2]), max(rect[3], cb[3])]
+        start = max(start, end)
+    return [F(0)] * 4 if rect is None else [F(v) for v in rect]
+
+
+def _xpoint(t, x, y):
+    m00, m01, m10, m11, tx, ty = t
+    return (F(F(m00 * x) + F(m01 * y)) + tx, F(F(m10 * x) + F(m11 * y)) + ty)
+
+
+def _xrect(t, r):
+    x0, y0, x1, y1 = r
+    c = [_xpoint(t, x0, y0), _xpoint(t, x1, y0), _xpoint(t, x0, y1), _xpoint(t, x1, y1)]
+    xs = [p[0] for p in c]
+    ys = [p[1] for p in c]
+    return [min(xs), min(ys), max(xs), max(ys)]
+
+
+def _intersect(a, b):
+    if not (a[0] < b[2] and a[1] < b[3] and b[0] < a[2] and b[1] < a[3]):
+        return None
+    return [max(a[0], b[0]), max(a[1], b[1]), min(a[2], b[2]), min(a[3], b[3])]
+
+
+def _tile_bounds(r):
+    s = F(1.0 / 16.0)
+    return [int(np.floor(F(r[0] * s))), int(np.floor(F(r[1] * s))),
+            int(np.ceil(F(r[2] * s))), int(np.ceil(F(r[3] * s)))]
+
+
+def add_path(points, flags, contour_ends, out_points, out_indices):
+    """SegmentsD3D11::add_path: returns the segment range."""
+    first = len(out_indices)
+    start = 0
+    for end in contour_ends:
+        end = int(end)
+        n = end - start
+        for i in range(n):
+            if not (flags[start + i] & CONTROL):
+                f = 0
+                if i + 1 < n and (flags[start + i + 1] & 0x01):
+                    if i + 2 < n and (flags[start + i + 2] & 0x02):
+                        f = CUBIC
+                    else:
+                        f = QUAD
+                out_indices.append((len(out_points), f))
+            out_points.append(points[start + i])
+        if n > 0:
+            out_points.append(points[start])
+        start = max(start, end)
+    return first, len(out_indices)
+
+
+class _Batch:
+    def __init__(self, batch_id, source, transform):
+        self.batch_id = batch_id
+        self.path_source = source
+        self.transform = transform
+        self.path_count = self.tile_count = self.segment_count = 0
+        self.prop, self.dice, self.tpi, self.backdrops = [], [], [], []
+        self.clip_batch_id = -1
+
+    def push(self, rect, paint, ctrl, global_id, seg_range, clip, z_write):
+        """TileBatchDataD3D11::push."""
+        idx = self.path_count
+        self.path_count += 1
+        w, h = rect[2] - rect[0], rect[3] - rect[1]
+        self.prop.append([rect[0], rect[1], rect[2], rect[3], self.tile_count, idx,
+                          1 if z_write else 0, clip[1] if clip else -1,
+                          len(self.backdrops), 0, 0, 0])
+        for x in range(w):
+            self.backdrops.append([0, x, idx])
+        self.dice.append([global_id, seg_range[0], self.segment_count, 0])
+        lo = (rect[0] & 0xFFFF) | ((rect[1] & 0xFFFF) << 16)
+        hi = (rect[2] & 0xFFFF) | ((rect[3] & 0xFFFF) << 16)
+        self.tpi.append([lo, hi, self.tile_count, (paint & 0xFFFF) | (ctrl << 16)])
+        self.tile_count += w * h
+        self.segment_count += seg_range[1] - seg_range[0]
+        if clip and self.clip_batch_id < 0:
+            self.clip_batch_id = clip[0]
+        return idx
+
+    def to_dict(self):
+        return {
+            "batch_id": self.batch_id, "path_source": self.path_source,
+            "path_count": self.path_count, "tile_count": self.tile_count,
+            "segment_count": self.segment_count,
+            "transform": np.asarray(self.transform, np.float32),
+            "dice_metadata": np.asarray(self.dice, np.uint32).reshape(-1, 4),
+            "tile_path_info": np.asarray(self.tpi, np.uint32).reshape(-1, 4),
+            "propagate_metadata": np.asarray(self.prop, np.int64).astype(np.int32).reshape(-1, 12),
+            "backdrops": np.asarray(self.backdrops, np.int32).reshape(-1, 3),
+            "clip_batch_id": self.clip_batch_id,
+        }
+
+
+def build(scene, transform=(1, 0, 0, 1, 0, 0)):
+    """SceneBuilder::build for the D3D11 level. Returns
+    {"paints": (N, 4) u8, "draw": {...}, "clip": {...}, "batches": [(kind, dict)]}."""
+    t = [F(v) for v in transform]
+    vb = [F(scene.view_box.x0), F(scene.view_box.y0), F(scene.view_box.x1), F(scene.view_box.y1)]
+    draw = scene.draw_arrays()
+    clip = scene.clip_arrays()
+
+    def segments(arrays):
+        pts, idx, ranges = [], [], []
+        for i in range(len(arrays)):
+            p, f, e = arrays.path(i)
+            ranges.append(add_path(p, f, e, pts, idx))
+        return ({"points": np.asarray(pts, np.float32).reshape(-1, 2),
+                 "indices": np.asarray(idx, np.uint32).reshape(-1, 2)}, ranges)
+
+    clip_segs, clip_ranges = segments(clip)
+    draw_segs, draw_ranges = segments(draw)
+    clip_bounds = [_bounds(*clip.path(i)[0::2]) for i in range(len(clip))]
+
+    prepare_batches = []
+    clip_map = {}
+
+    def add_clip(cid, level):
+        if cid < 0 or cid >= len(clip) or level >= MAX_CLIP_BATCHES:
+            return None
+        if cid in clip_map:
+            return (level, clip_map[cid])
+        sub = add_clip(int(clip.clip_paths[cid]), level + 1)
+        rect = _tile_bounds(_xrect(t, clip_bounds[cid]))  # not intersected with the view box
+        while level >= len(prepare_batches):
+            prepare_batches.append(_Batch(len(prepare_batches), 1, t))
+        idx = prepare_batches[level].push(rect, 0, 0, cid, clip_ranges[cid], sub, True)
+        clip_map[cid] = idx
+        return (level, idx)
+
+    paints = np.asarray([[c.r, c.g, c.b, c.a] for c in scene.paints], np.uint8).reshape(-1, 4)
+    draw_batch = None
+    for i in range(len(draw)):
+        p, f, e = draw.path(i)
+        bounds = _xrect(t, _bounds(p, e))
+        clipped = _intersect(bounds, vb)
+        if clipped is None:
+            continue
+        rule = int(draw.fill_rules[i])
+        ctrl = 0x2 if rule == 1 else 0x1
+        if draw_batch is None:
+            draw_batch = _Batch(MAX_CLIP_BATCHES, 0, t)
+        cl = add_clip(int(draw.clip_paths[i]), 0)
+        paint = int(draw.paints[i])
+        opaque = paint < len(paints) and paints[paint, 3] == 255
+        draw_batch.push(_tile_bounds(clipped), paint, ctrl, i, draw_ranges[i], cl, opaque)
+
+    batches = []
+    for b in reversed(prepare_batches):
+        if b.path_count > 0:
+            batches.append((PREPARE_CLIP, b.to_dict()))
+    if draw_batch is not None:
+        batches.append((DRAW, draw_batch.to_dict()))
+    return {"paints": paints, "draw": draw_segs, "clip": clip_segs, "batches": batches}

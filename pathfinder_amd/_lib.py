@@ -1,0 +1,193 @@
+"""ctypes binding of libpfhip.so (include/pfhip.h).
+
+The library is the product: this module only loads it and declares the C
+signatures. It raises if the library has not been built; there is no
+fallback implementation.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpfhip.so")
+
+# pfh_status
+OK = 0
+STATUS_NAMES = {
+    0: "PFH_OK",
+    1: "PFH_ERROR_INVALID_ARGUMENT",
+    2: "PFH_ERROR_HIP",
+    3: "PFH_ERROR_OUT_OF_MEMORY",
+    4: "PFH_ERROR_STATE",
+    5: "PFH_ERROR_NO_DEVICE",
+    6: "PFH_ERROR_LIMIT",
+}
+
+RENDERER_LEVEL_D3D11 = 2
+PATH_SOURCE_DRAW = 0
+PATH_SOURCE_CLIP = 1
+FILL_RULE_WINDING = 0
+FILL_RULE_EVEN_ODD = 1
+POINT_CONTROL_0 = 0x01
+POINT_CONTROL_1 = 0x02
+CURVE_IS_QUADRATIC = 0x80000000
+CURVE_IS_CUBIC = 0x40000000
+
+COMMAND_START = 0
+COMMAND_UPLOAD_TEXTURE_METADATA = 1
+COMMAND_UPLOAD_SCENE_D3D11 = 2
+COMMAND_PREPARE_CLIP_TILES_D3D11 = 3
+COMMAND_DRAW_TILES_D3D11 = 4
+COMMAND_FINISH = 5
+
+DEBUG_MICROLINES = 0
+DEBUG_FILLS = 1
+DEBUG_TILES = 2
+DEBUG_Z_BUFFER = 3
+DEBUG_FIRST_TILE_MAP = 4
+DEBUG_ALPHA_TILES = 5
+DEBUG_BACKDROPS = 6
+DEBUG_ALPHA_RANGE = 7
+
+
+class Vector2F(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class RectF(C.Structure):
+    _fields_ = [("origin", Vector2F), ("lower_right", Vector2F)]
+
+
+class ColorU(C.Structure):
+    _fields_ = [("r", C.c_uint8), ("g", C.c_uint8), ("b", C.c_uint8), ("a", C.c_uint8)]
+
+
+class ColorF(C.Structure):
+    _fields_ = [("r", C.c_float), ("g", C.c_float), ("b", C.c_float), ("a", C.c_float)]
+
+
+class Transform2F(C.Structure):
+    _fields_ = [("m00", C.c_float), ("m01", C.c_float), ("m10", C.c_float), ("m11", C.c_float),
+                ("tx", C.c_float), ("ty", C.c_float)]
+
+
+class Segments(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("point_count", C.c_uint32),
+                ("indices", C.c_void_p), ("index_count", C.c_uint32)]
+
+
+class TileBatch(C.Structure):
+    _fields_ = [("batch_id", C.c_uint32), ("path_source", C.c_uint32),
+                ("path_count", C.c_uint32), ("tile_count", C.c_uint32),
+                ("segment_count", C.c_uint32), ("transform", Transform2F),
+                ("dice_metadata", C.c_void_p), ("tile_path_info", C.c_void_p),
+                ("propagate_metadata", C.c_void_p), ("backdrops", C.c_void_p),
+                ("backdrop_count", C.c_uint32), ("clip_batch_id", C.c_int32)]
+
+
+class ContextOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("level", C.c_int32), ("area_lut_rgba", C.c_void_p),
+                ("initial_microline_capacity", C.c_uint32),
+                ("initial_fill_capacity", C.c_uint32), ("enable_timing", C.c_uint32)]
+
+
+class FrameOptions(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("has_background", C.c_int32),
+                ("background_color", ColorF)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("path_count", C.c_uint64), ("fill_count", C.c_uint64),
+                ("alpha_tile_count", C.c_uint64), ("total_tile_count", C.c_uint64),
+                ("drawcall_count", C.c_uint64), ("microline_count", C.c_uint64),
+                ("segment_count", C.c_uint64), ("column_count", C.c_uint64),
+                ("gpu_bytes_allocated", C.c_uint64), ("overflow_replays", C.c_uint32),
+                ("ms_dice", C.c_float), ("ms_bound", C.c_float), ("ms_bin", C.c_float),
+                ("ms_propagate", C.c_float), ("ms_sort", C.c_float), ("ms_fill", C.c_float),
+                ("ms_tile", C.c_float), ("ms_frame", C.c_float)]
+
+
+class BuildOptions(C.Structure):
+    _fields_ = [("transform", Transform2F)]
+
+
+class Outline(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("flags", C.c_void_p), ("point_count", C.c_uint32),
+                ("contour_ends", C.c_void_p), ("contour_count", C.c_uint32)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+I32 = C.c_int32
+SZ = C.c_size_t
+
+# name -> (restype, argtypes); every entry point of include/pfhip.h
+SIGNATURES = {
+    "pfh_context_create": (I32, [C.POINTER(ContextOptions), C.POINTER(P)]),
+    "pfh_context_destroy": (None, [P]),
+    "pfh_last_error": (C.c_char_p, [P]),
+    "pfh_begin_scene": (I32, [P, C.POINTER(FrameOptions)]),
+    "pfh_upload_texture_metadata": (I32, [P, P, U32]),
+    "pfh_upload_scene": (I32, [P, C.POINTER(Segments), C.POINTER(Segments)]),
+    "pfh_prepare_tiles": (I32, [P, C.POINTER(TileBatch)]),
+    "pfh_prepare_and_draw_tiles": (I32, [P, C.POINTER(TileBatch)]),
+    "pfh_end_scene": (I32, [P]),
+    "pfh_rerender_scene": (I32, [P]),
+    "pfh_synchronize": (I32, [P]),
+    "pfh_read_framebuffer": (I32, [P, P, SZ]),
+    "pfh_read_debug": (I32, [P, U32, I32, P, SZ, C.POINTER(SZ)]),
+    "pfh_read_mask": (I32, [P, P, SZ, C.POINTER(SZ)]),
+    "pfh_get_stats": (I32, [P, C.POINTER(RenderStats)]),
+    "pfh_scene_create": (P, []),
+    "pfh_scene_destroy": (None, [P]),
+    "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),
+    "pfh_scene_push_paint": (U32, [P, C.POINTER(ColorU)]),
+    "pfh_scene_push_clip_path": (U32, [P, C.POINTER(Outline), I32, I32]),
+    "pfh_scene_push_draw_path": (U32, [P, C.POINTER(Outline), U32, I32, I32, I32]),
+    "pfh_scene_push_draw_paths": (U32, [P, P, P, U32, P, U32, P, U32, P, P, P, P]),
+    "pfh_scene_push_clip_paths": (U32, [P, P, P, U32, P, U32, P, U32, P, P]),
+    "pfh_scene_draw_path_count": (U32, [P]),
+    "pfh_scene_build": (I32, [P, C.POINTER(BuildOptions), C.POINTER(P)]),
+    "pfh_command_list_destroy": (None, [P]),
+    "pfh_command_list_len": (U32, [P]),
+    "pfh_command_kind_at": (I32, [P, U32]),
+    "pfh_command_batch": (I32, [P, U32, C.POINTER(TileBatch)]),
+    "pfh_command_segments": (I32, [P, U32, C.POINTER(Segments), C.POINTER(Segments)]),
+    "pfh_command_paints": (I32, [P, U32, C.POINTER(P), C.POINTER(U32)]),
+    "pfh_render_commands": (I32, [P, P]),
+    "pfh_scene_build_and_render": (I32, [P, P, C.POINTER(FrameOptions),
+                                         C.POINTER(BuildOptions)]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libpfhip.so once; raises if it is missing (build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("pathfinder_amd: %s is not built; run "
+                           "`python -m pathfinder_amd.build`" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PfhError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+def check(status, ctx=None):
+    if status != OK:
+        msg = ""
+        if ctx:
+            raw = load().pfh_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise PfhError(status, msg)

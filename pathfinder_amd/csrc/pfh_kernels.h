@@ -1,0 +1,112 @@
+// pfh_kernels.h - launch interface of the gfx950 kernels (pfh_kernels.hip).
+#ifndef PFH_KERNELS_H
+#define PFH_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/pfhip.h"
+
+namespace pfh {
+
+// Per-batch device counters. Written by the kernels, read by the host once
+// per frame (pfh_end_scene) and by later kernels of the same batch.
+struct BatchCounters {
+    uint32_t microlines;      // total microlines diced (may exceed capacity)
+    uint32_t fills;           // total fills binned (may exceed capacity)
+    uint32_t alpha;           // alpha tiles allocated by this batch
+    uint32_t alpha_start;     // frame-global id of the batch's first alpha tile
+    uint32_t microline_count; // min(microlines, capacity)
+    uint32_t fill_count;      // min(fills, capacity)
+    uint32_t overflow;        // bit 0 microlines, bit 1 fills, bit 2 mask pages
+    uint32_t error;           // bit 0 bad segment/point index, bit 1 bad path index,
+                              // bit 2 tile index outside the batch
+    uint32_t frame_alpha_end; // frame alpha count after this batch
+    uint32_t pad[7];
+};
+
+// Frame-level device state shared by the batches of one frame.
+struct FrameState {
+    uint32_t alpha_count;     // Renderer core alpha_tile_count (gpu/renderer.rs:146, 357)
+    uint32_t pad[15];
+};
+
+struct DeviceScene {
+    const float2 *points;
+    const uint2 *indices;
+    uint32_t point_count, index_count;
+};
+
+// Everything the kernels of one batch need.
+struct BatchArgs {
+    // inputs
+    DeviceScene scene;
+    const uint4 *dice_md;          // pfh_dice_metadata
+    const uint4 *tile_path_info;   // pfh_tile_path_info
+    const int32_t *prop_md;        // pfh_propagate_metadata, 12 words per path
+    const int32_t *backdrops_init; // pfh_backdrop_info, 3 words per column
+    uint32_t path_count, tile_count, segment_count, column_count;
+    float xform[6];                // m00 m01 m10 m11 tx ty
+    // clip batch (propagate / fill)
+    const int32_t *clip_prop_md;
+    const uint4 *clip_tiles;
+    uint32_t clip_path_count, clip_tile_count;
+    // working buffers
+    int32_t *backdrops;            // working copy, 3 words per column
+    uint32_t *seg_offsets;         // segment_count + 1
+    uint4 *microlines;             // microline_capacity
+    uint32_t microline_capacity;
+    uint32_t *ml_fill_offsets;     // microline_capacity + 1
+    uint32_t *fills;               // 3 words per fill, fill_capacity
+    uint32_t *fill_tiles;          // tile index per fill
+    uint32_t fill_capacity;
+    uint4 *tiles;                  // tile_count
+    uint32_t *col_alpha_offsets;   // column_count + 1
+    uint2 *alpha_tiles;            // tile_count
+    int32_t *zbuffer;              // 8 + fb tiles
+    int32_t *first_tile_map;       // fb tiles
+    uint32_t *row_paths;           // fb_height_tiles * path_count
+    uint32_t *row_counts;          // fb_height_tiles
+    uint32_t *scan_partials;
+    BatchCounters *counters;
+    FrameState *frame;
+    // frame
+    uint32_t *mask;                // 64 words per alpha tile
+    uint32_t mask_capacity;        // alpha tiles
+    const float4 *lut;             // 256*256 texels, value = byte / 255
+    int32_t fb_width_tiles, fb_height_tiles;
+};
+
+struct TileArgs {
+    const uint4 *tiles;
+    uint32_t tile_count;
+    uint32_t mask_capacity;        // alpha tiles held by `mask`
+    const int32_t *first_tile_map;
+    const uint32_t *mask;
+    const float4 *paints;          // f16-rounded base colours
+    uint32_t paint_count;
+    uint32_t *framebuffer;         // fb_width_tiles*16 x fb_height_tiles*16 RGBA8 words
+    int32_t fb_width_tiles, fb_height_tiles;
+    int32_t load;                  // 1: LOAD_ACTION_LOAD, 0: clear
+    float clear[4];
+};
+
+void launch_batch_init(const BatchArgs &a, hipStream_t s);
+void launch_dice(const BatchArgs &a, hipStream_t s);
+void launch_bound(const BatchArgs &a, hipStream_t s);
+void launch_bin(const BatchArgs &a, hipStream_t s);
+void launch_propagate(const BatchArgs &a, hipStream_t s);
+void launch_sort(const BatchArgs &a, hipStream_t s);
+void launch_fill(const BatchArgs &a, hipStream_t s);
+void launch_tile(const TileArgs &a, hipStream_t s);
+void launch_frame_reset(FrameState *frame, hipStream_t s);
+void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s);
+void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s);
+
+// Scratch words needed by the scans of a batch.
+size_t scan_partials_words(uint32_t max_elements);
+
+}  // namespace pfh
+
+#endif

@@ -1,0 +1,1117 @@
+// pfh_kernels.hip - gfx950 kernels of the D3D11-level compute path.
+//
+// Stage map (reference shader -> kernels here):
+//   dice.cs.glsl      -> k_dice_count, scan, k_dice_emit
+//   bound.cs.glsl     -> k_bound
+//   bin.cs.glsl       -> k_bin<count>, scan, k_bin<emit>, k_fill_link
+//   propagate.cs.glsl -> k_propagate<count>, scan, k_propagate<emit>
+//   sort.cs.glsl      -> k_row_paths, k_sort_tiles (also builds the z-buffer and
+//                        the first-tile map that propagate builds with atomics)
+//   fill.cs.glsl      -> k_fill
+//   tile.cs.glsl      -> k_tile
+//
+// The reference orders microlines, fills, per-tile fill lists, alpha tiles and
+// pre-sort tile lists with global atomics, so its output depends on the order
+// invocations happen to run. Here every such order is the sequential one
+// (invocation i before i+1): counts, exclusive scans, then writes at the
+// scanned offsets; per-tile fill lists come from a per-path sort; tile lists
+// come from an ordered walk of the paths covering each framebuffer tile. The
+// only atomics left are integer adds of backdrop deltas (order-independent)
+// and one exchange per (tile, sort chunk) in k_fill_link.
+//
+// Floating point: built with -ffp-contract=off; division and sqrt are
+// correctly rounded (HIP defaults). Every formula follows the reference
+// shader's evaluation order with mix(a, b, t) = a + t * (b - a) and
+// round() = round half to even.
+#include "pfh_kernels.h"
+
+#include <hip/hip_fp16.h>
+
+namespace pfh {
+
+namespace {
+
+constexpr uint32_t kFlagQuad = 0x80000000u;
+constexpr uint32_t kFlagCubic = 0x40000000u;
+constexpr int kBlock = 256;
+constexpr uint32_t kScanTile = 4096;  // elements per scan block
+constexpr uint32_t kMaxGrid = 16384;
+
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ float min_glsl(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float max_glsl(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ float clamp_glsl(float x, float lo, float hi) {
+    return min_glsl(max_glsl(x, lo), hi);
+}
+__device__ __forceinline__ uint32_t f2u_sat(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+__device__ __forceinline__ int32_t f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x < -2147483648.0f) return (-2147483647 - 1);
+    return (int32_t)x;
+}
+__device__ __forceinline__ uint32_t unorm8(float x) {
+    float c = clamp_glsl(x, 0.0f, 1.0f);
+    return (uint32_t)floorf(c * 255.0f + 0.5f);
+}
+__device__ __forceinline__ int32_t sext24(uint32_t w) { return (int32_t)(w << 8) >> 8; }
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+inline uint32_t grid_for(uint64_t items, uint32_t per_block) {
+    uint64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return (uint32_t)g;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block (4 waves).
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds4, uint32_t *total) {
+    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) lds4[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) {
+        uint32_t s = lds4[w];
+        before += (w < wave) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+__device__ uint32_t block_reduce_sum(uint32_t v, uint32_t *lds4) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane_id() == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return t;
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of n u32 values in place (reduce, scan partials, downsweep).
+// n is read from n_ptr when given (a count produced on the device) and is
+// clamped to `cap`; out[n] receives the total, and so does *total_out.
+
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t *data,
+                                                        const uint32_t *n_ptr, uint32_t n_host,
+                                                        uint32_t cap, uint32_t *partials) {
+    __shared__ uint32_t lds4[4];
+    uint32_t n = n_ptr ? min(*n_ptr, cap) : n_host;
+    uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n) return;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kScanTile / kBlock; k++) {
+        uint32_t i = base + k * kBlock + threadIdx.x;
+        if (i < n) s += data[i];
+    }
+    uint32_t t = block_reduce_sum(s, lds4);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t *partials,
+                                                          const uint32_t *n_ptr, uint32_t n_host,
+                                                          uint32_t cap, uint32_t *total_out,
+                                                          uint32_t *data_end) {
+    __shared__ uint32_t lds4[4];
+    uint32_t n = n_ptr ? min(*n_ptr, cap) : n_host;
+    uint32_t nb = (n + kScanTile - 1) / kScanTile;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += kBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < nb ? partials[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(v, lds4, &tot);
+        if (i < nb) partials[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        if (total_out) *total_out = carry;
+        data_end[n] = carry;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_downsweep(uint32_t *data, const uint32_t *n_ptr,
+                                                           uint32_t n_host, uint32_t cap,
+                                                           const uint32_t *partials) {
+    __shared__ uint32_t lds4[4];
+    uint32_t n = n_ptr ? min(*n_ptr, cap) : n_host;
+    uint32_t base = blockIdx.x * kScanTile;
+    if (base >= n) return;
+    uint32_t carry = partials[blockIdx.x];
+    for (uint32_t k = 0; k < kScanTile / (4 * kBlock); k++) {
+        uint32_t i0 = base + k * 4 * kBlock + threadIdx.x * 4;
+        uint32_t v[4];
+        if (i0 + 3 < n) {
+            uint4 q = *reinterpret_cast<const uint4 *>(data + i0);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = (i0 + j < n) ? data[i0 + j] : 0u;
+        }
+        uint32_t s = v[0] + v[1] + v[2] + v[3];
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(s, lds4, &tot) + carry;
+        uint32_t o[4];
+        o[0] = ex;
+        o[1] = ex + v[0];
+        o[2] = o[1] + v[1];
+        o[3] = o[2] + v[2];
+        if (i0 + 3 < n) {
+            *reinterpret_cast<uint4 *>(data + i0) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (i0 + j < n) data[i0 + j] = o[j];
+        }
+        carry += tot;
+    }
+}
+
+void scan_exclusive(uint32_t *data, const uint32_t *n_ptr, uint32_t n_host, uint32_t cap,
+                    uint32_t *partials, uint32_t *total_out, hipStream_t s) {
+    uint32_t max_n = n_ptr ? cap : n_host;
+    uint32_t blocks = (max_n + kScanTile - 1) / kScanTile;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kBlock), 0, s, data, n_ptr, n_host, cap,
+                       partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, n_ptr, n_host, cap,
+                       total_out, data);
+    hipLaunchKernelGGL(k_scan_downsweep, dim3(blocks), dim3(kBlock), 0, s, data, n_ptr, n_host,
+                       cap, partials);
+}
+
+// ------------------------------------------------------------------ batch init
+
+__global__ void __launch_bounds__(kBlock) k_batch_init(BatchArgs a) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (gid == 0) {
+        BatchCounters c = {};
+        *a.counters = c;
+    }
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    for (uint32_t i = gid; i < fbn + 8; i += stride) a.zbuffer[i] = 0;
+    for (uint32_t i = gid; i < fbn; i += stride) a.first_tile_map[i] = -1;
+    for (uint32_t i = gid; i < a.column_count * 3; i += stride) a.backdrops[i] = a.backdrops_init[i];
+}
+
+__global__ void k_frame_reset(FrameState *f) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        FrameState z = {};
+        *f = z;
+    }
+}
+
+// ------------------------------------------------------------------ dice
+
+struct DiceSeg {
+    float2 p0, c0, c1, p3;
+    uint32_t path;
+    int32_t n;
+    bool curve;
+};
+
+__device__ __forceinline__ float2 get_point(const BatchArgs &a, uint32_t i) {
+    float2 p = a.scene.points[i];
+    return make_float2((a.xform[0] * p.x + a.xform[1] * p.y) + a.xform[4],
+                       (a.xform[2] * p.x + a.xform[3] * p.y) + a.xform[5]);
+}
+
+// dice.cs.glsl:130-202: path search, point fetch and segment count.
+__device__ bool dice_setup(const BatchArgs &a, uint32_t seg, DiceSeg &d) {
+    if (a.path_count == 0) return false;
+    uint32_t lo = 0, hi = a.path_count;
+    int it = 0;
+    while (it < 1024 && lo + 1 < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        uint32_t mid_seg = a.dice_md[mid].z;
+        if (seg < mid_seg) {
+            hi = mid;
+        } else {
+            lo = mid;
+            if (seg == mid_seg) break;
+        }
+        it++;
+    }
+    d.path = lo;
+    uint4 md = a.dice_md[lo];
+    uint32_t gseg = seg - md.z + md.y;
+    if (gseg >= a.scene.index_count) return false;
+    uint2 ix = a.scene.indices[gseg];
+    uint32_t from = ix.x, flags = ix.y, to = from;
+    if (flags & kFlagCubic) to += 3;
+    else if (flags & kFlagQuad) to += 2;
+    else to += 1;
+    if (to >= a.scene.point_count) return false;
+    d.p0 = get_point(a, from);
+    d.p3 = get_point(a, to);
+    d.curve = (flags & (kFlagCubic | kFlagQuad)) != 0;
+    float countf;
+    if (d.curve) {
+        float2 k = get_point(a, from + 1);
+        if (flags & kFlagQuad) {
+            const float third = 1.0f / 3.0f;
+            float kx2 = k.x * 2.0f, ky2 = k.y * 2.0f;
+            d.c0 = make_float2((d.p0.x + kx2) * third, (d.p0.y + ky2) * third);
+            d.c1 = make_float2((d.p3.x + kx2) * third, (d.p3.y + ky2) * third);
+        } else {
+            d.c0 = k;
+            d.c1 = get_point(a, from + 2);
+        }
+        float ax = fabsf((d.c1.x - 2.0f * d.c0.x) + d.p0.x);
+        float ay = fabsf((d.c1.y - 2.0f * d.c0.y) + d.p0.y);
+        float bx = fabsf((d.p3.x - 2.0f * d.c1.x) + d.c0.x);
+        float by = fabsf((d.p3.y - 2.0f * d.c1.y) + d.c0.y);
+        float mx = 6.0f * max_glsl(ax, bx), my = 6.0f * max_glsl(ay, by);
+        float len = sqrtf(mx * mx + my * my);
+        countf = sqrtf(len / 2.0f);  // 8 * TOLERANCE
+    } else {
+        d.c0 = d.c1 = make_float2(0.0f, 0.0f);
+        float dx = d.p3.x - d.p0.x, dy = d.p3.y - d.p0.y;
+        countf = sqrtf(dx * dx + dy * dy) / 16.0f;  // MICROLINE_LENGTH
+    }
+    int32_t n = f2i_sat(ceilf(countf));
+    d.n = n < 1 ? 1 : n;
+    // A segment this long lies far outside the +-32768 px microline range.
+    return d.n <= (1 << 20);
+}
+
+__global__ void __launch_bounds__(kBlock) k_dice_count(BatchArgs a) {
+    for (uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x; seg < a.segment_count;
+         seg += gridDim.x * blockDim.x) {
+        DiceSeg d;
+        uint32_t n = 0;
+        if (dice_setup(a, seg, d)) {
+            n = (uint32_t)d.n;
+        } else {
+            atomicOr(&a.counters->error, 1u);
+        }
+        a.seg_offsets[seg] = n;
+    }
+}
+
+// dice.cs.glsl:76-90 emitMicroline.
+__device__ __forceinline__ uint4 pack_microline(float x0, float y0, float x1, float y1,
+                                                uint32_t path) {
+    float v[4] = {x0, y0, x1, y1};
+    int32_t px[4], fr[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int32_t sub = (int32_t)rintf(clamp_glsl(v[i], -32768.0f, 32767.0f) * 256.0f);
+        px[i] = (int32_t)floorf((float)sub / 256.0f);
+        fr[i] = sub - px[i] * 256;
+    }
+    return make_uint4(((uint32_t)px[0] & 0xffffu) | ((uint32_t)px[1] << 16),
+                      ((uint32_t)px[2] & 0xffffu) | ((uint32_t)px[3] << 16),
+                      (uint32_t)fr[0] | ((uint32_t)fr[1] << 8) | ((uint32_t)fr[2] << 16) |
+                          ((uint32_t)fr[3] << 24),
+                      path);
+}
+
+__global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
+    for (uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x; seg < a.segment_count;
+         seg += gridDim.x * blockDim.x) {
+        DiceSeg d;
+        if (!dice_setup(a, seg, d)) continue;
+        uint32_t first = a.seg_offsets[seg];
+        float2 prev = d.p0;
+        for (int32_t i = 0; i < d.n; i++) {
+            float t = (float)(i + 1) / (float)d.n;
+            float2 next;
+            if (d.curve) {
+                float2 p01 = make_float2(mixf(d.p0.x, d.c0.x, t), mixf(d.p0.y, d.c0.y, t));
+                float2 p12 = make_float2(mixf(d.c0.x, d.c1.x, t), mixf(d.c0.y, d.c1.y, t));
+                float2 p23 = make_float2(mixf(d.c1.x, d.p3.x, t), mixf(d.c1.y, d.p3.y, t));
+                float2 p012 = make_float2(mixf(p01.x, p12.x, t), mixf(p01.y, p12.y, t));
+                float2 p123 = make_float2(mixf(p12.x, p23.x, t), mixf(p12.y, p23.y, t));
+                next = make_float2(mixf(p012.x, p123.x, t), mixf(p012.y, p123.y, t));
+            } else {
+                next = make_float2(mixf(d.p0.x, d.p3.x, t), mixf(d.p0.y, d.p3.y, t));
+            }
+            uint32_t o = first + (uint32_t)i;
+            if (o < a.microline_capacity)
+                a.microlines[o] = pack_microline(prev.x, prev.y, next.x, next.y, d.path);
+            prev = next;
+        }
+    }
+}
+
+// Clamp a device-side total to the buffer capacity and flag overflow.
+__global__ void k_finalize_count(uint32_t *total, uint32_t cap, uint32_t *clamped,
+                                 uint32_t *overflow, uint32_t bit) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t t = *total;
+        *clamped = t < cap ? t : cap;
+        if (t > cap) atomicOr(overflow, bit);
+    }
+}
+
+// ------------------------------------------------------------------ bound
+
+// bound.cs.glsl:49-84.
+__global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < a.tile_count;
+         t += gridDim.x * blockDim.x) {
+        if (a.path_count == 0) break;
+        uint32_t lo = 0, hi = a.path_count;
+        int it = 0;
+        while (it < 1024 && lo + 1 < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            uint32_t mid_tile = a.tile_path_info[mid].z;
+            if (t < mid_tile) {
+                hi = mid;
+            } else {
+                lo = mid;
+                if (t == mid_tile) break;
+            }
+            it++;
+        }
+        a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, a.tile_path_info[lo].w);
+    }
+}
+
+// ------------------------------------------------------------------ bin
+
+struct IRect {
+    int32_t x0, y0, x1, y1;
+};
+
+// One fill of bin.cs.glsl:98-121 addFill: returns false when culled.
+__device__ __forceinline__ bool make_fill(float x0, float y0, float x1, float y1, int32_t tx,
+                                          int32_t ty, const IRect &r, uint32_t off,
+                                          uint32_t tile_count, uint32_t *tile, uint2 *words) {
+    if (tx < r.x0 || ty < r.y0 || tx >= r.x1 || ty >= r.y1) return false;
+    *tile = off + (uint32_t)(tx - r.x0) + (uint32_t)(ty - r.y0) * (uint32_t)(r.x1 - r.x0);
+    if (*tile >= tile_count) return false;  // metadata inconsistent with the tile count
+    float ox = (float)(tx * 16), oy = (float)(ty * 16);
+    uint32_t sx0 = f2u_sat((x0 - ox) * 256.0f), sy0 = f2u_sat((y0 - oy) * 256.0f);
+    uint32_t sx1 = f2u_sat((x1 - ox) * 256.0f), sy1 = f2u_sat((y1 - oy) * 256.0f);
+    if (sx0 == sx1) return false;
+    *words = make_uint2(sx0 | (sy0 << 16), sx1 | (sy1 << 16));
+    return true;
+}
+
+// bin.cs.glsl:154-256 for one microline. EMIT = false counts the fills;
+// EMIT = true writes them from `fill_base` on and applies backdrop deltas.
+template <bool EMIT>
+__device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_base) {
+    uint32_t path = ml.w;
+    int32_t w0 = (int32_t)ml.x, w1 = (int32_t)ml.y, w2 = (int32_t)ml.z;
+    float lx0 = (float)((int32_t)((uint32_t)w0 << 16) >> 16) + (float)(w2 & 0xff) / 256.0f;
+    float ly0 = (float)(w0 >> 16) + (float)((w2 >> 8) & 0xff) / 256.0f;
+    float lx1 = (float)((int32_t)((uint32_t)w1 << 16) >> 16) + (float)((w2 >> 16) & 0xff) / 256.0f;
+    float ly1 = (float)(w1 >> 16) + (float)((w2 >> 24) & 0xff) / 256.0f;
+    if (path >= a.path_count) return 0;
+    const int32_t *md = a.prop_md + (size_t)path * 12;
+    IRect r = {md[0], md[1], md[2], md[3]};
+    uint32_t off = (uint32_t)md[4], boff = (uint32_t)md[8];
+
+    int32_t ftx = (int32_t)floorf(lx0 / 16.0f), fty = (int32_t)floorf(ly0 / 16.0f);
+    int32_t ttx = (int32_t)floorf(lx1 / 16.0f), tty = (int32_t)floorf(ly1 / 16.0f);
+    float vx = lx1 - lx0, vy = ly1 - ly0;
+    int32_t sx = vx < 0.0f ? -1 : 1, sy = vy < 0.0f ? -1 : 1;
+    float fcx = (float)((ftx + (vx >= 0.0f ? 1 : 0)) * 16);
+    float fcy = (float)((fty + (vy >= 0.0f ? 1 : 0)) * 16);
+    float tmx = (fcx - lx0) / vx, tmy = (fcy - ly0) / vy;
+    float tdx = fabsf(16.0f / vx), tdy = fabsf(16.0f / vy);
+    float cx = lx0, cy = ly0;
+    int32_t tx = ftx, ty = fty;
+    int last = 0;
+    uint32_t count = 0;
+    for (uint32_t it = 0; it < 1024u; it++) {
+        int next;
+        if (tmx < tmy) next = 1;
+        else if (tmx > tmy) next = 2;
+        else if (sx > 0) next = 1;
+        else next = 2;
+        float nt = min_glsl(next == 1 ? tmx : tmy, 1.0f);
+        if (tx == ttx && ty == tty) next = 0;
+        float nx = mixf(lx0, lx1, nt), ny = mixf(ly0, ly1, nt);
+        uint32_t tile;
+        uint2 fw;
+        if (make_fill(cx, cy, nx, ny, tx, ty, r, off, a.tile_count, &tile, &fw)) {
+            if (EMIT) {
+                uint32_t f = fill_base + count;
+                if (f < a.fill_capacity) {
+                    a.fills[(size_t)f * 3 + 0] = fw.x;
+                    a.fills[(size_t)f * 3 + 1] = fw.y;
+                    a.fill_tiles[f] = tile;
+                }
+            }
+            count++;
+        }
+        bool aux = false;
+        float ax0 = 0, ay0 = 0, ax1 = 0, ay1 = 0;
+        if (sy < 0 && next == 2) {
+            aux = true;
+            ax0 = nx; ay0 = ny; ax1 = (float)(tx * 16); ay1 = (float)(ty * 16);
+        } else if (sy > 0 && last == 2) {
+            aux = true;
+            ax0 = (float)(tx * 16); ay0 = (float)(ty * 16); ax1 = cx; ay1 = cy;
+        }
+        if (aux && make_fill(ax0, ay0, ax1, ay1, tx, ty, r, off, a.tile_count, &tile, &fw)) {
+            if (EMIT) {
+                uint32_t f = fill_base + count;
+                if (f < a.fill_capacity) {
+                    a.fills[(size_t)f * 3 + 0] = fw.x;
+                    a.fills[(size_t)f * 3 + 1] = fw.y;
+                    a.fill_tiles[f] = tile;
+                }
+            }
+            count++;
+        }
+        if (EMIT) {
+            int32_t delta = 0;
+            if (sx < 0 && last == 1) delta = 1;
+            else if (sx > 0 && next == 1) delta = -1;
+            if (delta != 0) {
+                // bin.cs.glsl:123-143 adjustBackdrop
+                bool ox = tx < r.x0, oy = ty < r.y0, oz = tx >= r.x1, ow = ty >= r.y1;
+                if (ox || oy || oz || ow) {
+                    if (!ox && oy && !oz) {
+                        uint32_t bi = boff + (uint32_t)(tx - r.x0);
+                        if (bi < a.column_count) atomicAdd(&a.backdrops[(size_t)bi * 3], delta);
+                    }
+                } else {
+                    uint32_t t = off + (uint32_t)(tx - r.x0) +
+                                 (uint32_t)(ty - r.y0) * (uint32_t)(r.x1 - r.x0);
+                    if (t < a.tile_count) atomicAdd(&a.tiles[t].z, (uint32_t)delta << 24);
+                }
+            }
+        }
+        if (next == 1) {
+            tmx += tdx;
+            tx += sx;
+        } else if (next == 2) {
+            tmy += tdy;
+            ty += sy;
+        } else {
+            break;
+        }
+        cx = nx; cy = ny;
+        last = next;
+    }
+    return count;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_count(BatchArgs a) {
+    uint32_t n = a.counters->microline_count;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n;
+         m += gridDim.x * blockDim.x)
+        a.ml_fill_offsets[m] = bin_microline<false>(a, a.microlines[m], 0);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_emit(BatchArgs a) {
+    uint32_t n = a.counters->microline_count;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n;
+         m += gridDim.x * blockDim.x)
+        bin_microline<true>(a, a.microlines[m], a.ml_fill_offsets[m]);
+}
+
+// Per-tile fill lists (bin.cs.glsl:113-117): the reference pushes each fill
+// onto its tile's list with atomicExchange, so in sequential order fill f
+// links to the previous fill of the same tile and the tile heads the last
+// one. A path's fills are contiguous and only touch the path's own tiles, so
+// one workgroup per path sorts its fills by (tile, fill index) in LDS chunks
+// and stitches the runs; chunks of one path run in order, so the exchange on
+// the tile head carries a list across chunks.
+constexpr uint32_t kLinkChunk = 2048;
+
+__global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
+    __shared__ unsigned long long keys[kLinkChunk];
+    uint32_t path = blockIdx.x;
+    if (path >= a.path_count) return;
+    uint32_t mcount = a.counters->microline_count, fcount = a.counters->fill_count;
+    uint32_t s0 = a.dice_md[path].z;
+    uint32_t s1 = (path + 1 < a.path_count) ? a.dice_md[path + 1].z : a.segment_count;
+    if (s0 > a.segment_count) s0 = a.segment_count;
+    if (s1 > a.segment_count) s1 = a.segment_count;
+    if (s1 < s0) s1 = s0;
+    uint32_t m0 = min(a.seg_offsets[s0], mcount), m1 = min(a.seg_offsets[s1], mcount);
+    uint32_t f0 = min(a.ml_fill_offsets[m0], fcount), f1 = min(a.ml_fill_offsets[m1], fcount);
+    for (uint32_t base = f0; base < f1; base += kLinkChunk) {
+        uint32_t n = min(kLinkChunk, f1 - base);
+        uint32_t size = 64;
+        while (size < n) size <<= 1;
+        for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
+            keys[i] = i < n ? (((unsigned long long)a.fill_tiles[base + i] << 32) |
+                               (unsigned long long)(base + i))
+                            : ~0ull;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= size; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
+                    uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        unsigned long long x = keys[i], y = keys[ixj];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) {
+                            keys[i] = y;
+                            keys[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            unsigned long long cur = keys[i];
+            uint32_t tile = (uint32_t)(cur >> 32), f = (uint32_t)cur;
+            uint32_t link;
+            if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) {
+                uint32_t lo = i, hi = n;  // last index with this tile: upper bound - 1
+                while (lo + 1 < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if ((uint32_t)(keys[mid] >> 32) == tile) lo = mid;
+                    else hi = mid;
+                }
+                uint32_t last_f = (uint32_t)keys[lo];
+                link = atomicExch(&a.tiles[tile].y, last_f);
+            } else {
+                link = (uint32_t)keys[i - 1];
+            }
+            a.fills[(size_t)f * 3 + 2] = link;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ propagate
+
+// propagate.cs.glsl:103-218 for one column. EMIT = false counts the alpha
+// tiles the column allocates; EMIT = true writes tile words and alpha tiles
+// from `alpha_base` (batch-local) on. Linking and z-buffer writes are done by
+// k_sort_tiles.
+template <bool EMIT>
+__device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t alpha_base,
+                                     uint32_t first_alpha) {
+    int32_t backdrop = a.backdrops[(size_t)col * 3 + 0];
+    int32_t tile_x = a.backdrops[(size_t)col * 3 + 1];
+    uint32_t dpath = (uint32_t)a.backdrops[(size_t)col * 3 + 2];
+    if (dpath >= a.path_count) {
+        atomicOr(&a.counters->error, 2u);
+        return 0;
+    }
+    const int32_t *md = a.prop_md + (size_t)dpath * 12;
+    uint32_t rx0 = (uint32_t)md[0], ry0 = (uint32_t)md[1], rx1 = (uint32_t)md[2],
+             ry1 = (uint32_t)md[3];
+    uint32_t height = ry1 - ry0, width = rx1 - rx0;
+    uint32_t dofs = (uint32_t)md[4];
+    int32_t cpi = md[7];
+    uint32_t crx0 = 0, cry0 = 0, crx1 = 0, cry1 = 0, cofs = 0;
+    bool clip = cpi >= 0 && a.clip_prop_md != nullptr && (uint32_t)cpi < a.clip_path_count;
+    if (cpi >= 0 && !clip) atomicOr(&a.counters->error, 2u);
+    if (clip) {
+        const int32_t *cm = a.clip_prop_md + (size_t)cpi * 12;
+        crx0 = (uint32_t)cm[0]; cry0 = (uint32_t)cm[1];
+        crx1 = (uint32_t)cm[2]; cry1 = (uint32_t)cm[3];
+        cofs = (uint32_t)cm[4];
+    }
+    uint32_t count = 0;
+    for (uint32_t y = 0; y < height; y++) {
+        uint32_t t = dofs + y * width + (uint32_t)tile_x;
+        if (t >= a.tile_count) {
+            atomicOr(&a.counters->error, 4u);
+            break;
+        }
+        uint4 tile = a.tiles[t];
+        int32_t draw_alpha = -1, clip_alpha = -1;
+        int32_t first_fill = (int32_t)tile.y;
+        int32_t delta = (int32_t)tile.z >> 24;
+        uint32_t word = tile.w & 0x00ffffffu;
+        int32_t tile_backdrop = backdrop;
+        bool have_mask = first_fill >= 0;
+        bool need_alpha = have_mask;
+        if (clip) {
+            uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;  // unsigned, as the shader
+            if (ux >= crx0 && uy >= cry0 && ux < crx1 && uy < cry1) {
+                uint32_t ct = cofs + (uy - cry0) * (crx1 - crx0) + (ux - crx0);
+                uint4 ctile = ct < a.clip_tile_count ? a.clip_tiles[ct] : make_uint4(0, 0, 0, 0);
+                int32_t this_clip_alpha = sext24(ctile.z);
+                int32_t clip_backdrop = (int32_t)ctile.w >> 24;
+                if (this_clip_alpha >= 0) {
+                    if (have_mask) {
+                        clip_alpha = this_clip_alpha;
+                        need_alpha = true;
+                    } else if (tile_backdrop != 0) {
+                        draw_alpha = this_clip_alpha;
+                        clip_alpha = -1;
+                        need_alpha = false;
+                    } else {
+                        draw_alpha = -1;
+                        clip_alpha = -1;
+                        need_alpha = false;
+                    }
+                } else if (clip_backdrop == 0) {
+                    tile_backdrop = 0;
+                    need_alpha = false;
+                }
+            } else {
+                tile_backdrop = 0;
+                need_alpha = false;
+            }
+        }
+        if (need_alpha) {
+            if (EMIT) {
+                uint32_t ai = alpha_base + count;
+                a.alpha_tiles[ai] = make_uint2(t, (uint32_t)clip_alpha);
+                draw_alpha = (int32_t)(ai + first_alpha);
+            }
+            count++;
+        }
+        if (EMIT) {
+            uint2 zw = make_uint2(((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24),
+                                  word | ((uint32_t)tile_backdrop << 24));
+            *reinterpret_cast<uint2 *>(&a.tiles[t].z) = zw;
+        }
+        backdrop += delta;
+    }
+    return count;
+}
+
+__global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
+         c += gridDim.x * blockDim.x)
+        a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
+}
+
+// Allocates the batch's alpha tile range in the frame (uFirstAlphaTileIndex,
+// d3d11/renderer.rs:624-625, 643-646) and writes the z-buffer header the
+// reference leaves behind: bin uploads [6,0,0,0,0,microlines,0,0], bin bumps
+// [1], propagate_tiles zeroes the first tile_area words and propagate bumps [4].
+__global__ void k_propagate_finalize(BatchArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    BatchCounters *c = a.counters;
+    uint32_t count = a.col_alpha_offsets[a.column_count];
+    uint32_t start = a.frame->alpha_count;
+    c->alpha = count;
+    c->alpha_start = start;
+    uint32_t end = start + count;
+    a.frame->alpha_count = end;
+    c->frame_alpha_end = end;
+    if (end > a.mask_capacity) atomicOr(&c->overflow, 4u);
+    int32_t hdr[8] = {6, (int32_t)c->fills, 0, 0, 0, (int32_t)c->microlines, 0, 0};
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    for (uint32_t i = 0; i < 8; i++) a.zbuffer[i] = i < fbn ? 0 : hdr[i];
+    a.zbuffer[4] += (int32_t)count;
+}
+
+__global__ void __launch_bounds__(kBlock) k_propagate_emit(BatchArgs a) {
+    uint32_t first_alpha = a.counters->alpha_start;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
+         c += gridDim.x * blockDim.x)
+        propagate_column<true>(a, c, a.col_alpha_offsets[c], first_alpha);
+}
+
+// ------------------------------------------------------------------ sort
+
+// Lists the paths whose tile rect covers each framebuffer tile row, in path
+// order (one workgroup per row).
+__global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
+    __shared__ uint32_t lds4[4];
+    int32_t y = (int32_t)blockIdx.x;
+    if (y >= a.fb_height_tiles) return;
+    uint32_t carry = 0;
+    uint32_t *out = a.row_paths + (size_t)y * a.path_count;
+    for (uint32_t base = 0; base < a.path_count; base += kBlock) {
+        uint32_t p = base + threadIdx.x;
+        uint32_t hit = 0;
+        if (p < a.path_count) {
+            uint4 info = a.tile_path_info[p];
+            int32_t miny = (int32_t)(info.x) >> 16, maxy = (int32_t)(info.y) >> 16;
+            int32_t minx = (int32_t)(info.x << 16) >> 16, maxx = (int32_t)(info.y << 16) >> 16;
+            hit = (y >= miny && y < maxy && minx < maxx && maxx > 0 &&
+                   minx < a.fb_width_tiles) ? 1u : 0u;
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(hit, lds4, &tot);
+        if (hit) out[carry + ex] = p;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) a.row_counts[y] = carry;
+}
+
+// sort.cs.glsl:60-95 together with the linking half of propagate.cs.glsl:
+// 201-216. For framebuffer tile (x, y) the tiles linked into the list are the
+// covering paths' tiles with a nonzero backdrop or an alpha tile; the z value
+// is the largest index of a linked, z-writing, solid tile (atomicMax from 0).
+// In sequential order each link pushes onto the list head, so before the sort
+// a tile's next is the previous linked tile (the next lower index); the sort
+// keeps tiles with index >= z in ascending order and leaves culled tiles with
+// their pre-sort next. One wavefront walks the covering paths from the top
+// down, 64 at a time, and writes exactly those words.
+__global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (wave >= fbn) return;
+    uint32_t lane = lane_id();
+    int32_t x = (int32_t)(wave % (uint32_t)a.fb_width_tiles);
+    int32_t y = (int32_t)(wave / (uint32_t)a.fb_width_tiles);
+    const uint32_t *list = a.row_paths + (size_t)y * a.path_count;
+    uint32_t len = a.row_counts[y];
+
+    bool z_found = false;
+    int32_t z = 0;
+    int32_t succ = -1;        // lowest linked tile above the current chunk
+    int32_t pending = -1;     // culled tile whose next is in a lower chunk
+    int32_t lowest_survivor = -1;
+    for (uint32_t end = len; end > 0;) {
+        uint32_t begin = end > 64 ? end - 64 : 0;
+        uint32_t k = begin + lane;
+        bool valid = k < end;
+        int32_t t = -1;
+        bool linked = false, occl = false;
+        if (valid) {
+            uint32_t p = list[k];
+            uint4 info = a.tile_path_info[p];
+            int32_t miny = (int32_t)(info.x) >> 16;
+            int32_t minx = (int32_t)(info.x << 16) >> 16, maxx = (int32_t)(info.y << 16) >> 16;
+            uint32_t ti = info.z + (uint32_t)(y - miny) * (uint32_t)(maxx - minx) +
+                          (uint32_t)(x - minx);
+            if (x >= minx && x < maxx && ti < a.tile_count) {
+                t = (int32_t)ti;
+                uint4 w = a.tiles[t];
+                int32_t alpha = sext24(w.z);
+                int32_t bd = (int32_t)w.w >> 24;
+                linked = bd != 0 || alpha >= 0;
+                occl = linked && a.prop_md[(size_t)p * 12 + 6] != 0 && bd != 0 && alpha < 0;
+            }
+        }
+        unsigned long long lmask = __ballot(linked);
+        unsigned long long omask = __ballot(occl);
+        int32_t zlane;  // lanes >= zlane survive
+        if (z_found) {
+            zlane = 64;
+        } else if (omask) {
+            zlane = 63 - __clzll((long long)omask);
+            z = __shfl(t, zlane, 64);
+            z_found = true;
+        } else {
+            zlane = 0;
+        }
+        if (lmask) {
+            // the pending culled tile's next is this chunk's highest linked tile
+            int32_t hi_lane = 63 - __clzll((long long)lmask);
+            int32_t hi_t = __shfl(t, hi_lane, 64);
+            if (pending >= 0 && lane == 0) a.tiles[pending].x = (uint32_t)hi_t;
+            if (pending >= 0) pending = -1;
+            int32_t lo_lane = __ffsll((long long)lmask) - 1;
+            int32_t lo_t = __shfl(t, lo_lane, 64);
+            if (linked) {
+                unsigned long long above = lane == 63 ? 0ull : (lmask & (~0ull << (lane + 1)));
+                unsigned long long below = lmask & ((1ull << lane) - 1ull);
+                int32_t up_lane = above ? __ffsll((long long)above) - 1 : 64;
+                int32_t down_lane = below ? 63 - __clzll((long long)below) : -1;
+                int32_t up_t = __shfl(t, up_lane & 63, 64);
+                int32_t down_t = __shfl(t, down_lane & 63, 64);
+                if ((int32_t)lane >= zlane) {
+                    a.tiles[t].x = (uint32_t)(up_lane < 64 ? up_t : succ);
+                } else if (down_lane >= 0) {
+                    a.tiles[t].x = (uint32_t)down_t;
+                }
+            } else {
+                // keep the shuffles uniform across the wave
+                (void)__shfl(t, 0, 64);
+                (void)__shfl(t, 0, 64);
+            }
+            // lowest linked lane culled -> its next lies in a lower chunk
+            if (lo_lane < zlane) pending = lo_t;
+            unsigned long long surv = lmask & (zlane >= 64 ? 0ull : (~0ull << zlane));
+            if (surv) lowest_survivor = __shfl(t, __ffsll((long long)surv) - 1, 64);
+            succ = lo_t;
+        }
+        end = begin;
+    }
+    if (lane == 0) {
+        if (pending >= 0) a.tiles[pending].x = 0xffffffffu;
+        a.first_tile_map[wave] = lowest_survivor;
+        a.zbuffer[8 + wave] = z_found ? z : 0;
+    }
+}
+
+// ------------------------------------------------------------------ fill
+
+// texture(uAreaLUT, (u, v)) with GL_LINEAR + CLAMP_TO_EDGE and fp32 weights.
+__device__ __forceinline__ float4 sample_lut(const float4 *__restrict__ lut, float u, float v) {
+    float s = u * 256.0f - 0.5f, t = v * 256.0f - 0.5f;
+    float fs = floorf(s), ft = floorf(t);
+    float wa = s - fs, wb = t - ft;
+    int32_t i0 = f2i_sat(fs), j0 = f2i_sat(ft);
+    int32_t i1 = i0 < 2147483647 ? i0 + 1 : i0, j1 = j0 < 2147483647 ? j0 + 1 : j0;
+    i0 = min(max(i0, 0), 255);
+    i1 = min(max(i1, 0), 255);
+    j0 = min(max(j0, 0), 255);
+    j1 = min(max(j1, 0), 255);
+    float4 t00 = lut[j0 * 256 + i0], t10 = lut[j0 * 256 + i1];
+    float4 t01 = lut[j1 * 256 + i0], t11 = lut[j1 * 256 + i1];
+    float ia = 1.0f - wa, ib = 1.0f - wb;
+    float4 r;
+    r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
+    r.y = (t00.y * ia + t10.y * wa) * ib + (t01.y * ia + t11.y * wa) * wb;
+    r.z = (t00.z * ia + t10.z * wa) * ib + (t01.z * ia + t11.z * wa) * wb;
+    r.w = (t00.w * ia + t10.w * wa) * ib + (t01.w * ia + t11.w * wa) * wb;
+    return r;
+}
+
+// fill_area.inc.glsl:11-27 computeCoverage.
+__device__ __forceinline__ float4 compute_coverage(const float4 *__restrict__ lut, float fx,
+                                                   float fy, float tx, float ty) {
+    bool lt = fx < tx;
+    float lx = lt ? fx : tx, ly = lt ? fy : ty, rx = lt ? tx : fx, ry = lt ? ty : fy;
+    float wx = clamp_glsl(fx, -0.5f, 0.5f), wy = clamp_glsl(tx, -0.5f, 0.5f);
+    float offset = mixf(wx, wy, 0.5f) - lx;
+    float t = offset / (rx - lx);
+    float yy = mixf(ly, ry, t);
+    float d = (ry - ly) / (rx - lx);
+    float dX = wx - wy;
+    float4 s = sample_lut(lut, (yy + 8.0f) / 16.0f, fabsf(d * dX) / 16.0f);
+    return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
+}
+
+// fill.cs.glsl:61-88 + fill_compute.inc.glsl:11-25. One wavefront per alpha
+// tile; lane (lx, ly) owns the four pixels of column lx, rows 4ly..4ly+3,
+// which are the four channels of mask texel (lx, ly).
+__global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
+    uint32_t lane = lane_id();
+    uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    uint32_t n = a.counters->alpha;
+    uint32_t start = a.counters->alpha_start;
+    float fragx = (float)(lane & 15u) + 0.5f, fragy = (float)((lane >> 4) * 4u) + 0.5f;
+    for (uint32_t ba = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; ba < n; ba += waves) {
+        uint32_t alpha_index = start + ba;
+        if (alpha_index >= a.mask_capacity) break;
+        uint2 at = a.alpha_tiles[ba];
+        uint4 tile = a.tiles[at.x];
+        if (sext24(tile.z) < 0) continue;
+        int32_t f = (int32_t)tile.y;
+        int32_t backdrop = (int32_t)tile.w >> 24;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int it = 0;
+        do {
+            if ((uint32_t)f >= a.fill_capacity) break;
+            uint32_t fw0 = a.fills[(size_t)f * 3 + 0], fw1 = a.fills[(size_t)f * 3 + 1];
+            int32_t link = (int32_t)a.fills[(size_t)f * 3 + 2];
+            float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
+            float sy = (float)(fw0 >> 16) / 256.0f - fragy;
+            float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
+            float ey = (float)(fw1 >> 16) / 256.0f - fragy;
+            float4 c = compute_coverage(a.lut, sx, sy, ex, ey);
+            acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+            f = link;
+            it++;
+        } while (f >= 0 && it < 1024);
+        float bd = (float)backdrop;
+        float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
+        float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
+        float c2 = clamp_glsl(fabsf(bd + acc.z), 0.0f, 1.0f);
+        float c3 = clamp_glsl(fabsf(bd + acc.w), 0.0f, 1.0f);
+        int32_t clip = (int32_t)at.y;
+        if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
+            uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+            c0 = min_glsl(c0, (float)(ct & 0xffu) / 255.0f);
+            c1 = min_glsl(c1, (float)((ct >> 8) & 0xffu) / 255.0f);
+            c2 = min_glsl(c2, (float)((ct >> 16) & 0xffu) / 255.0f);
+            c3 = min_glsl(c3, (float)(ct >> 24) / 255.0f);
+        }
+        a.mask[(size_t)alpha_index * 64 + lane] =
+            unorm8(c0) | (unorm8(c1) << 8) | (unorm8(c2) << 16) | (unorm8(c3) << 24);
+    }
+}
+
+// ------------------------------------------------------------------ tile
+
+// tile.cs.glsl:71-163 with calculateColor/sampleMask (tile_fragment.inc.glsl:
+// 539-614) for solid colours and the Normal composite op. One wavefront per
+// framebuffer tile, lane (lx, ly) shades rows 4ly..4ly+3 of column lx.
+__global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (wave >= fbn) return;
+    uint32_t lane = lane_id(), lx = lane & 15u, ly = lane >> 4;
+    int32_t t = a.first_tile_map[wave];
+    if (t < 0 && a.load) return;
+    uint32_t gx = wave % (uint32_t)a.fb_width_tiles, gy = wave / (uint32_t)a.fb_width_tiles;
+    size_t pitch = (size_t)a.fb_width_tiles * 16;
+    uint32_t *px = a.framebuffer + (size_t)(gy * 16 + ly * 4) * pitch + gx * 16 + lx;
+    float d[4][4];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+        if (!a.load) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) d[sub][c] = a.clear[c];
+        } else {
+            uint32_t v = px[sub * pitch];
+#pragma unroll
+            for (int c = 0; c < 4; c++) d[sub][c] = (float)((v >> (8 * c)) & 0xffu) / 255.0f;
+        }
+    }
+    for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count;
+         guard++) {
+        uint4 w = a.tiles[t];
+        int32_t alpha = sext24(w.z);
+        uint32_t color = w.w & 0xffffu;
+        int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
+        float backdrop;
+        if (alpha >= 0) {
+            backdrop = 0.0f;
+        } else {
+            backdrop = (float)((int32_t)w.w >> 24);
+            ctrl &= ~0x3;
+        }
+        float4 base = color < a.paint_count ? a.paints[color] : make_float4(0, 0, 0, 0);
+        int32_t mask_ctrl = ctrl & 0x3;
+        uint32_t texel = 0;
+        if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
+            texel = a.mask[(size_t)(uint32_t)alpha * 64 + lane];
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            float mask_alpha = 1.0f;
+            if (mask_ctrl != 0) {
+                float cov = (float)((texel >> (8 * sub)) & 0xffu) / 255.0f + backdrop;
+                if (mask_ctrl & 0x1) {
+                    cov = fabsf(cov);
+                } else {
+                    float m = cov - 2.0f * floorf(cov / 2.0f);
+                    cov = 1.0f - fabsf(1.0f - m);
+                }
+                mask_alpha = min_glsl(mask_alpha, cov);
+            }
+            float al = base.w * mask_alpha;
+            float ia = 1.0f - al;
+            d[sub][0] = d[sub][0] * ia + base.x * al;
+            d[sub][1] = d[sub][1] * ia + base.y * al;
+            d[sub][2] = d[sub][2] * ia + base.z * al;
+            d[sub][3] = d[sub][3] * ia + al;
+        }
+        t = (int32_t)w.x;
+    }
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++)
+        px[sub * pitch] = unorm8(d[sub][0]) | (unorm8(d[sub][1]) << 8) |
+                          (unorm8(d[sub][2]) << 16) | (unorm8(d[sub][3]) << 24);
+}
+
+// ------------------------------------------------------------------ tables
+
+__global__ void k_lut_expand(const uint8_t *rgba, float4 *lut) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 256u * 256u) return;
+    lut[i] = make_float4((float)rgba[i * 4 + 0] / 255.0f, (float)rgba[i * 4 + 1] / 255.0f,
+                         (float)rgba[i * 4 + 2] / 255.0f, (float)rgba[i * 4 + 3] / 255.0f);
+}
+
+// ColorU::to_f32 (color/src/lib.rs:70-73) then f16::from_f32 (gpu/renderer.rs:724-727).
+__global__ void k_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float inv255 = 1.0f / 255.0f;
+    float c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        c[k] = __half2float(__float2half_rn((float)rgba[i * 4 + k] * inv255));
+    paints[i] = make_float4(c[0], c[1], c[2], c[3]);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+size_t scan_partials_words(uint32_t max_elements) {
+    return (size_t)(max_elements + kScanTile - 1) / kScanTile + 1;
+}
+
+void launch_frame_reset(FrameState *frame, hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, s, frame);
+}
+
+void launch_batch_init(const BatchArgs &a, hipStream_t s) {
+    uint64_t n = (uint64_t)a.fb_width_tiles * a.fb_height_tiles + 8;
+    if ((uint64_t)a.column_count * 3 > n) n = (uint64_t)a.column_count * 3;
+    hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a);
+}
+
+void launch_dice(const BatchArgs &a, hipStream_t s) {
+    uint32_t g = grid_for(a.segment_count, kBlock);
+    hipLaunchKernelGGL(k_dice_count, dim3(g), dim3(kBlock), 0, s, a);
+    scan_exclusive(a.seg_offsets, nullptr, a.segment_count, a.segment_count, a.scan_partials,
+                   &a.counters->microlines, s);
+    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->microlines,
+                       a.microline_capacity, &a.counters->microline_count, &a.counters->overflow,
+                       1u);
+    hipLaunchKernelGGL(k_dice_emit, dim3(g), dim3(kBlock), 0, s, a);
+}
+
+void launch_bound(const BatchArgs &a, hipStream_t s) {
+    if (a.tile_count == 0) return;
+    hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock)), dim3(kBlock), 0, s, a);
+}
+
+void launch_bin(const BatchArgs &a, hipStream_t s) {
+    uint32_t g = grid_for(a.microline_capacity, kBlock);
+    hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
+    scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
+                   a.scan_partials, &a.counters->fills, s);
+    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
+                       a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
+    hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);
+    if (a.path_count)
+        hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
+}
+
+void launch_propagate(const BatchArgs &a, hipStream_t s) {
+    uint32_t g = grid_for(a.column_count, kBlock);
+    if (a.column_count)
+        hipLaunchKernelGGL(k_propagate_count, dim3(g), dim3(kBlock), 0, s, a);
+    scan_exclusive(a.col_alpha_offsets, nullptr, a.column_count, a.column_count, a.scan_partials,
+                   nullptr, s);
+    hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
+    if (a.column_count)
+        hipLaunchKernelGGL(k_propagate_emit, dim3(g), dim3(kBlock), 0, s, a);
+}
+
+void launch_sort(const BatchArgs &a, hipStream_t s) {
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (fbn == 0) return;
+    hipLaunchKernelGGL(k_row_paths, dim3((uint32_t)a.fb_height_tiles), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_sort_tiles, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+}
+
+void launch_fill(const BatchArgs &a, hipStream_t s) {
+    if (a.tile_count == 0) return;
+    uint32_t g = grid_for((uint64_t)a.tile_count * 64, kBlock);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, s, a);
+}
+
+void launch_tile(const TileArgs &a, hipStream_t s) {
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (fbn == 0) return;
+    hipLaunchKernelGGL(k_tile, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+}
+
+void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s) {
+    hipLaunchKernelGGL(k_lut_expand, dim3(256), dim3(256), 0, s, rgba, lut);
+}
+
+void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_paints_expand, dim3((count + 255) / 256), dim3(256), 0, s, rgba, paints,
+                       count);
+}
+
+}  // namespace pfh

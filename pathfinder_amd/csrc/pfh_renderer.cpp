@@ -1,0 +1,755 @@
+// pfh_renderer.cpp - the renderer seam: RendererD3D11 + the RendererCore state
+// it uses, behind the C ABI of include/pfhip.h.
+//
+// Reference: renderer/src/gpu/d3d11/renderer.rs (RendererD3D11), the parts of
+// renderer/src/gpu/renderer.rs it calls (area LUT upload :207-222, texture
+// metadata :702-775, mask pages :1060-1122, begin/end scene :350-460).
+//
+// Differences by design:
+//  - no host readbacks inside a frame: counts stay on the device and every
+//    kernel reads them there; pfh_end_scene waits once, and if a microline,
+//    fill or mask buffer overflowed it grows it (next power of two, as
+//    d3d11/renderer.rs:227, 351 does) and renders the frame again;
+//  - device memory comes from a size-class pool owned by the context, so
+//    steady-state frames allocate nothing (GPUMemoryAllocator's role);
+//  - batches stay resident until the next pfh_begin_scene, so their
+//    buffers can be read back and the frame re-rendered without uploads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pfh_internal.h"
+#include "pfh_kernels.h"
+
+extern "C" const uint8_t pfh_area_lut_rgba[];
+
+namespace {
+
+uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p > 0x80000000ull ? 0x80000000u : (uint32_t)p;
+}
+
+// Size-class device memory pool (power-of-two classes from 256 bytes).
+class Pool {
+  public:
+    ~Pool() { release_all(); }
+    void *alloc(size_t bytes, hipError_t *err) {
+        size_t cls = 256;
+        while (cls < bytes) cls <<= 1;
+        auto &fl = free_[cls];
+        if (!fl.empty()) {
+            void *p = fl.back();
+            fl.pop_back();
+            *err = hipSuccess;
+            return p;
+        }
+        void *p = nullptr;
+        *err = hipMalloc(&p, cls);
+        if (*err != hipSuccess) {
+            // return cached blocks to the device and try once more
+            trim();
+            *err = hipMalloc(&p, cls);
+            if (*err != hipSuccess) return nullptr;
+        }
+        size_[p] = cls;
+        bytes_ += cls;
+        return p;
+    }
+    void free(void *p) {
+        if (!p) return;
+        auto it = size_.find(p);
+        if (it == size_.end()) return;
+        free_[it->second].push_back(p);
+    }
+    void trim() {
+        for (auto &kv : free_) {
+            for (void *p : kv.second) {
+                (void)hipFree(p);
+                bytes_ -= kv.first;
+                size_.erase(p);
+            }
+            kv.second.clear();
+        }
+    }
+    void release_all() {
+        for (auto &kv : size_) (void)hipFree(kv.first);
+        size_.clear();
+        free_.clear();
+        bytes_ = 0;
+    }
+    uint64_t bytes() const { return bytes_; }
+
+  private:
+    std::map<size_t, std::vector<void *>> free_;
+    std::map<void *, size_t> size_;
+    uint64_t bytes_ = 0;
+};
+
+enum { STAGE_DICE, STAGE_BOUND, STAGE_BIN, STAGE_PROPAGATE, STAGE_FILL, STAGE_SORT, STAGE_TILE,
+       STAGE_COUNT };
+
+struct Batch {
+    pfh::BatchData host;
+    bool draw = false;
+    int32_t load = 1;
+    float clear[4] = {0, 0, 0, 0};
+    // device inputs
+    pfh_dice_metadata *d_dice = nullptr;
+    pfh_tile_path_info *d_tpi = nullptr;
+    pfh_propagate_metadata *d_prop = nullptr;
+    pfh_backdrop_info *d_backdrops_init = nullptr;
+    // device working buffers
+    pfh_backdrop_info *d_backdrops = nullptr;
+    uint32_t *d_seg_offsets = nullptr;
+    uint4 *d_microlines = nullptr;
+    uint32_t *d_ml_fill_offsets = nullptr;
+    uint32_t *d_fills = nullptr;
+    uint32_t *d_fill_tiles = nullptr;
+    uint4 *d_tiles = nullptr;
+    uint32_t *d_col_alpha = nullptr;
+    uint2 *d_alpha = nullptr;
+    int32_t *d_zbuffer = nullptr;
+    int32_t *d_ftm = nullptr;
+    uint32_t *d_row_paths = nullptr;
+    uint32_t *d_row_counts = nullptr;
+    uint32_t *d_scan = nullptr;
+    pfh::BatchCounters *d_counters = nullptr;
+    uint32_t cap_m = 0, cap_f = 0;
+    pfh::BatchCounters counters = {};
+    hipEvent_t ev[STAGE_COUNT + 1] = {};
+    bool have_events = false;
+};
+
+}  // namespace
+
+struct pfh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    Pool pool;
+    bool timing = false;
+    // tables
+    float4 *d_lut = nullptr;
+    float4 *d_paints = nullptr;
+    uint32_t paint_count = 0;
+    std::vector<pfh_color_u> paints_host;
+    // scene sources
+    std::vector<pfh_vector2f> draw_points_host, clip_points_host;
+    std::vector<pfh_segment_indices> draw_idx_host, clip_idx_host;
+    float2 *d_draw_points = nullptr, *d_clip_points = nullptr;
+    uint2 *d_draw_idx = nullptr, *d_clip_idx = nullptr;
+    // frame
+    bool in_scene = false;
+    int32_t width = 0, height = 0, fbw = 0, fbh = 0;
+    bool has_background = false;
+    float background[4] = {0, 0, 0, 0};
+    bool dest_dirty = false;
+    uint32_t *d_fb = nullptr, *d_fb_saved = nullptr;
+    size_t fb_words = 0;
+    uint32_t *d_mask = nullptr;
+    uint32_t cap_a = 0;
+    pfh::FrameState *d_frame = nullptr;
+    uint32_t cap_m = 16384, cap_f = 16384;
+    std::vector<std::unique_ptr<Batch>> batches;
+    pfh_render_stats stats = {};
+    uint32_t replays = 0;
+};
+
+namespace {
+
+pfh_status fail(pfh_context *ctx, pfh_status s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->error = buf;
+    return s;
+}
+
+#define HIP_TRY(ctx, expr)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), PFH_ERROR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+pfh_status dalloc(pfh_context *ctx, T **out, size_t count) {
+    hipError_t e;
+    void *p = ctx->pool.alloc(std::max<size_t>(count, 1) * sizeof(T), &e);
+    if (!p) return fail(ctx, PFH_ERROR_OUT_OF_MEMORY, "device allocation of %zu bytes failed: %s",
+                        count * sizeof(T), hipGetErrorString(e));
+    *out = static_cast<T *>(p);
+    return PFH_OK;
+}
+
+template <class T>
+void dfree(pfh_context *ctx, T *&p) {
+    ctx->pool.free(p);
+    p = nullptr;
+}
+
+void free_batch(pfh_context *ctx, Batch &b) {
+    dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
+    dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
+    dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
+    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
+    dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_scan);
+    dfree(ctx, b.d_counters);
+    if (b.have_events)
+        for (auto &e : b.ev) (void)hipEventDestroy(e);
+    b.have_events = false;
+}
+
+void free_batches(pfh_context *ctx) {
+    for (auto &b : ctx->batches) free_batch(ctx, *b);
+    ctx->batches.clear();
+}
+
+template <class T>
+pfh_status upload(pfh_context *ctx, T **dst, const std::vector<T> &src) {
+    pfh_status s = dalloc(ctx, dst, src.size());
+    if (s != PFH_OK) return s;
+    if (!src.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    return PFH_OK;
+}
+
+// (Re)allocates the capacity-dependent buffers of a batch.
+pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
+    if (b.cap_m == ctx->cap_m && b.cap_f == ctx->cap_f && b.d_microlines) return PFH_OK;
+    dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets);
+    dfree(ctx, b.d_fills); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_scan);
+    b.cap_m = ctx->cap_m;
+    b.cap_f = ctx->cap_f;
+    pfh_status s;
+    if ((s = dalloc(ctx, &b.d_microlines, b.cap_m)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_ml_fill_offsets, (size_t)b.cap_m + 1)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_fills, (size_t)b.cap_f * 3)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_fill_tiles, b.cap_f)) != PFH_OK) return s;
+    uint32_t scan_n = std::max({b.host.segment_count, b.cap_m, (uint32_t)b.host.backdrops.size()});
+    if ((s = dalloc(ctx, &b.d_scan, pfh::scan_partials_words(scan_n))) != PFH_OK) return s;
+    return PFH_OK;
+}
+
+pfh_status ensure_mask(pfh_context *ctx, uint32_t tiles) {
+    if (ctx->d_mask && tiles <= ctx->cap_a) return PFH_OK;
+    uint32_t cap = std::max<uint32_t>(ctx->cap_a ? ctx->cap_a : 65536, next_pow2(tiles));
+    dfree(ctx, ctx->d_mask);
+    ctx->cap_a = cap;
+    return dalloc(ctx, &ctx->d_mask, (size_t)cap * 64);
+}
+
+Batch *find_batch(pfh_context *ctx, uint32_t id) {
+    for (auto &b : ctx->batches)
+        if (b->host.batch_id == id) return b.get();
+    return nullptr;
+}
+
+pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
+    pfh::BatchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    bool clip_src = b.host.path_source == PFH_PATH_SOURCE_CLIP;
+    a.scene.points = clip_src ? ctx->d_clip_points : ctx->d_draw_points;
+    a.scene.indices = clip_src ? ctx->d_clip_idx : ctx->d_draw_idx;
+    a.scene.point_count = (uint32_t)(clip_src ? ctx->clip_points_host.size()
+                                              : ctx->draw_points_host.size());
+    a.scene.index_count = (uint32_t)(clip_src ? ctx->clip_idx_host.size()
+                                              : ctx->draw_idx_host.size());
+    a.dice_md = reinterpret_cast<const uint4 *>(b.d_dice);
+    a.tile_path_info = reinterpret_cast<const uint4 *>(b.d_tpi);
+    a.prop_md = reinterpret_cast<const int32_t *>(b.d_prop);
+    a.backdrops_init = reinterpret_cast<const int32_t *>(b.d_backdrops_init);
+    a.path_count = b.host.path_count;
+    a.tile_count = b.host.tile_count;
+    a.segment_count = b.host.segment_count;
+    a.column_count = (uint32_t)b.host.backdrops.size();
+    const pfh_transform2f &t = b.host.transform;
+    a.xform[0] = t.m00; a.xform[1] = t.m01; a.xform[2] = t.m10; a.xform[3] = t.m11;
+    a.xform[4] = t.tx; a.xform[5] = t.ty;
+    if (b.host.clip_batch_id >= 0) {
+        Batch *c = find_batch(ctx, (uint32_t)b.host.clip_batch_id);
+        if (c) {
+            a.clip_prop_md = reinterpret_cast<const int32_t *>(c->d_prop);
+            a.clip_tiles = c->d_tiles;
+            a.clip_path_count = c->host.path_count;
+            a.clip_tile_count = c->host.tile_count;
+        }
+    }
+    a.backdrops = reinterpret_cast<int32_t *>(b.d_backdrops);
+    a.seg_offsets = b.d_seg_offsets;
+    a.microlines = b.d_microlines;
+    a.microline_capacity = b.cap_m;
+    a.ml_fill_offsets = b.d_ml_fill_offsets;
+    a.fills = b.d_fills;
+    a.fill_tiles = b.d_fill_tiles;
+    a.fill_capacity = b.cap_f;
+    a.tiles = b.d_tiles;
+    a.col_alpha_offsets = b.d_col_alpha;
+    a.alpha_tiles = b.d_alpha;
+    a.zbuffer = b.d_zbuffer;
+    a.first_tile_map = b.d_ftm;
+    a.row_paths = b.d_row_paths;
+    a.row_counts = b.d_row_counts;
+    a.scan_partials = b.d_scan;
+    a.counters = b.d_counters;
+    a.frame = ctx->d_frame;
+    a.mask = ctx->d_mask;
+    a.mask_capacity = ctx->cap_a;
+    a.lut = ctx->d_lut;
+    a.fb_width_tiles = ctx->fbw;
+    a.fb_height_tiles = ctx->fbh;
+    return a;
+}
+
+// prepare_tiles (d3d11/renderer.rs:427-542) and, for draw batches, draw_tiles
+// (:711-782), enqueued on the context stream.
+pfh_status run_batch(pfh_context *ctx, Batch &b) {
+    pfh::BatchArgs a = batch_args(ctx, b);
+    hipStream_t s = ctx->stream;
+    bool ev = ctx->timing && b.have_events;
+    if (ev) (void)hipEventRecord(b.ev[0], s);
+    pfh::launch_batch_init(a, s);
+    pfh::launch_dice(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_DICE], s);
+    pfh::launch_bound(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_BOUND], s);
+    pfh::launch_bin(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_BIN], s);
+    pfh::launch_propagate(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_PROPAGATE], s);
+    pfh::launch_fill(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_FILL], s);
+    pfh::launch_sort(a, s);
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_SORT], s);
+    if (b.draw) {
+        pfh::TileArgs t;
+        t.tiles = b.d_tiles;
+        t.tile_count = b.host.tile_count;
+        t.mask_capacity = ctx->cap_a;
+        t.first_tile_map = b.d_ftm;
+        t.mask = ctx->d_mask;
+        t.paints = ctx->d_paints;
+        t.paint_count = ctx->paint_count;
+        t.framebuffer = ctx->d_fb;
+        t.fb_width_tiles = ctx->fbw;
+        t.fb_height_tiles = ctx->fbh;
+        t.load = b.load;
+        for (int i = 0; i < 4; i++) t.clear[i] = b.clear[i];
+        pfh::launch_tile(t, s);
+    }
+    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_TILE], s);
+    HIP_TRY(ctx, hipGetLastError());
+    return PFH_OK;
+}
+
+pfh_status replay_frame(pfh_context *ctx) {
+    pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
+    if (ctx->d_fb_saved)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_fb, ctx->d_fb_saved, ctx->fb_words * 4,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    for (auto &b : ctx->batches) {
+        pfh_status s = size_capacity_buffers(ctx, *b);
+        if (s != PFH_OK) return s;
+        if ((s = run_batch(ctx, *b)) != PFH_OK) return s;
+    }
+    return PFH_OK;
+}
+
+pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
+    if (!ctx || !in) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "batch outside begin/end scene");
+    if (find_batch(ctx, in->batch_id))
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "batch %u already prepared", in->batch_id);
+    if (in->path_count && (!in->dice_metadata || !in->tile_path_info || !in->propagate_metadata))
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "batch %u: missing metadata", in->batch_id);
+    if (in->backdrop_count && !in->backdrops)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "batch %u: missing backdrops", in->batch_id);
+    if (in->clip_batch_id >= 0 && !find_batch(ctx, (uint32_t)in->clip_batch_id))
+        return fail(ctx, PFH_ERROR_STATE, "batch %u: clip batch %d not prepared", in->batch_id,
+                    in->clip_batch_id);
+    std::unique_ptr<Batch> bp(new (std::nothrow) Batch());
+    if (!bp) return fail(ctx, PFH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    Batch &b = *bp;
+    pfh::BatchData &h = b.host;
+    h.batch_id = in->batch_id;
+    h.path_source = in->path_source;
+    h.path_count = in->path_count;
+    h.tile_count = in->tile_count;
+    h.segment_count = in->segment_count;
+    h.transform = in->transform;
+    h.clip_batch_id = in->clip_batch_id;
+    h.dice_metadata.assign(in->dice_metadata, in->dice_metadata + in->path_count);
+    h.tile_path_info.assign(in->tile_path_info, in->tile_path_info + in->path_count);
+    h.propagate_metadata.assign(in->propagate_metadata, in->propagate_metadata + in->path_count);
+    h.backdrops.assign(in->backdrops, in->backdrops + in->backdrop_count);
+    b.draw = draw;
+    if (draw) {
+        // clear_color_for_draw_operation (gpu/renderer.rs:1190-1211)
+        b.load = (ctx->dest_dirty || !ctx->has_background) ? 1 : 0;
+        for (int i = 0; i < 4; i++) b.clear[i] = b.load ? 0.0f : ctx->background[i];
+        ctx->dest_dirty = true;
+    }
+    // The microline count is at least the segment count; start there.
+    if (h.segment_count > ctx->cap_m) ctx->cap_m = next_pow2(h.segment_count);
+    if (ctx->cap_f < ctx->cap_m) ctx->cap_f = ctx->cap_m;
+
+    size_t fbn = (size_t)ctx->fbw * ctx->fbh;
+    pfh_status s;
+    if ((s = upload(ctx, &b.d_dice, h.dice_metadata)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_tpi, h.tile_path_info)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_prop, h.propagate_metadata)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_backdrops_init, h.backdrops)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_backdrops, h.backdrops.size())) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_alpha, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_zbuffer, fbn + 8)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_ftm, fbn)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_row_paths, (size_t)ctx->fbh * std::max<uint32_t>(h.path_count, 1))) !=
+        PFH_OK)
+        return s;
+    if ((s = dalloc(ctx, &b.d_row_counts, (size_t)ctx->fbh)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_counters, 1)) != PFH_OK) return s;
+    if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
+    if (ctx->timing) {
+        for (auto &e : b.ev) HIP_TRY(ctx, hipEventCreate(&e));
+        b.have_events = true;
+    }
+    ctx->batches.push_back(std::move(bp));
+    return run_batch(ctx, b);
+}
+
+}  // namespace
+
+extern "C" {
+
+pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **out) {
+    if (!out) return PFH_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    pfh_context_options opt = {};
+    opt.level = PFH_RENDERER_LEVEL_D3D11;
+    if (options) opt = *options;
+    if (opt.level != PFH_RENDERER_LEVEL_D3D11) return PFH_ERROR_INVALID_ARGUMENT;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= opt.device || opt.device < 0)
+        return PFH_ERROR_NO_DEVICE;
+    std::unique_ptr<pfh_context> ctx(new (std::nothrow) pfh_context());
+    if (!ctx) return PFH_ERROR_OUT_OF_MEMORY;
+    ctx->device = opt.device;
+    ctx->timing = opt.enable_timing != 0;
+    if (opt.initial_microline_capacity) ctx->cap_m = next_pow2(opt.initial_microline_capacity);
+    if (opt.initial_fill_capacity) ctx->cap_f = next_pow2(opt.initial_fill_capacity);
+    if (hipSetDevice(opt.device) != hipSuccess) return PFH_ERROR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, opt.device) != hipSuccess) return PFH_ERROR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PFH_ERROR_NO_DEVICE;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return PFH_ERROR_HIP;
+    // Renderer::new uploads the area LUT (gpu/renderer.rs:207-222).
+    uint8_t *d_bytes = nullptr;
+    if (dalloc(ctx.get(), &d_bytes, 256 * 256 * 4) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    if (dalloc(ctx.get(), &ctx->d_lut, 256 * 256) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    if (dalloc(ctx.get(), &ctx->d_frame, 1) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    const uint8_t *lut = opt.area_lut_rgba ? opt.area_lut_rgba : pfh_area_lut_rgba;
+    if (hipMemcpy(d_bytes, lut, 256 * 256 * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return PFH_ERROR_HIP;
+    pfh::launch_lut_expand(d_bytes, ctx->d_lut, ctx->stream);
+    pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return PFH_ERROR_HIP;
+    dfree(ctx.get(), d_bytes);
+    *out = ctx.release();
+    return PFH_OK;
+}
+
+void pfh_context_destroy(pfh_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_batches(ctx);
+    ctx->pool.release_all();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *pfh_last_error(const pfh_context *ctx) { return ctx ? ctx->error.c_str() : ""; }
+
+pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
+    if (!ctx || !frame) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (frame->width <= 0 || frame->height <= 0 || frame->width > 32768 || frame->height > 32768)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad framebuffer size %dx%d", frame->width,
+                    frame->height);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    free_batches(ctx);
+    int32_t fbw = (frame->width + 15) / 16, fbh = (frame->height + 15) / 16;
+    size_t words = (size_t)fbw * fbh * 256;
+    if (!ctx->d_fb || words != ctx->fb_words) {
+        dfree(ctx, ctx->d_fb);
+        dfree(ctx, ctx->d_fb_saved);
+        pfh_status s = dalloc(ctx, &ctx->d_fb, words);
+        if (s != PFH_OK) return s;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_fb, 0, words * 4, ctx->stream));
+        ctx->fb_words = words;
+    }
+    ctx->width = frame->width;
+    ctx->height = frame->height;
+    ctx->fbw = fbw;
+    ctx->fbh = fbh;
+    ctx->has_background = frame->has_background != 0;
+    ctx->background[0] = frame->background_color.r;
+    ctx->background[1] = frame->background_color.g;
+    ctx->background[2] = frame->background_color.b;
+    ctx->background[3] = frame->background_color.a;
+    ctx->dest_dirty = false;
+    // A frame that starts by loading the destination must start from the same
+    // pixels when it is rendered again.
+    if (!ctx->has_background) {
+        if (!ctx->d_fb_saved) {
+            pfh_status s = dalloc(ctx, &ctx->d_fb_saved, words);
+            if (s != PFH_OK) return s;
+        }
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_fb_saved, ctx->d_fb, words * 4,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        dfree(ctx, ctx->d_fb_saved);
+    }
+    pfh_status s = ensure_mask(ctx, 0);
+    if (s != PFH_OK) return s;
+    pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    ctx->in_scene = true;
+    return PFH_OK;
+}
+
+pfh_status pfh_upload_texture_metadata(pfh_context *ctx, const pfh_color_u *colors,
+                                       uint32_t count) {
+    if (!ctx || (count && !colors)) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (count > 65536) return fail(ctx, PFH_ERROR_LIMIT, "more than 65536 paints (u16 PaintId)");
+    ctx->paints_host.assign(colors, colors + count);
+    dfree(ctx, ctx->d_paints);
+    pfh_status s = dalloc(ctx, &ctx->d_paints, count);
+    if (s != PFH_OK) return s;
+    uint8_t *d_bytes = nullptr;
+    std::vector<uint8_t> bytes(reinterpret_cast<const uint8_t *>(colors),
+                               reinterpret_cast<const uint8_t *>(colors) + 4 * (size_t)count);
+    if ((s = upload(ctx, &d_bytes, bytes)) != PFH_OK) return s;
+    pfh::launch_paints_expand(d_bytes, ctx->d_paints, count, ctx->stream);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d_bytes);
+    ctx->paint_count = count;
+    return PFH_OK;
+}
+
+pfh_status pfh_upload_scene(pfh_context *ctx, const pfh_segments *draw, const pfh_segments *clip) {
+    if (!ctx || !draw || !clip) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->draw_points_host.assign(draw->points, draw->points + draw->point_count);
+    ctx->draw_idx_host.assign(draw->indices, draw->indices + draw->index_count);
+    ctx->clip_points_host.assign(clip->points, clip->points + clip->point_count);
+    ctx->clip_idx_host.assign(clip->indices, clip->indices + clip->index_count);
+    dfree(ctx, ctx->d_draw_points); dfree(ctx, ctx->d_draw_idx);
+    dfree(ctx, ctx->d_clip_points); dfree(ctx, ctx->d_clip_idx);
+    pfh_status s;
+    if ((s = dalloc(ctx, &ctx->d_draw_points, draw->point_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &ctx->d_draw_idx, draw->index_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &ctx->d_clip_points, clip->point_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &ctx->d_clip_idx, clip->index_count)) != PFH_OK) return s;
+    if (draw->point_count)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_draw_points, ctx->draw_points_host.data(),
+                                    (size_t)draw->point_count * 8, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (draw->index_count)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_draw_idx, ctx->draw_idx_host.data(),
+                                    (size_t)draw->index_count * 8, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (clip->point_count)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_clip_points, ctx->clip_points_host.data(),
+                                    (size_t)clip->point_count * 8, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (clip->index_count)
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->d_clip_idx, ctx->clip_idx_host.data(),
+                                    (size_t)clip->index_count * 8, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    return PFH_OK;
+}
+
+pfh_status pfh_prepare_tiles(pfh_context *ctx, const pfh_tile_batch *batch) {
+    return add_batch(ctx, batch, false);
+}
+
+pfh_status pfh_prepare_and_draw_tiles(pfh_context *ctx, const pfh_tile_batch *batch) {
+    return add_batch(ctx, batch, true);
+}
+
+static pfh_status collect(pfh_context *ctx, bool *overflow) {
+    *overflow = false;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t max_m = 0, max_f = 0, alpha_end = 0;
+    for (auto &b : ctx->batches) {
+        HIP_TRY(ctx, hipMemcpy(&b->counters, b->d_counters, sizeof(pfh::BatchCounters),
+                               hipMemcpyDeviceToHost));
+        if (b->counters.error)
+            return fail(ctx, PFH_ERROR_INVALID_ARGUMENT,
+                        "batch %u: segment, point or path index out of range (code %u)",
+                        b->host.batch_id, b->counters.error);
+        max_m = std::max<uint64_t>(max_m, b->counters.microlines);
+        max_f = std::max<uint64_t>(max_f, b->counters.fills);
+        alpha_end = std::max<uint64_t>(alpha_end, b->counters.frame_alpha_end);
+        if (b->counters.overflow) *overflow = true;
+    }
+    if (alpha_end >= (1u << 23))
+        return fail(ctx, PFH_ERROR_LIMIT, "%llu alpha tiles exceed the 24-bit signed alpha tile id",
+                    (unsigned long long)alpha_end);
+    if (*overflow) {
+        if (max_m > ctx->cap_m) ctx->cap_m = next_pow2(max_m);
+        if (max_f > ctx->cap_f) ctx->cap_f = next_pow2(max_f);
+        pfh_status s = ensure_mask(ctx, (uint32_t)alpha_end);
+        if (s != PFH_OK) return s;
+    }
+    return PFH_OK;
+}
+
+pfh_status pfh_end_scene(pfh_context *ctx) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (!ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "end_scene without begin_scene");
+    ctx->in_scene = false;
+    bool overflow = false;
+    pfh_status s = collect(ctx, &overflow);
+    for (int round = 0; s == PFH_OK && overflow && round < 4; round++) {
+        ctx->replays++;
+        if ((s = replay_frame(ctx)) != PFH_OK) break;
+        s = collect(ctx, &overflow);
+    }
+    if (s != PFH_OK) return s;
+    if (overflow) return fail(ctx, PFH_ERROR_OUT_OF_MEMORY, "buffers kept overflowing");
+    pfh_render_stats &st = ctx->stats;
+    std::memset(&st, 0, sizeof(st));
+    for (auto &b : ctx->batches) {
+        st.path_count += b->host.path_count;
+        st.fill_count += b->counters.fills;
+        st.alpha_tile_count += b->counters.alpha;
+        st.total_tile_count += b->host.tile_count;
+        st.microline_count += b->counters.microlines;
+        st.segment_count += b->host.segment_count;
+        st.column_count += b->host.backdrops.size();
+        st.drawcall_count += b->draw ? 7 : 6;
+        if (b->have_events) {
+            float ms[STAGE_COUNT];
+            for (int i = 0; i < STAGE_COUNT; i++) {
+                ms[i] = 0;
+                (void)hipEventElapsedTime(&ms[i], b->ev[i], b->ev[i + 1]);
+            }
+            st.ms_dice += ms[STAGE_DICE];
+            st.ms_bound += ms[STAGE_BOUND];
+            st.ms_bin += ms[STAGE_BIN];
+            st.ms_propagate += ms[STAGE_PROPAGATE];
+            st.ms_fill += ms[STAGE_FILL];
+            st.ms_sort += ms[STAGE_SORT];
+            st.ms_tile += ms[STAGE_TILE];
+            float f = 0;
+            (void)hipEventElapsedTime(&f, b->ev[0], b->ev[STAGE_COUNT]);
+            st.ms_frame += f;
+        }
+    }
+    st.gpu_bytes_allocated = ctx->pool.bytes();
+    st.overflow_replays = ctx->replays;
+    return PFH_OK;
+}
+
+pfh_status pfh_rerender_scene(pfh_context *ctx) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "rerender inside a scene");
+    return replay_frame(ctx);
+}
+
+pfh_status pfh_synchronize(pfh_context *ctx) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PFH_OK;
+}
+
+pfh_status pfh_read_framebuffer(pfh_context *ctx, uint8_t *rgba, size_t stride) {
+    if (!ctx || !rgba) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->d_fb) return fail(ctx, PFH_ERROR_STATE, "no framebuffer yet");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (stride == 0) stride = (size_t)ctx->width * 4;
+    HIP_TRY(ctx, hipMemcpy2D(rgba, stride, ctx->d_fb, (size_t)ctx->fbw * 16 * 4,
+                             (size_t)ctx->width * 4, (size_t)ctx->height, hipMemcpyDeviceToHost));
+    return PFH_OK;
+}
+
+pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind kind, void *dst,
+                          size_t capacity, size_t *out_len) {
+    if (!ctx || !out_len) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    Batch *b = find_batch(ctx, batch_id);
+    if (!b) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "no batch %u", batch_id);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(&b->counters, b->d_counters, sizeof(pfh::BatchCounters),
+                           hipMemcpyDeviceToHost));
+    size_t fbn = (size_t)ctx->fbw * ctx->fbh;
+    const void *src = nullptr;
+    size_t bytes = 0;
+    uint32_t range[2] = {b->counters.alpha_start, b->counters.alpha_start + b->counters.alpha};
+    switch (kind) {
+    case PFH_DEBUG_MICROLINES: src = b->d_microlines; bytes = (size_t)b->counters.microline_count * 16; break;
+    case PFH_DEBUG_FILLS: src = b->d_fills; bytes = (size_t)b->counters.fill_count * 12; break;
+    case PFH_DEBUG_TILES: src = b->d_tiles; bytes = (size_t)b->host.tile_count * 16; break;
+    case PFH_DEBUG_Z_BUFFER: src = b->d_zbuffer; bytes = (fbn + 8) * 4; break;
+    case PFH_DEBUG_FIRST_TILE_MAP: src = b->d_ftm; bytes = fbn * 4; break;
+    case PFH_DEBUG_ALPHA_TILES: src = b->d_alpha; bytes = (size_t)b->counters.alpha * 8; break;
+    case PFH_DEBUG_BACKDROPS: src = b->d_backdrops; bytes = b->host.backdrops.size() * 12; break;
+    case PFH_DEBUG_ALPHA_RANGE: bytes = 8; break;
+    default: return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad debug kind %d", (int)kind);
+    }
+    *out_len = bytes;
+    if (!dst) return PFH_OK;  // size query
+    if (capacity < bytes) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "buffer too small");
+    if (kind == PFH_DEBUG_ALPHA_RANGE) {
+        std::memcpy(dst, range, 8);
+    } else if (bytes) {
+        HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
+    return PFH_OK;
+}
+
+pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
+                         size_t *out_words) {
+    if (!ctx || !out_words) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t alpha = 0;
+    for (auto &b : ctx->batches) {
+        HIP_TRY(ctx, hipMemcpy(&b->counters, b->d_counters, sizeof(pfh::BatchCounters),
+                               hipMemcpyDeviceToHost));
+        alpha = std::max(alpha, b->counters.frame_alpha_end);
+    }
+    alpha = std::min(alpha, ctx->cap_a);
+    *out_words = (size_t)alpha * 64;
+    if (!dst) return PFH_OK;
+    if (capacity_words < *out_words)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "buffer too small");
+    if (alpha) HIP_TRY(ctx, hipMemcpy(dst, ctx->d_mask, *out_words * 4, hipMemcpyDeviceToHost));
+    return PFH_OK;
+}
+
+pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats) {
+    if (!ctx || !stats) return PFH_ERROR_INVALID_ARGUMENT;
+    *stats = ctx->stats;
+    return PFH_OK;
+}
+
+}  // extern "C"

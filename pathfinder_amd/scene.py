@@ -1,0 +1,580 @@
+"""Scene / SceneBuilder / Renderer API over libpfhip.so.
+
+Mirrors the reference's front-end types for the D3D11-level path:
+  Scene, DrawPath, ClipPath, Paint          renderer/src/scene.rs, paint.rs
+  Outline, Contour, PointFlags, FillRule    content/src/outline.rs, fill.rs
+  BuildOptions, RenderTransform             renderer/src/options.rs:53-90
+  RendererOptions, RendererMode, DestFramebuffer, RendererLevel
+                                            renderer/src/gpu/options.rs
+  Renderer (begin_scene / render_command / end_scene)
+                                            renderer/src/gpu/renderer.rs:350-460
+Building (SceneBuilder::build) and rendering run in the native library.
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import _lib as L
+
+
+class FillRule(enum.IntEnum):
+    WINDING = L.FILL_RULE_WINDING
+    EVEN_ODD = L.FILL_RULE_EVEN_ODD
+
+
+class BlendMode(enum.IntEnum):
+    SRC_OVER = 0
+
+
+class RendererLevel(enum.IntEnum):
+    D3D11 = L.RENDERER_LEVEL_D3D11
+
+
+class ColorU:
+    __slots__ = ("r", "g", "b", "a")
+
+    def __init__(self, r, g, b, a=255):
+        self.r, self.g, self.b, self.a = int(r), int(g), int(b), int(a)
+
+    def is_opaque(self):
+        return self.a == 255
+
+    def __iter__(self):
+        return iter((self.r, self.g, self.b, self.a))
+
+    def __repr__(self):
+        return "ColorU(%d, %d, %d, %d)" % (self.r, self.g, self.b, self.a)
+
+
+class ColorF:
+    __slots__ = ("r", "g", "b", "a")
+
+    def __init__(self, r, g, b, a=1.0):
+        self.r, self.g, self.b, self.a = float(r), float(g), float(b), float(a)
+
+
+class Paint:
+    """A solid paint (Paint::from_color, paint.rs)."""
+
+    def __init__(self, color):
+        self.color = color
+
+    @staticmethod
+    def from_color(color):
+        return Paint(color)
+
+
+class RectF:
+    def __init__(self, x0, y0, x1, y1):
+        self.x0, self.y0, self.x1, self.y1 = float(x0), float(y0), float(x1), float(y1)
+
+    @staticmethod
+    def from_size(w, h):
+        return RectF(0.0, 0.0, w, h)
+
+
+class Transform2F:
+    """Row-major 2x2 matrix plus translation (PFTransform2F, c/src/lib.rs:167-177)."""
+
+    def __init__(self, m00=1.0, m01=0.0, m10=0.0, m11=1.0, tx=0.0, ty=0.0):
+        self.m = (float(m00), float(m01), float(m10), float(m11), float(tx), float(ty))
+
+    @staticmethod
+    def identity():
+        return Transform2F()
+
+    @staticmethod
+    def from_scale(sx, sy=None):
+        return Transform2F(sx, 0.0, 0.0, sx if sy is None else sy)
+
+    @staticmethod
+    def from_translation(tx, ty):
+        return Transform2F(tx=tx, ty=ty)
+
+    def to_c(self):
+        return L.Transform2F(*self.m)
+
+
+class BuildOptions:
+    def __init__(self, transform=None):
+        self.transform = transform or Transform2F()
+
+
+class Contour:
+    """A subpath: points with PointFlags (content/src/outline.rs:34-60)."""
+
+    def __init__(self):
+        self.points = []
+        self.flags = []
+
+    def push_endpoint(self, x, y):
+        self.points.append((x, y))
+        self.flags.append(0)
+
+    def push_quadratic(self, cx, cy, x, y):
+        self.points.append((cx, cy))
+        self.flags.append(L.POINT_CONTROL_0)
+        self.push_endpoint(x, y)
+
+    def push_cubic(self, c0x, c0y, c1x, c1y, x, y):
+        self.points.append((c0x, c0y))
+        self.flags.append(L.POINT_CONTROL_0)
+        self.points.append((c1x, c1y))
+        self.flags.append(L.POINT_CONTROL_1)
+        self.push_endpoint(x, y)
+
+    def close(self):
+        # Contours are always closed on this path (SegmentsD3D11::add_path
+        # appends the first point, builder.rs:835).
+        pass
+
+
+class Outline:
+    """Flat arrays: points (N, 2) f32, flags (N,) u8, contour_ends (K,) u32."""
+
+    def __init__(self, points=None, flags=None, contour_ends=None):
+        self.points = np.zeros((0, 2), np.float32) if points is None else \
+            np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+        self.flags = np.zeros(len(self.points), np.uint8) if flags is None else \
+            np.ascontiguousarray(flags, np.uint8)
+        self.contour_ends = np.zeros(0, np.uint32) if contour_ends is None else \
+            np.ascontiguousarray(contour_ends, np.uint32)
+
+    def push_contour(self, contour):
+        if not contour.points:
+            return
+        pts = np.asarray(contour.points, np.float32).reshape(-1, 2)
+        self.points = np.concatenate([self.points, pts])
+        self.flags = np.concatenate([self.flags, np.asarray(contour.flags, np.uint8)])
+        self.contour_ends = np.concatenate(
+            [self.contour_ends, np.asarray([len(self.points)], np.uint32)])
+
+    def bounds(self):
+        if len(self.points) == 0:
+            return RectF(0, 0, 0, 0)
+        lo = self.points.min(axis=0)
+        hi = self.points.max(axis=0)
+        return RectF(lo[0], lo[1], hi[0], hi[1])
+
+
+class DrawPath:
+    def __init__(self, outline, paint):
+        self.outline = outline
+        self.paint = int(paint)
+        self.clip_path = -1
+        self.fill_rule = FillRule.WINDING
+        self.blend_mode = BlendMode.SRC_OVER
+
+    def set_clip_path(self, clip_path):
+        self.clip_path = -1 if clip_path is None else int(clip_path)
+
+    def set_fill_rule(self, rule):
+        self.fill_rule = FillRule(rule)
+
+    def set_blend_mode(self, mode):
+        self.blend_mode = BlendMode(mode)
+
+
+class ClipPath:
+    def __init__(self, outline):
+        self.outline = outline
+        self.clip_path = -1
+        self.fill_rule = FillRule.WINDING
+
+    def set_clip_path(self, clip_path):
+        self.clip_path = -1 if clip_path is None else int(clip_path)
+
+    def set_fill_rule(self, rule):
+        self.fill_rule = FillRule(rule)
+
+
+class PathArrays:
+    """Many paths in flat arrays (the bulk form of pfh_scene_push_*_paths)."""
+
+    def __init__(self, points, flags, contour_ends, path_contour_ends, paints=None,
+                 fill_rules=None, clip_paths=None):
+        self.points = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+        self.flags = np.ascontiguousarray(flags, np.uint8)
+        self.contour_ends = np.ascontiguousarray(contour_ends, np.uint32)
+        self.path_contour_ends = np.ascontiguousarray(path_contour_ends, np.uint32)
+        n = len(self.path_contour_ends)
+        self.paints = np.zeros(n, np.uint32) if paints is None else \
+            np.ascontiguousarray(paints, np.uint32)
+        self.fill_rules = np.zeros(n, np.int32) if fill_rules is None else \
+            np.ascontiguousarray(fill_rules, np.int32)
+        self.clip_paths = np.full(n, -1, np.int32) if clip_paths is None else \
+            np.ascontiguousarray(clip_paths, np.int32)
+
+    def __len__(self):
+        return len(self.path_contour_ends)
+
+    def path(self, i):
+        """(points, flags, contour_ends relative to the path) of path i."""
+        c0 = int(self.path_contour_ends[i - 1]) if i > 0 else 0
+        c1 = int(self.path_contour_ends[i])
+        p0 = int(self.contour_ends[c0 - 1]) if c0 > 0 else 0
+        p1 = int(self.contour_ends[c1 - 1]) if c1 > 0 else p0
+        return (self.points[p0:p1], self.flags[p0:p1],
+                self.contour_ends[c0:c1].astype(np.int64) - p0)
+
+    @staticmethod
+    def concat(parts):
+        pts, fl, ce, pce, pa, fr, cp = [], [], [], [], [], [], []
+        npts = ncont = 0
+        for a in parts:
+            pts.append(a.points)
+            fl.append(a.flags)
+            ce.append(a.contour_ends.astype(np.int64) + npts)
+            pce.append(a.path_contour_ends.astype(np.int64) + ncont)
+            pa.append(a.paints)
+            fr.append(a.fill_rules)
+            cp.append(a.clip_paths)
+            npts += len(a.points)
+            ncont += len(a.contour_ends)
+        cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
+        return PathArrays(np.concatenate(pts) if pts else np.zeros((0, 2), np.float32),
+                          cat(fl, np.uint8), cat(ce, np.uint32), cat(pce, np.uint32),
+                          cat(pa, np.uint32), cat(fr, np.int32), cat(cp, np.int32))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+class Scene:
+    """Scene (renderer/src/scene.rs:39-378) restricted to solid paints."""
+
+    def __init__(self):
+        self.view_box = RectF(0, 0, 0, 0)
+        self.paints = []
+        self._paint_cache = {}
+        self._draw_parts = []
+        self._clip_parts = []
+
+    # Scene::set_view_box
+    def set_view_box(self, rect):
+        self.view_box = rect
+
+    # Scene::push_paint + Palette dedup (paint.rs:381-390)
+    def push_paint(self, paint):
+        c = paint.color if isinstance(paint, Paint) else paint
+        key = (c.r, c.g, c.b, c.a)
+        if key in self._paint_cache:
+            return self._paint_cache[key]
+        pid = len(self.paints)
+        if pid >= 65536:
+            raise ValueError("PaintId is 16 bits (paint.rs:81)")
+        self.paints.append(ColorU(*key))
+        self._paint_cache[key] = pid
+        return pid
+
+    def _single(self, outline, paint, fill_rule, clip_path):
+        return PathArrays(outline.points, outline.flags, outline.contour_ends,
+                          np.asarray([len(outline.contour_ends)], np.uint32),
+                          np.asarray([paint], np.uint32), np.asarray([int(fill_rule)], np.int32),
+                          np.asarray([clip_path], np.int32))
+
+    def draw_path_count(self):
+        return sum(len(p) for p in self._draw_parts)
+
+    def clip_path_count(self):
+        return sum(len(p) for p in self._clip_parts)
+
+    def push_draw_path(self, draw_path):
+        pid = self.draw_path_count()
+        self._draw_parts.append(self._single(draw_path.outline, draw_path.paint,
+                                             draw_path.fill_rule, draw_path.clip_path))
+        return pid
+
+    def push_clip_path(self, clip_path):
+        pid = self.clip_path_count()
+        self._clip_parts.append(self._single(clip_path.outline, 0, clip_path.fill_rule,
+                                             clip_path.clip_path))
+        return pid
+
+    def push_draw_paths(self, arrays):
+        pid = self.draw_path_count()
+        self._draw_parts.append(arrays)
+        return pid
+
+    def push_clip_paths(self, arrays):
+        pid = self.clip_path_count()
+        self._clip_parts.append(arrays)
+        return pid
+
+    def draw_arrays(self):
+        return PathArrays.concat(self._draw_parts)
+
+    def clip_arrays(self):
+        return PathArrays.concat(self._clip_parts)
+
+    def _native(self):
+        lib = L.load()
+        h = lib.pfh_scene_create()
+        if not h:
+            raise MemoryError("pfh_scene_create")
+        vb = L.RectF(L.Vector2F(self.view_box.x0, self.view_box.y0),
+                     L.Vector2F(self.view_box.x1, self.view_box.y1))
+        lib.pfh_scene_set_view_box(h, C.byref(vb))
+        for c in self.paints:
+            lib.pfh_scene_push_paint(h, C.byref(L.ColorU(c.r, c.g, c.b, c.a)))
+        for a in self._clip_parts:
+            lib.pfh_scene_push_clip_paths(h, _ptr(a.points), _ptr(a.flags), len(a.points),
+                                          _ptr(a.contour_ends), len(a.contour_ends),
+                                          _ptr(a.path_contour_ends), len(a),
+                                          _ptr(a.fill_rules), _ptr(a.clip_paths))
+        for a in self._draw_parts:
+            lib.pfh_scene_push_draw_paths(h, _ptr(a.points), _ptr(a.flags), len(a.points),
+                                          _ptr(a.contour_ends), len(a.contour_ends),
+                                          _ptr(a.path_contour_ends), len(a), _ptr(a.paints),
+                                          _ptr(a.fill_rules), _ptr(a.clip_paths), None)
+        return h
+
+    def build(self, options=None):
+        """SceneBuilder::build (builder.rs:148-222) -> CommandList."""
+        lib = L.load()
+        h = self._native()
+        try:
+            bo = L.BuildOptions((options or BuildOptions()).transform.to_c())
+            out = C.c_void_p()
+            L.check(lib.pfh_scene_build(h, C.byref(bo), C.byref(out)))
+            return CommandList(out)
+        finally:
+            lib.pfh_scene_destroy(h)
+
+    def build_and_render(self, renderer, options=None):
+        """Scene::build_and_render (scene.rs:369-378)."""
+        commands = self.build(options)
+        renderer.begin_scene()
+        renderer.render_commands(commands)
+        renderer.end_scene()
+        return commands
+
+
+class CommandList:
+    """The RenderCommand stream of one build (gpu_data.rs:37-105)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = L.load()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.pfh_command_list_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return self._lib.pfh_command_list_len(self._h)
+
+    def kinds(self):
+        return [self._lib.pfh_command_kind_at(self._h, i) for i in range(len(self))]
+
+    def batches(self):
+        """[(kind, dict of numpy arrays)] for every batch command."""
+        out = []
+        for i, k in enumerate(self.kinds()):
+            if k in (L.COMMAND_PREPARE_CLIP_TILES_D3D11, L.COMMAND_DRAW_TILES_D3D11):
+                tb = L.TileBatch()
+                L.check(self._lib.pfh_command_batch(self._h, i, C.byref(tb)))
+                out.append((k, batch_to_numpy(tb)))
+        return out
+
+    def segments(self):
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_UPLOAD_SCENE_D3D11:
+                d, c = L.Segments(), L.Segments()
+                L.check(self._lib.pfh_command_segments(self._h, i, C.byref(d), C.byref(c)))
+                return segments_to_numpy(d), segments_to_numpy(c)
+        return None
+
+    def paints(self):
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_UPLOAD_TEXTURE_METADATA:
+                ptr, n = C.c_void_p(), C.c_uint32()
+                L.check(self._lib.pfh_command_paints(self._h, i, C.byref(ptr), C.byref(n)))
+                return _copy(ptr.value, n.value * 4, np.uint8).reshape(-1, 4)
+        return np.zeros((0, 4), np.uint8)
+
+
+def _copy(addr, nbytes, dtype):
+    if not nbytes:
+        return np.zeros(0, dtype)
+    buf = (C.c_uint8 * nbytes).from_address(addr)
+    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+
+def segments_to_numpy(s):
+    return {"points": _copy(s.points, s.point_count * 8, np.float32).reshape(-1, 2),
+            "indices": _copy(s.indices, s.index_count * 8, np.uint32).reshape(-1, 2)}
+
+
+def batch_to_numpy(tb):
+    t = tb.transform
+    return {
+        "batch_id": tb.batch_id, "path_source": tb.path_source, "path_count": tb.path_count,
+        "tile_count": tb.tile_count, "segment_count": tb.segment_count,
+        "transform": np.asarray([t.m00, t.m01, t.m10, t.m11, t.tx, t.ty], np.float32),
+        "dice_metadata": _copy(tb.dice_metadata, tb.path_count * 16, np.uint32).reshape(-1, 4),
+        "tile_path_info": _copy(tb.tile_path_info, tb.path_count * 16, np.uint32).reshape(-1, 4),
+        "propagate_metadata": _copy(tb.propagate_metadata, tb.path_count * 48,
+                                    np.int32).reshape(-1, 12),
+        "backdrops": _copy(tb.backdrops, tb.backdrop_count * 12, np.int32).reshape(-1, 3),
+        "clip_batch_id": tb.clip_batch_id,
+    }
+
+
+def numpy_to_batch(b):
+    """Builds a TileBatch whose pointers reference the arrays of dict b."""
+    keep = {k: np.ascontiguousarray(b[k]) for k in
+            ("dice_metadata", "tile_path_info", "propagate_metadata", "backdrops")}
+    tb = L.TileBatch()
+    tb.batch_id = b["batch_id"]
+    tb.path_source = b["path_source"]
+    tb.path_count = b["path_count"]
+    tb.tile_count = b["tile_count"]
+    tb.segment_count = b["segment_count"]
+    tb.transform = L.Transform2F(*[float(v) for v in b["transform"]])
+    tb.dice_metadata = _ptr(keep["dice_metadata"])
+    tb.tile_path_info = _ptr(keep["tile_path_info"])
+    tb.propagate_metadata = _ptr(keep["propagate_metadata"])
+    tb.backdrops = _ptr(keep["backdrops"])
+    tb.backdrop_count = len(keep["backdrops"])
+    tb.clip_batch_id = b["clip_batch_id"]
+    return tb, keep
+
+
+class DestFramebuffer:
+    def __init__(self, width, height):
+        self.width, self.height = int(width), int(height)
+
+    @staticmethod
+    def full_window(width, height):
+        return DestFramebuffer(width, height)
+
+
+class RendererMode:
+    def __init__(self, level=RendererLevel.D3D11):
+        self.level = RendererLevel(level)
+
+
+class RendererOptions:
+    def __init__(self, dest, background_color=None, show_debug_ui=False):
+        self.dest = dest
+        self.background_color = background_color
+        self.show_debug_ui = show_debug_ui
+
+
+class Renderer:
+    """Renderer (gpu/renderer.rs) at the D3D11 level, on one HIP device."""
+
+    def __init__(self, options, mode=None, device=0, timing=None, area_lut=None,
+                 initial_microline_capacity=0, initial_fill_capacity=0):
+        self._lib = L.load()
+        self.options = options
+        self.mode = mode or RendererMode()
+        o = L.ContextOptions()
+        o.device = device
+        o.level = int(self.mode.level)
+        self._lut = None
+        if area_lut is not None:
+            self._lut = np.ascontiguousarray(area_lut, np.uint8)
+            o.area_lut_rgba = _ptr(self._lut)
+        o.initial_microline_capacity = initial_microline_capacity
+        o.initial_fill_capacity = initial_fill_capacity
+        o.enable_timing = 1 if (options.show_debug_ui if timing is None else timing) else 0
+        h = C.c_void_p()
+        L.check(self._lib.pfh_context_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pfh_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, status):
+        L.check(status, self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def begin_scene(self):
+        d = self.options.dest
+        fo = L.FrameOptions()
+        fo.width, fo.height = d.width, d.height
+        bg = self.options.background_color
+        fo.has_background = 1 if bg is not None else 0
+        if bg is not None:
+            fo.background_color = L.ColorF(bg.r, bg.g, bg.b, bg.a)
+        self._keep = []
+        self._check(self._lib.pfh_begin_scene(self._h, C.byref(fo)))
+
+    def render_commands(self, commands):
+        self._check(self._lib.pfh_render_commands(self._h, commands.handle))
+
+    def upload_texture_metadata(self, rgba):
+        a = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 4)
+        self._check(self._lib.pfh_upload_texture_metadata(self._h, _ptr(a), len(a)))
+
+    def upload_scene(self, draw, clip):
+        segs = []
+        for s in (draw, clip):
+            pts = np.ascontiguousarray(s["points"], np.float32)
+            idx = np.ascontiguousarray(s["indices"], np.uint32)
+            self._keep += [pts, idx]
+            segs.append(L.Segments(_ptr(pts), len(pts), _ptr(idx), len(idx)))
+        self._check(self._lib.pfh_upload_scene(self._h, C.byref(segs[0]), C.byref(segs[1])))
+
+    def prepare_tiles(self, batch):
+        tb, keep = numpy_to_batch(batch)
+        self._keep.append(keep)
+        self._check(self._lib.pfh_prepare_tiles(self._h, C.byref(tb)))
+
+    def prepare_and_draw_tiles(self, batch):
+        tb, keep = numpy_to_batch(batch)
+        self._keep.append(keep)
+        self._check(self._lib.pfh_prepare_and_draw_tiles(self._h, C.byref(tb)))
+
+    def end_scene(self):
+        self._check(self._lib.pfh_end_scene(self._h))
+
+    def rerender(self):
+        self._check(self._lib.pfh_rerender_scene(self._h))
+
+    def synchronize(self):
+        self._check(self._lib.pfh_synchronize(self._h))
+
+    def read_pixels(self):
+        d = self.options.dest
+        out = np.zeros((d.height, d.width, 4), np.uint8)
+        self._check(self._lib.pfh_read_framebuffer(self._h, _ptr(out), d.width * 4))
+        return out
+
+    def read_debug(self, batch_id, kind):
+        n = C.c_size_t()
+        self._check(self._lib.pfh_read_debug(self._h, batch_id, kind, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint8)
+        self._check(self._lib.pfh_read_debug(self._h, batch_id, kind, _ptr(out), out.size,
+                                             C.byref(n)))
+        return out[:n.value]
+
+    def read_mask(self):
+        n = C.c_size_t()
+        self._check(self._lib.pfh_read_mask(self._h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint32)
+        self._check(self._lib.pfh_read_mask(self._h, _ptr(out), out.size, C.byref(n)))
+        return out[:n.value]
+
+    def stats(self):
+        s = L.RenderStats()
+        self._check(self._lib.pfh_get_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in L.RenderStats._fields_}

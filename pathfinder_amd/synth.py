@@ -1,0 +1,234 @@
+"""Seeded synthetic scenes for the benchmark workloads and tests.
+
+Each generator returns a Scene plus its framebuffer size and background, built
+from numpy's PCG64 generator seeded with 0x5EED0000 + workload index, so every
+run (and the oracle) sees the same inputs.
+
+  small   1024x1024, 256 closed paths of 8-32 cubics, control points uniform
+          in [0,1024)^2, nonzero, random sRGB, 50% opaque (else alpha
+          U[0.2, 0.9]).
+  4k      3840x2160, 20,000 star-shaped paths (log-uniform 10-600 mixed
+          quadratic/cubic segments, log-uniform sizes 4-2000 px), 60% opaque,
+          20% even-odd, 500 rectangle/ellipse clip paths; about 5M microlines.
+"""
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .scene import ColorF, PathArrays, RectF, Scene
+
+SEED_BASE = 0x5EED0000
+KAPPA = 0.5522847498
+
+
+def _colors(rng, n, opaque_frac, scene):
+    rgb = rng.integers(0, 256, size=(n, 3))
+    opaque = rng.random(n) < opaque_frac
+    alpha = np.where(opaque, 255, np.round(rng.uniform(0.2, 0.9, n) * 255)).astype(np.int64)
+    ids = np.zeros(n, np.uint32)
+    from .scene import ColorU
+    for i in range(n):
+        ids[i] = scene.push_paint(ColorU(rgb[i, 0], rgb[i, 1], rgb[i, 2], alpha[i]))
+    return ids
+
+
+def small_scene(seed=SEED_BASE + 0, paths=256, size=1024):
+    """256 random closed cubic paths on a 1024x1024 canvas."""
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    scene.set_view_box(RectF(0, 0, size, size))
+    paints = _colors(rng, paths, 0.5, scene)
+    pts, flags, ends, pends = [], [], [], []
+    npts = 0
+    for p in range(paths):
+        k = int(rng.integers(8, 33))
+        start = rng.uniform(0, size, 2)
+        ctrl = rng.uniform(0, size, (k, 3, 2))
+        ctrl[-1, 2] = start  # the last cubic returns to the start point
+        path_pts = [start]
+        path_fl = [0]
+        for j in range(k):
+            path_pts += [ctrl[j, 0], ctrl[j, 1], ctrl[j, 2]]
+            path_fl += [L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0]
+        pts.append(np.asarray(path_pts, np.float32))
+        flags.append(np.asarray(path_fl, np.uint8))
+        npts += len(path_pts)
+        ends.append(npts)
+        pends.append(len(ends))
+    arr = PathArrays(np.concatenate(pts), np.concatenate(flags), np.asarray(ends, np.uint32),
+                     np.asarray(pends, np.uint32), paints)
+    scene.push_draw_paths(arr)
+    return scene, (size, size), ColorF(1.0, 1.0, 1.0, 1.0)
+
+
+def _blobs(rng, cx, cy, radius, k, quad_frac, jitter):
+    """Closed star-shaped contours, one per path, path i with k[i] mixed
+    quadratic/cubic segments. Returns (points, flags, contour_ends)."""
+    k = np.asarray(k, np.int64)
+    total = int(k.sum())
+    path = np.repeat(np.arange(len(k)), k)
+    first = np.concatenate([[0], np.cumsum(k)[:-1]])
+    j = np.arange(total) - first[path]
+    kk = k[path].astype(np.float64)
+    theta = (j + rng.uniform(0.0, 0.8, total)) * (2.0 * math.pi) / kk
+    last = j == k[path] - 1
+    th1 = np.where(last, theta[first[path]] + 2.0 * math.pi, np.roll(theta, -1))
+    rad = radius[path]
+    r_on = rad * rng.uniform(1.0 - jitter, 1.0, total)
+    quad = rng.random(total) < quad_frac
+    r_c = rad[:, None] * rng.uniform(1.0 - jitter, 1.0 + 0.5 * jitter, (total, 2))
+    npts = np.where(quad, 2, 3)
+    off = np.concatenate([[0], np.cumsum(npts)[:-1]])
+    n = int(npts.sum())
+    pts = np.zeros((n, 2), np.float64)
+    fl = np.zeros(n, np.uint8)
+    px, py = cx[path], cy[path]
+    pts[off, 0] = px + r_on * np.cos(theta)
+    pts[off, 1] = py + r_on * np.sin(theta)
+    q = np.nonzero(quad)[0]
+    tm = 0.5 * (theta[q] + th1[q])
+    pts[off[q] + 1, 0] = px[q] + r_c[q, 0] * np.cos(tm)
+    pts[off[q] + 1, 1] = py[q] + r_c[q, 0] * np.sin(tm)
+    fl[off[q] + 1] = L.POINT_CONTROL_0
+    c = np.nonzero(~quad)[0]
+    for m, (f, flag) in enumerate(((1.0 / 3.0, L.POINT_CONTROL_0), (2.0 / 3.0, L.POINT_CONTROL_1))):
+        tm = theta[c] + f * (th1[c] - theta[c])
+        pts[off[c] + 1 + m, 0] = px[c] + r_c[c, m] * np.cos(tm)
+        pts[off[c] + 1 + m, 1] = py[c] + r_c[c, m] * np.sin(tm)
+        fl[off[c] + 1 + m] = flag
+    seg_end = off + npts
+    ends = seg_end[np.cumsum(k) - 1]
+    return pts.astype(np.float32), fl, ends.astype(np.uint32)
+
+
+def _clip_shape(kind, cx, cy, hw, hh):
+    if kind == 0:  # rectangle: four lines
+        p = [(cx - hw, cy - hh), (cx + hw, cy - hh), (cx + hw, cy + hh), (cx - hw, cy + hh)]
+        return np.asarray(p, np.float32), np.zeros(4, np.uint8)
+    pts, fl = [], []
+    quads = [((1, 0), (0, 1)), ((0, 1), (-1, 0)), ((-1, 0), (0, -1)), ((0, -1), (1, 0))]
+    for (ax, ay), (bx, by) in quads:  # ellipse: four cubic arcs
+        pts.append((cx + ax * hw, cy + ay * hh))
+        fl.append(0)
+        pts.append((cx + (ax + KAPPA * bx) * hw, cy + (ay + KAPPA * by) * hh))
+        fl.append(L.POINT_CONTROL_0)
+        pts.append((cx + (bx + KAPPA * ax) * hw, cy + (by + KAPPA * ay) * hh))
+        fl.append(L.POINT_CONTROL_1)
+    return np.asarray(pts, np.float32), np.asarray(fl, np.uint8)
+
+
+def scene_4k(seed=SEED_BASE + 1, paths=20000, clips=500, width=3840, height=2160,
+             clipped_paths=2000, seg_lo=10, seg_hi=600, size_lo=4.0, size_hi=2000.0,
+             jitter=0.25):
+    """20k overlapping star-shaped paths with 500 clip paths on a 4K canvas."""
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    scene.set_view_box(RectF(0, 0, width, height))
+    paints = _colors(rng, paths, 0.6, scene)
+    sizes = np.exp(rng.uniform(math.log(size_lo), math.log(size_hi), paths))
+    segs = np.round(np.exp(rng.uniform(math.log(seg_lo), math.log(seg_hi), paths))).astype(int)
+    cx = rng.uniform(0, width, paths)
+    cy = rng.uniform(0, height, paths)
+    even_odd = rng.random(paths) < 0.2
+    clip_of = np.full(paths, -1, np.int32)
+    chosen = rng.choice(paths, size=min(clipped_paths, paths), replace=False)
+    clip_of[chosen] = np.arange(len(chosen)) % clips
+
+    # clip paths: centred on the first path that uses them
+    cpts, cfl, cends, cpends = [], [], [], []
+    n = 0
+    for c in range(clips):
+        users = chosen[np.arange(len(chosen)) % clips == c]
+        j = users[0] if len(users) else int(rng.integers(paths))
+        s = sizes[j] * rng.uniform(0.5, 1.5)
+        hw, hh = 0.5 * s * rng.uniform(0.6, 1.0), 0.5 * s * rng.uniform(0.6, 1.0)
+        ccx = min(max(cx[j], hw), width - hw) if 2 * hw < width else width / 2
+        ccy = min(max(cy[j], hh), height - hh) if 2 * hh < height else height / 2
+        p, f = _clip_shape(int(rng.integers(2)), ccx, ccy, hw, hh)
+        cpts.append(p)
+        cfl.append(f)
+        n += len(p)
+        cends.append(n)
+        cpends.append(len(cends))
+    if clips:
+        scene.push_clip_paths(PathArrays(np.concatenate(cpts), np.concatenate(cfl),
+                                         np.asarray(cends, np.uint32),
+                                         np.asarray(cpends, np.uint32)))
+
+    pts, fl, ends = _blobs(rng, cx, cy, 0.5 * sizes, segs, 0.5, jitter)
+    pends = np.arange(1, paths + 1, dtype=np.uint32)
+    arr = PathArrays(pts, fl, ends, pends, paints,
+                     np.where(even_odd, L.FILL_RULE_EVEN_ODD, L.FILL_RULE_WINDING).astype(np.int32),
+                     clip_of)
+    scene.push_draw_paths(arr)
+    return scene, (width, height), ColorF(1.0, 1.0, 1.0, 1.0)
+
+
+def _contour(points, flags):
+    return np.asarray(points, np.float32), np.asarray(flags, np.uint8)
+
+
+def tiny_scenes():
+    """Small hand-made scenes that exercise one feature each:
+    name -> (scene, (w, h), background or None)."""
+    from .scene import ColorU
+    out = {}
+
+    def make(w, h):
+        s = Scene()
+        s.set_view_box(RectF(0, 0, w, h))
+        return s
+
+    def add(scene, contours, color, rule=L.FILL_RULE_WINDING, clip=-1, is_clip=False):
+        pts = np.concatenate([c[0] for c in contours])
+        fl = np.concatenate([c[1] for c in contours])
+        ends = np.cumsum([len(c[0]) for c in contours]).astype(np.uint32)
+        if is_clip:
+            return scene.push_clip_paths(PathArrays(pts, fl, ends, [len(ends)], None, [rule],
+                                                    [clip]))
+        pid = scene.push_paint(ColorU(*color))
+        return scene.push_draw_paths(PathArrays(pts, fl, ends, [len(ends)], [pid], [rule],
+                                                [clip]))
+
+    s = make(64, 64)
+    add(s, [_contour([(8, 8), (40, 8), (40, 40), (8, 40)], [0, 0, 0, 0])], (200, 30, 30, 255))
+    out["square"] = (s, (64, 64), ColorF(1, 1, 1, 1))
+
+    s = make(64, 48)
+    add(s, [_contour([(3.3, 5.1), (60.2, 20.7), (12.9, 44.4)], [0, 0, 0])], (20, 90, 200, 180))
+    out["triangle"] = (s, (64, 48), ColorF(1, 1, 1, 1))
+
+    s = make(96, 96)
+    add(s, [_contour([(10, 80), (20, -30), (120, 140), (85, 12)],
+                     [0, L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0])], (30, 160, 60, 255))
+    add(s, [_contour([(5, 50), (48, -10), (90, 50), (48, 30)],
+                     [0, L.POINT_CONTROL_0, 0, L.POINT_CONTROL_0])], (250, 200, 0, 128))
+    out["curves"] = (s, (96, 96), ColorF(1, 1, 1, 1))
+
+    # winding number 2 where two contours of one path overlap
+    s = make(80, 80)
+    a = _contour([(5, 5), (60, 5), (60, 60), (5, 60)], [0, 0, 0, 0])
+    b = _contour([(20, 20), (75, 20), (75, 75), (20, 75)], [0, 0, 0, 0])
+    add(s, [a, b], (90, 20, 140, 255))
+    add(s, [a, b], (255, 128, 0, 100), rule=L.FILL_RULE_EVEN_ODD)
+    out["overlap"] = (s, (80, 80), ColorF(1, 1, 1, 1))
+
+    # a clip path (ellipse) applied to a square and a triangle, nested clip on a third
+    s = make(128, 96)
+    e = _clip_shape(1, 60, 48, 40, 30)
+    c0 = add(s, [e], None, is_clip=True)
+    c2 = add(s, [_clip_shape(1, 80, 50, 35, 35)], None, is_clip=True)
+    r = _clip_shape(0, 70, 40, 30, 40)
+    c1 = add(s, [r], None, is_clip=True, clip=c2)
+    add(s, [_contour([(10, 10), (110, 10), (110, 86), (10, 86)], [0, 0, 0, 0])],
+        (0, 128, 255, 255), clip=c0)
+    add(s, [_contour([(0, 90), (64, 0), (128, 90)], [0, 0, 0])], (255, 0, 64, 200), clip=c0)
+    add(s, [_contour([(30, 20), (120, 30), (60, 90)], [0, 0, 0])], (20, 200, 20, 255), clip=c1)
+    out["clip"] = (s, (128, 96), ColorF(0.9, 0.9, 0.9, 1))
+
+    # a path partly outside the view box, and an empty frame edge tile
+    s = make(50, 40)
+    add(s, [_contour([(-20, -10), (45, 5), (30, 60)], [0, 0, 0])], (10, 10, 10, 255))
+    out["offscreen"] = (s, (50, 40), None)
+    return out

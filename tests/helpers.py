@@ -1,0 +1,69 @@
+"""Shared by the tests: render one scene with the product and with the oracle."""
+import numpy as np
+
+import oracle
+from oracle import packing
+from pathfinder_amd import _lib as L
+
+KINDS = {
+    "microlines": (L.DEBUG_MICROLINES, oracle.MICROLINES, np.uint32, 4),
+    "fills": (L.DEBUG_FILLS, oracle.FILLS, np.uint32, 3),
+    "tiles": (L.DEBUG_TILES, oracle.TILES, np.uint32, 4),
+    "zbuffer": (L.DEBUG_Z_BUFFER, oracle.ZBUFFER, np.int32, 1),
+    "first_tile_map": (L.DEBUG_FIRST_TILE_MAP, oracle.FIRST_TILE_MAP, np.int32, 1),
+    "alpha_tiles": (L.DEBUG_ALPHA_TILES, oracle.ALPHA_TILES, np.uint32, 2),
+    "backdrops": (L.DEBUG_BACKDROPS, oracle.BACKDROPS, np.int32, 3),
+    "alpha_range": (L.DEBUG_ALPHA_RANGE, oracle.ALPHA_RANGE, np.uint32, 1),
+}
+
+
+def product_read(renderer, batch_id, name):
+    kind, _, dt, per = KINDS[name]
+    raw = renderer.read_debug(batch_id, kind)
+    a = raw.view(dt)
+    return a.reshape(-1, per) if per > 1 else a
+
+
+def oracle_frame(scene, size, background, threads=8, transform=(1, 0, 0, 1, 0, 0)):
+    cmds = packing.build(scene, transform)
+    o = oracle.Oracle(threads=threads)
+    o.render(cmds, size[0], size[1], background)
+    return cmds, o
+
+
+def product_frame(renderer, scene, transform=None):
+    from pathfinder_amd import BuildOptions, Transform2F
+    opts = BuildOptions(Transform2F(*transform) if transform else None)
+    cl = scene.build_and_render(renderer, opts)
+    return cl
+
+
+def compare_frame(renderer, o, cmds, names=tuple(KINDS)):
+    """Returns {name: (n_mismatch, n_total)} over every batch, plus mask/framebuffer."""
+    out = {}
+    for _, b in cmds["batches"]:
+        bid = b["batch_id"]
+        for name in names:
+            got = product_read(renderer, bid, name)
+            want = o.read(bid, KINDS[name][1])
+            key = "%s[%d]" % (name, bid)
+            if got.shape != want.shape:
+                out[key] = ("shape", got.shape, want.shape)
+                continue
+            out[key] = (int(np.count_nonzero(got != want)), int(got.size))
+    got = renderer.read_mask()
+    want = o.read_mask()
+    out["mask"] = (("shape", got.shape, want.shape) if got.shape != want.shape else
+                   (int(np.count_nonzero(got != want)), int(got.size)))
+    got = renderer.read_pixels()
+    want = o.read_framebuffer()
+    out["framebuffer"] = (int(np.count_nonzero(got != want)), int(got.size))
+    return out
+
+
+def assert_bit_exact(report, nonempty=("microlines", "tiles", "framebuffer")):
+    bad = {k: v for k, v in report.items() if v[0] != 0}
+    assert not bad, "mismatches: %r" % bad
+    for name in nonempty:
+        sizes = [v[1] for k, v in report.items() if k == name or k.startswith(name + "[")]
+        assert sizes and sum(sizes) > 0, "nothing compared for %s: %r" % (name, report)

@@ -1,0 +1,92 @@
+"""Bit-exact parity of the HIP path with the oracle, through the C ABI.
+
+Every batch's microlines, fills (with list links), tiles, z-buffer, first-tile
+map, alpha tiles and column backdrops, the frame's alpha mask and the
+framebuffer must equal the oracle's word for word.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_bit_exact, compare_frame, oracle_frame, product_frame
+from pathfinder_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TINY = sorted(synth.tiny_scenes())
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_tiny_scene_bit_exact(name, renderer_factory):
+    scene, size, bg = synth.tiny_scenes()[name]
+    _say("case tiny", name, size)
+    cmds, o = oracle_frame(scene, size, bg)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
+
+
+def test_small_workload_bit_exact(renderer_factory):
+    scene, size, bg = synth.small_scene()
+    _say("case small 1024x1024 256 paths")
+    cmds, o = oracle_frame(scene, size, bg)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
+
+
+def test_transform_bit_exact(renderer_factory):
+    scene, size, bg = synth.small_scene(paths=48)
+    t = (0.8, 0.3, -0.25, 0.9, 100.5, -40.25)
+    _say("case small transformed", t)
+    cmds, o = oracle_frame(scene, size, bg, transform=t)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene, transform=t)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
+
+
+def test_overflow_replay_bit_exact(renderer_factory):
+    """Tiny starting capacities force the grow-and-render-again path."""
+    scene, size, bg = synth.small_scene(paths=64)
+    _say("case overflow replay")
+    cmds, o = oracle_frame(scene, size, bg)
+    r = renderer_factory(size[0], size[1], bg, initial_microline_capacity=64,
+                         initial_fill_capacity=64)
+    product_frame(r, scene)
+    assert r.stats()["overflow_replays"] >= 1
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
+
+
+def test_rerender_is_identical(renderer_factory):
+    scene, size, bg = synth.small_scene(paths=96)
+    _say("case rerender")
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    first = r.read_pixels().copy()
+    for _ in range(3):
+        r.rerender()
+    r.synchronize()
+    assert np.array_equal(first, r.read_pixels())
+
+
+@pytest.mark.slow
+def test_4k_workload_bit_exact(renderer_factory):
+    scene, size, bg = synth.scene_4k()
+    _say("case 4k 3840x2160 20k paths 500 clips")
+    cmds, o = oracle_frame(scene, size, bg, threads=16)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
