@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Frame benchmark of the D3D11-level compute path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload 4k|small] [--no-cpu-baseline]
+
+A step is one frame of the 3840x2160 workload (20,000 paths, 500 clip paths,
+about 5M microlines): every batch runs bound, dice, bin, propagate, sort and
+fill, and the draw batch composites into the RGBA8 framebuffer. The scene and
+batch metadata are resident in HBM before the timed region (the frame is
+re-rendered from device memory); packing on the host is not in the timed
+region. Prints one JSON line.
+
+value = microlines per second of frame time (Msegments/s); ms_per_step = frame
+time. roofline: the kernel with the largest time per frame, its algorithmic
+bytes per frame (formulas below) over its time per frame measured with HIP
+events on the launch stream inside the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "4K frame ms & Msegments/s vs llvmpipe ref; fwd+bwd/fwd ratio at ≤1 LSB"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(st, fb_tiles, pixels, points, load_action):
+    """Compulsory bytes per frame of each kernel, every buffer touched once.
+
+    Stage totals follow the per-stage model of the reference's buffers
+    (microline 16 B, fill 12 B, tile 16 B, alpha tile 8 B, backdrop 12 B,
+    mask 256 B per alpha tile, RGBA8 pixels); count passes are charged for
+    what they read and the counts they write."""
+    S, P, M, F = st["segment_count"], st["path_count"], st["microline_count"], st["fill_count"]
+    T, C, A = st["total_tile_count"], st["column_count"], st["alpha_tile_count"]
+    Lp, L, AL = st["linked_tile_count"], st["composited_tile_count"], \
+        st["composited_alpha_tile_count"]
+    Ac = st["clipped_alpha_tile_count"]
+    Np, Fb, N = points, fb_tiles, pixels
+    return {
+        "dice_count": 8 * S + 8 * Np + 16 * P + 4 * S,
+        "dice_emit": 8 * S + 8 * Np + 16 * P + 4 * S + 16 * M,
+        "bound": 16 * P + 16 * T,
+        "bin_count": 16 * M + 48 * P + 4 * M,
+        "bin_emit": 16 * M + 48 * P + 4 * M + 12 * F + 4 * F,
+        "fill_link": 4 * F + 4 * F + 4 * F,
+        "propagate_count": 12 * C + 16 * T + 4 * C,
+        "propagate_emit": 12 * C + 16 * T + 12 * T + 8 * A + 4 * C,
+        "sort_tiles": 8 * Fb + 4 * Lp + 4 * L,
+        "fill": 24 * A + 12 * F + 256 * A + 256 * Ac,
+        "tile": 4 * Fb + 16 * L + 256 * AL + 4 * N * (2 if load_action else 1),
+    }
+
+
+def cpu_baseline(cmds, size, bg, microlines):
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    o = oracle.Oracle(threads=threads)
+    t0 = time.perf_counter()
+    o.render(cmds, size[0], size[1], bg)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": round(microlines / dt / 1e6, 4), "unit": "Msegments/s", "cores": threads,
+            "kind": "port",
+            "sample": "one whole frame of the same workload through the oracle (the seven "
+                      "shaders restated in C, sequential dice/bin/propagate, threaded "
+                      "bound/sort/fill/tile): %.2f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="4k", choices=["4k", "small"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each GPU would render its own band of framebuffer tiles).", file=sys.stderr)
+        sys.exit(2)
+
+    from pathfinder_amd import DestFramebuffer, Renderer, RendererOptions, synth
+    from pathfinder_amd import _lib as L
+
+    if args.workload == "4k":
+        scene, size, bg = synth.scene_4k()
+        workload = "4k-20k-paths"
+    else:
+        scene, size, bg = synth.small_scene()
+        workload = "1024-256-cubic-paths"
+    print("bench: workload %s %dx%d" % (workload, size[0], size[1]), file=sys.stderr, flush=True)
+
+    r = Renderer(RendererOptions(DestFramebuffer.full_window(*size), bg))
+    t0 = time.perf_counter()
+    commands = scene.build()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r.begin_scene()
+    r.render_commands(commands)
+    r.end_scene()
+    t_first = time.perf_counter() - t0
+    # the first frame may have grown buffers; this one is the steady state with uploads
+    t0 = time.perf_counter()
+    r.begin_scene()
+    r.render_commands(commands)
+    r.end_scene()
+    t_upload_frame = time.perf_counter() - t0
+    st = r.stats()
+
+    # warm-up, timing every kernel to find the dominant one
+    r.profile(L.PROFILE_ALL)
+    r.profile_reset()
+    for _ in range(max(args.warmup, 1)):
+        r.rerender()
+    prof = r.profile_read()
+    per_frame = {k: v[1] / max(args.warmup, 1) for k, v in prof.items() if v[0]}
+    dominant = max(per_frame, key=per_frame.get)
+
+    # timed region: events only around the dominant kernel
+    r.profile(L.PROFILE_ONE, dominant)
+    r.profile_reset()
+    r.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.rerender()
+    r.synchronize()
+    dt = time.perf_counter() - t0
+    prof_t = r.profile_read()[dominant]
+    r.profile(L.PROFILE_OFF)
+
+    ms_per_step = dt / args.steps * 1e3
+    M = st["microline_count"]
+    fb_tiles = ((size[0] + 15) // 16) * ((size[1] + 15) // 16)
+    draw_pts, clip_pts = commands.segments()
+    points = len(draw_pts["points"]) + len(clip_pts["points"])
+    alg = algorithmic_bytes(st, fb_tiles, size[0] * size[1], points, bg is None)
+    k_ms = prof_t[1] / args.steps
+    bytes_k = alg.get(dominant)
+    achieved = bytes_k / (k_ms * 1e-3) / 1e9 if bytes_k else None
+
+    print("bench: first frame %.1f ms (with buffer growth), frame with uploads %.2f ms, "
+          "build %.1f ms" % (t_first * 1e3, t_upload_frame * 1e3, t_build * 1e3),
+          file=sys.stderr)
+    print("bench: per-kernel ms/frame (warm-up): " +
+          ", ".join("%s %.3f" % (k, v) for k, v in sorted(per_frame.items(),
+                                                        key=lambda kv: -kv[1])),
+          file=sys.stderr)
+    print("bench: stats %s" % json.dumps(st), file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": round(M / (ms_per_step * 1e-3) / 1e6, 3),
+        "unit": "Msegments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, pathfinder_amd/synth.py)",
+        "config": {"workload": workload, "width": size[0], "height": size[1],
+                   "paths": st["path_count"], "segments": st["segment_count"],
+                   "microlines": M, "fills": st["fill_count"],
+                   "tiles": st["total_tile_count"], "alpha_tiles": st["alpha_tile_count"],
+                   "frame_ms_with_uploads": round(t_upload_frame * 1e3, 3)},
+        "roofline": {"bound": "hbm", "kernel": dominant,
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None,
+                     "kernel_ms_per_frame": round(k_ms, 4),
+                     "algorithmic_bytes_per_frame": bytes_k},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import packing
+        cmds = packing.build(scene)
+        line["cpu_baseline"] = cpu_baseline(cmds, size, bg, M)
+    r.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,7 +161,7 @@ typedef struct pfh_context_options {
     const uint8_t *area_lut_rgba; /* 256x256 RGBA8 area LUT; NULL = built-in copy */
     uint32_t initial_microline_capacity; /* 0 = 16384 (d3d11/renderer.rs:43) */
     uint32_t initial_fill_capacity;      /* 0 = 16384 (d3d11/renderer.rs:44) */
-    uint32_t enable_timing;       /* record per-stage device times */
+    uint32_t enable_timing;       /* start with per-kernel timing on (PFH_PROFILE_ALL) */
 } pfh_context_options;
 
 /* Renderer::new (gpu/renderer.rs:181-340) + RendererD3D11::new. */
@@ -228,16 +228,39 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
 pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
                          size_t *out_words);
 
-/* RenderStats (gpu/perf.rs:20-44) plus path-specific counts and device times. */
+/* RenderStats (gpu/perf.rs:20-44) plus the path-specific counts of the last
+ * scene: microlines, columns, tiles linked into framebuffer-tile lists before
+ * the z-buffer cull, tiles composited after it and how many of those have an
+ * alpha mask, and alpha tiles that are clipped. */
 typedef struct pfh_render_stats {
     uint64_t path_count, fill_count, alpha_tile_count, total_tile_count, drawcall_count;
     uint64_t microline_count, segment_count, column_count;
+    uint64_t linked_tile_count, composited_tile_count, composited_alpha_tile_count;
+    uint64_t clipped_alpha_tile_count;
     uint64_t gpu_bytes_allocated;
     uint32_t overflow_replays;
-    /* device milliseconds of the last frame, by stage (valid when timing is on) */
-    float ms_dice, ms_bound, ms_bin, ms_propagate, ms_sort, ms_fill, ms_tile, ms_frame;
+    uint32_t pad;
 } pfh_render_stats;
 pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats);
+
+/* Per-kernel device timing: HIP events recorded on the context stream around
+ * kernel launches (the reference times draw calls with GL timer queries per
+ * TimeCategory, gpu/perf.rs:78-219). PFH_PROFILE_ONE times only `kernel`. */
+#define PFH_PROFILE_OFF 0
+#define PFH_PROFILE_ALL 1
+#define PFH_PROFILE_ONE 2
+typedef struct pfh_kernel_time {
+    const char *name;
+    uint64_t launches;
+    double total_ms;
+} pfh_kernel_time;
+uint32_t pfh_profile_kernel_count(void);
+const char *pfh_profile_kernel_name(uint32_t kernel);
+pfh_status pfh_profile_set(pfh_context *ctx, int32_t mode, int32_t kernel);
+pfh_status pfh_profile_reset(pfh_context *ctx);
+/* Waits for the stream and returns totals since the last reset, one entry per kernel. */
+pfh_status pfh_profile_read(pfh_context *ctx, pfh_kernel_time *out, uint32_t capacity,
+                            uint32_t *count);
 
 /* ------------------------------------------------------------ scene front-end */
 

@@ -135,6 +135,94 @@ class _Batch:
         }
 
 
+def _contour_table(arrays):
+    """(start, end, path) of every contour, in order."""
+    ends = arrays.contour_ends.astype(np.int64)
+    starts = np.concatenate([[0], ends[:-1]]) if len(ends) else ends
+    starts = np.minimum(starts, ends)
+    pce = arrays.path_contour_ends.astype(np.int64)
+    path = np.searchsorted(pce, np.arange(len(ends)), side="right")
+    return starts, ends, path
+
+
+def path_bounds(arrays):
+    """Outline::bounds of every path: union over its non-empty contours."""
+    starts, ends, path = _contour_table(arrays)
+    n = len(arrays)
+    out = [[F(0)] * 4 for _ in range(n)]
+    keep = ends > starts
+    if not np.any(keep):
+        return out
+    s0, e0, p0 = starts[keep], ends[keep], path[keep]
+    pts = arrays.points
+    # per contour min/max, then per path
+    x = pts[:, 0]
+    y = pts[:, 1]
+    cx0 = np.minimum.reduceat(x, s0)
+    cy0 = np.minimum.reduceat(y, s0)
+    cx1 = np.maximum.reduceat(x, s0)
+    cy1 = np.maximum.reduceat(y, s0)
+    # reduceat reduces [s0[i], s0[i+1]); contours are contiguous, so fix the ends
+    for arr, fn, src in ((cx0, np.min, x), (cy0, np.min, y), (cx1, np.max, x), (cy1, np.max, y)):
+        gaps = np.nonzero(np.append(s0[1:], len(src)) != e0)[0]
+        for g in gaps:
+            arr[g] = fn(src[s0[g]:e0[g]])
+    first = np.concatenate([[True], p0[1:] != p0[:-1]])
+    idx = np.nonzero(first)[0]
+    px0 = np.minimum.reduceat(cx0, idx)
+    py0 = np.minimum.reduceat(cy0, idx)
+    px1 = np.maximum.reduceat(cx1, idx)
+    py1 = np.maximum.reduceat(cy1, idx)
+    for k, pid in enumerate(p0[idx]):
+        out[int(pid)] = [F(px0[k]), F(py0[k]), F(px1[k]), F(py1[k])]
+    return out
+
+
+def _excl_cumsum(a):
+    out = np.zeros(len(a), np.int64)
+    if len(a) > 1:
+        out[1:] = np.cumsum(a[:-1])
+    return out
+
+
+def segments_vectorized(arrays):
+    """SegmentsD3D11::add_path over every path of `arrays` at once."""
+    starts, ends, path = _contour_table(arrays)
+    lens = ends - starts
+    keep = lens > 0
+    starts, path, lens = starts[keep], path[keep], lens[keep]
+    # output layout: each contour's points followed by its first point again
+    out_len = lens + 1
+    out_off = _excl_cumsum(out_len)
+    total = int(out_len.sum())
+    cid = np.repeat(np.arange(len(lens)), out_len)
+    local = np.arange(total) - out_off[cid]
+    src = np.where(local < lens[cid], starts[cid] + local, starts[cid])
+    points = arrays.points[src] if total else np.zeros((0, 2), np.float32)
+    # one segment per on-curve point; its kind comes from the next two flags
+    pid = np.repeat(np.arange(len(lens)), lens)
+    loc = np.arange(int(lens.sum())) - np.repeat(_excl_cumsum(lens), lens)
+    gidx = starts[pid] + loc
+    fl = arrays.flags
+    last = max(len(fl) - 1, 0)
+    n_in = lens[pid]
+    onc = (fl[gidx] & CONTROL) == 0 if len(gidx) else np.zeros(0, bool)
+    nxt = np.where(loc + 1 < n_in, fl[np.minimum(gidx + 1, last)], 0) if len(gidx) else gidx
+    nxt2 = np.where(loc + 2 < n_in, fl[np.minimum(gidx + 2, last)], 0) if len(gidx) else gidx
+    kind = np.where((nxt & 1) != 0, np.where((nxt2 & 2) != 0, CUBIC, QUAD), 0)
+    first_point = out_off[pid] + loc
+    sel = np.nonzero(onc)[0]
+    indices = np.zeros((len(sel), 2), np.uint32)
+    indices[:, 0] = first_point[sel]
+    indices[:, 1] = kind[sel]
+    counts = np.bincount(path[pid[sel]], minlength=len(arrays)) if len(sel) else \
+        np.zeros(len(arrays), np.int64)
+    ends_ = np.cumsum(counts)
+    ranges = [(int(e - c), int(e)) for c, e in zip(counts, ends_)]
+    return ({"points": np.ascontiguousarray(points, np.float32).reshape(-1, 2),
+             "indices": indices}, ranges)
+
+
 def build(scene, transform=(1, 0, 0, 1, 0, 0)):
     """SceneBuilder::build for the D3D11 level. Returns
     {"paints": (N, 4) u8, "draw": {...}, "clip": {...}, "batches": [(kind, dict)]}."""
@@ -144,16 +232,12 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
     clip = scene.clip_arrays()
 
     def segments(arrays):
-        pts, idx, ranges = [], [], []
-        for i in range(len(arrays)):
-            p, f, e = arrays.path(i)
-            ranges.append(add_path(p, f, e, pts, idx))
-        return ({"points": np.asarray(pts, np.float32).reshape(-1, 2),
-                 "indices": np.asarray(idx, np.uint32).reshape(-1, 2)}, ranges)
+        return segments_vectorized(arrays)
 
     clip_segs, clip_ranges = segments(clip)
     draw_segs, draw_ranges = segments(draw)
-    clip_bounds = [_bounds(*clip.path(i)[0::2]) for i in range(len(clip))]
+    clip_bounds = path_bounds(clip)
+    draw_bounds = path_bounds(draw)
 
     prepare_batches = []
     clip_map = {}
@@ -174,8 +258,7 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
     paints = np.asarray([[c.r, c.g, c.b, c.a] for c in scene.paints], np.uint8).reshape(-1, 4)
     draw_batch = None
     for i in range(len(draw)):
-        p, f, e = draw.path(i)
-        bounds = _xrect(t, _bounds(p, e))
+        bounds = _xrect(t, draw_bounds[i])
         clipped = _intersect(bounds, vb)
         if clipped is None:
             continue

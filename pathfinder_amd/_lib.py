@@ -100,10 +100,20 @@ class RenderStats(C.Structure):
                 ("alpha_tile_count", C.c_uint64), ("total_tile_count", C.c_uint64),
                 ("drawcall_count", C.c_uint64), ("microline_count", C.c_uint64),
                 ("segment_count", C.c_uint64), ("column_count", C.c_uint64),
+                ("linked_tile_count", C.c_uint64), ("composited_tile_count", C.c_uint64),
+                ("composited_alpha_tile_count", C.c_uint64),
+                ("clipped_alpha_tile_count", C.c_uint64),
                 ("gpu_bytes_allocated", C.c_uint64), ("overflow_replays", C.c_uint32),
-                ("ms_dice", C.c_float), ("ms_bound", C.c_float), ("ms_bin", C.c_float),
-                ("ms_propagate", C.c_float), ("ms_sort", C.c_float), ("ms_fill", C.c_float),
-                ("ms_tile", C.c_float), ("ms_frame", C.c_float)]
+                ("pad", C.c_uint32)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
+PROFILE_OFF = 0
+PROFILE_ALL = 1
+PROFILE_ONE = 2
 
 
 class BuildOptions(C.Structure):
@@ -137,6 +147,11 @@ SIGNATURES = {
     "pfh_read_debug": (I32, [P, U32, I32, P, SZ, C.POINTER(SZ)]),
     "pfh_read_mask": (I32, [P, P, SZ, C.POINTER(SZ)]),
     "pfh_get_stats": (I32, [P, C.POINTER(RenderStats)]),
+    "pfh_profile_kernel_count": (U32, []),
+    "pfh_profile_kernel_name": (C.c_char_p, [U32]),
+    "pfh_profile_set": (I32, [P, I32, I32]),
+    "pfh_profile_reset": (I32, [P]),
+    "pfh_profile_read": (I32, [P, C.POINTER(KernelTime), U32, C.POINTER(U32)]),
     "pfh_scene_create": (P, []),
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),

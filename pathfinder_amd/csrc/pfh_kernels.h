@@ -68,6 +68,7 @@ struct BatchArgs {
     int32_t *first_tile_map;       // fb tiles
     uint32_t *row_paths;           // fb_height_tiles * path_count
     uint32_t *row_counts;          // fb_height_tiles
+    uint4 *list_stats;             // per fb tile: linked, survivors, survivor alpha tiles
     uint32_t *scan_partials;
     BatchCounters *counters;
     FrameState *frame;
@@ -92,14 +93,31 @@ struct TileArgs {
     float clear[4];
 };
 
-void launch_batch_init(const BatchArgs &a, hipStream_t s);
-void launch_dice(const BatchArgs &a, hipStream_t s);
-void launch_bound(const BatchArgs &a, hipStream_t s);
-void launch_bin(const BatchArgs &a, hipStream_t s);
-void launch_propagate(const BatchArgs &a, hipStream_t s);
-void launch_sort(const BatchArgs &a, hipStream_t s);
-void launch_fill(const BatchArgs &a, hipStream_t s);
-void launch_tile(const TileArgs &a, hipStream_t s);
+// Every kernel launch of a frame, for per-kernel timing.
+enum KernelId {
+    K_BATCH_INIT, K_DICE_COUNT, K_DICE_SCAN, K_DICE_EMIT, K_BOUND, K_BIN_COUNT, K_BIN_SCAN,
+    K_BIN_EMIT, K_FILL_LINK, K_PROPAGATE_COUNT, K_PROPAGATE_SCAN, K_PROPAGATE_EMIT,
+    K_ROW_PATHS, K_SORT_TILES, K_FILL, K_TILE, K_COUNT
+};
+extern const char *const kKernelNames[K_COUNT];
+
+// Brackets kernel launches with events on the launch stream.
+class Profiler {
+  public:
+    virtual ~Profiler() {}
+    virtual bool wants(int kernel) const = 0;
+    virtual void begin(hipStream_t s, int kernel) = 0;
+    virtual void end(hipStream_t s, int kernel) = 0;
+};
+
+void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p);
 void launch_frame_reset(FrameState *frame, hipStream_t s);
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s);
 void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s);

@@ -770,12 +770,13 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
     int32_t succ = -1;        // lowest linked tile above the current chunk
     int32_t pending = -1;     // culled tile whose next is in a lower chunk
     int32_t lowest_survivor = -1;
+    uint32_t n_linked = 0, n_survivors = 0, n_survivor_alpha = 0;
     for (uint32_t end = len; end > 0;) {
         uint32_t begin = end > 64 ? end - 64 : 0;
         uint32_t k = begin + lane;
         bool valid = k < end;
         int32_t t = -1;
-        bool linked = false, occl = false;
+        bool linked = false, occl = false, alpha_tile = false;
         if (valid) {
             uint32_t p = list[k];
             uint4 info = a.tile_path_info[p];
@@ -789,11 +790,13 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
                 int32_t alpha = sext24(w.z);
                 int32_t bd = (int32_t)w.w >> 24;
                 linked = bd != 0 || alpha >= 0;
+                alpha_tile = alpha >= 0;
                 occl = linked && a.prop_md[(size_t)p * 12 + 6] != 0 && bd != 0 && alpha < 0;
             }
         }
         unsigned long long lmask = __ballot(linked);
         unsigned long long omask = __ballot(occl);
+        unsigned long long amask = __ballot(alpha_tile);
         int32_t zlane;  // lanes >= zlane survive
         if (z_found) {
             zlane = 64;
@@ -833,6 +836,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
             if (lo_lane < zlane) pending = lo_t;
             unsigned long long surv = lmask & (zlane >= 64 ? 0ull : (~0ull << zlane));
             if (surv) lowest_survivor = __shfl(t, __ffsll((long long)surv) - 1, 64);
+            n_linked += __popcll(lmask);
+            n_survivors += __popcll(surv);
+            n_survivor_alpha += __popcll(surv & amask);
             succ = lo_t;
         }
         end = begin;
@@ -841,6 +847,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
         if (pending >= 0) a.tiles[pending].x = 0xffffffffu;
         a.first_tile_map[wave] = lowest_survivor;
         a.zbuffer[8 + wave] = z_found ? z : 0;
+        a.list_stats[wave] = make_uint4(n_linked, n_survivors, n_survivor_alpha, 0);
     }
 }
 
@@ -1031,6 +1038,26 @@ __global__ void k_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t co
 
 // ------------------------------------------------------------------ launchers
 
+const char *const kKernelNames[K_COUNT] = {
+    "batch_init", "dice_count", "dice_scan", "dice_emit", "bound", "bin_count", "bin_scan",
+    "bin_emit", "fill_link", "propagate_count", "propagate_scan", "propagate_emit",
+    "row_paths", "sort_tiles", "fill", "tile"};
+
+namespace {
+struct Scope {
+    Profiler *p;
+    hipStream_t s;
+    int k;
+    bool on;
+    Scope(Profiler *p_, hipStream_t s_, int k_) : p(p_), s(s_), k(k_), on(p_ && p_->wants(k_)) {
+        if (on) p->begin(s, k);
+    }
+    ~Scope() {
+        if (on) p->end(s, k);
+    }
+};
+}  // namespace
+
 size_t scan_partials_words(uint32_t max_elements) {
     return (size_t)(max_elements + kScanTile - 1) / kScanTile + 1;
 }
@@ -1039,68 +1066,102 @@ void launch_frame_reset(FrameState *frame, hipStream_t s) {
     hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, s, frame);
 }
 
-void launch_batch_init(const BatchArgs &a, hipStream_t s) {
+void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint64_t n = (uint64_t)a.fb_width_tiles * a.fb_height_tiles + 8;
     if ((uint64_t)a.column_count * 3 > n) n = (uint64_t)a.column_count * 3;
+    Scope t(p, s, K_BATCH_INIT);
     hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a);
 }
 
-void launch_dice(const BatchArgs &a, hipStream_t s) {
+void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t g = grid_for(a.segment_count, kBlock);
-    hipLaunchKernelGGL(k_dice_count, dim3(g), dim3(kBlock), 0, s, a);
-    scan_exclusive(a.seg_offsets, nullptr, a.segment_count, a.segment_count, a.scan_partials,
-                   &a.counters->microlines, s);
-    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->microlines,
-                       a.microline_capacity, &a.counters->microline_count, &a.counters->overflow,
-                       1u);
+    {
+        Scope t(p, s, K_DICE_COUNT);
+        hipLaunchKernelGGL(k_dice_count, dim3(g), dim3(kBlock), 0, s, a);
+    }
+    {
+        Scope t(p, s, K_DICE_SCAN);
+        scan_exclusive(a.seg_offsets, nullptr, a.segment_count, a.segment_count, a.scan_partials,
+                       &a.counters->microlines, s);
+        hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->microlines,
+                           a.microline_capacity, &a.counters->microline_count,
+                           &a.counters->overflow, 1u);
+    }
+    Scope t(p, s, K_DICE_EMIT);
     hipLaunchKernelGGL(k_dice_emit, dim3(g), dim3(kBlock), 0, s, a);
 }
 
-void launch_bound(const BatchArgs &a, hipStream_t s) {
+void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (a.tile_count == 0) return;
+    Scope t(p, s, K_BOUND);
     hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock)), dim3(kBlock), 0, s, a);
 }
 
-void launch_bin(const BatchArgs &a, hipStream_t s) {
+void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t g = grid_for(a.microline_capacity, kBlock);
-    hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
-    scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
-                   a.scan_partials, &a.counters->fills, s);
-    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
-                       a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
-    hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);
-    if (a.path_count)
+    {
+        Scope t(p, s, K_BIN_COUNT);
+        hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
+    }
+    {
+        Scope t(p, s, K_BIN_SCAN);
+        scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
+                       a.scan_partials, &a.counters->fills, s);
+        hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
+                           a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
+    }
+    {
+        Scope t(p, s, K_BIN_EMIT);
+        hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);
+    }
+    if (a.path_count) {
+        Scope t(p, s, K_FILL_LINK);
         hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
+    }
 }
 
-void launch_propagate(const BatchArgs &a, hipStream_t s) {
+void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t g = grid_for(a.column_count, kBlock);
-    if (a.column_count)
+    if (a.column_count) {
+        Scope t(p, s, K_PROPAGATE_COUNT);
         hipLaunchKernelGGL(k_propagate_count, dim3(g), dim3(kBlock), 0, s, a);
-    scan_exclusive(a.col_alpha_offsets, nullptr, a.column_count, a.column_count, a.scan_partials,
-                   nullptr, s);
-    hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
-    if (a.column_count)
+    }
+    {
+        Scope t(p, s, K_PROPAGATE_SCAN);
+        scan_exclusive(a.col_alpha_offsets, nullptr, a.column_count, a.column_count,
+                       a.scan_partials, nullptr, s);
+        hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
+    }
+    if (a.column_count) {
+        Scope t(p, s, K_PROPAGATE_EMIT);
         hipLaunchKernelGGL(k_propagate_emit, dim3(g), dim3(kBlock), 0, s, a);
+    }
 }
 
-void launch_sort(const BatchArgs &a, hipStream_t s) {
+void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     if (fbn == 0) return;
-    hipLaunchKernelGGL(k_row_paths, dim3((uint32_t)a.fb_height_tiles), dim3(kBlock), 0, s, a);
+    {
+        Scope t(p, s, K_ROW_PATHS);
+        hipLaunchKernelGGL(k_row_paths, dim3((uint32_t)a.fb_height_tiles), dim3(kBlock), 0, s,
+                           a);
+    }
+    Scope t(p, s, K_SORT_TILES);
     hipLaunchKernelGGL(k_sort_tiles, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
 }
 
-void launch_fill(const BatchArgs &a, hipStream_t s) {
+void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (a.tile_count == 0) return;
     uint32_t g = grid_for((uint64_t)a.tile_count * 64, kBlock);
     if (g > 8192) g = 8192;
+    Scope t(p, s, K_FILL);
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, s, a);
 }
 
-void launch_tile(const TileArgs &a, hipStream_t s) {
+void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p) {
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     if (fbn == 0) return;
+    Scope t(p, s, K_TILE);
     hipLaunchKernelGGL(k_tile, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
 }
 

@@ -95,8 +95,77 @@ class Pool {
     uint64_t bytes_ = 0;
 };
 
-enum { STAGE_DICE, STAGE_BOUND, STAGE_BIN, STAGE_PROPAGATE, STAGE_FILL, STAGE_SORT, STAGE_TILE,
-       STAGE_COUNT };
+// Per-kernel HIP-event timing on the context stream.
+class EventProfiler : public pfh::Profiler {
+  public:
+    int mode = PFH_PROFILE_OFF;
+    int only = -1;
+    uint64_t launches[pfh::K_COUNT] = {};
+    double total_ms[pfh::K_COUNT] = {};
+
+    ~EventProfiler() override {
+        for (auto &p : pending_) {
+            (void)hipEventDestroy(p.start);
+            (void)hipEventDestroy(p.stop);
+        }
+        for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+    }
+    bool wants(int k) const override {
+        return mode == PFH_PROFILE_ALL || (mode == PFH_PROFILE_ONE && k == only);
+    }
+    void begin(hipStream_t s, int k) override {
+        Pending p;
+        p.kernel = k;
+        p.start = take();
+        p.stop = nullptr;
+        if (p.start) (void)hipEventRecord(p.start, s);
+        pending_.push_back(p);
+    }
+    void end(hipStream_t s, int k) override {
+        if (pending_.empty() || pending_.back().kernel != k || pending_.back().stop) return;
+        Pending &p = pending_.back();
+        p.stop = take();
+        if (p.stop) (void)hipEventRecord(p.stop, s);
+    }
+    // Folds finished measurements into the totals (the stream must be idle).
+    void collect() {
+        for (auto &p : pending_) {
+            float ms = 0;
+            if (p.start && p.stop && hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+                launches[p.kernel]++;
+                total_ms[p.kernel] += ms;
+            }
+            if (p.start) free_.push_back(p.start);
+            if (p.stop) free_.push_back(p.stop);
+        }
+        pending_.clear();
+    }
+    void reset() {
+        collect();
+        for (int i = 0; i < pfh::K_COUNT; i++) {
+            launches[i] = 0;
+            total_ms[i] = 0;
+        }
+    }
+
+  private:
+    struct Pending {
+        int kernel;
+        hipEvent_t start, stop;
+    };
+    hipEvent_t take() {
+        if (!free_.empty()) {
+            hipEvent_t e = free_.back();
+            free_.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> free_;
+};
 
 struct Batch {
     pfh::BatchData host;
@@ -122,12 +191,11 @@ struct Batch {
     int32_t *d_ftm = nullptr;
     uint32_t *d_row_paths = nullptr;
     uint32_t *d_row_counts = nullptr;
+    uint4 *d_list_stats = nullptr;
     uint32_t *d_scan = nullptr;
     pfh::BatchCounters *d_counters = nullptr;
     uint32_t cap_m = 0, cap_f = 0;
     pfh::BatchCounters counters = {};
-    hipEvent_t ev[STAGE_COUNT + 1] = {};
-    bool have_events = false;
 };
 
 }  // namespace
@@ -137,7 +205,7 @@ struct pfh_context {
     hipStream_t stream = nullptr;
     std::string error;
     Pool pool;
-    bool timing = false;
+    EventProfiler prof;
     // tables
     float4 *d_lut = nullptr;
     float4 *d_paints = nullptr;
@@ -206,11 +274,8 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
     dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
-    dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_scan);
-    dfree(ctx, b.d_counters);
-    if (b.have_events)
-        for (auto &e : b.ev) (void)hipEventDestroy(e);
-    b.have_events = false;
+    dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
+    dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
 }
 
 void free_batches(pfh_context *ctx) {
@@ -304,6 +369,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.first_tile_map = b.d_ftm;
     a.row_paths = b.d_row_paths;
     a.row_counts = b.d_row_counts;
+    a.list_stats = b.d_list_stats;
     a.scan_partials = b.d_scan;
     a.counters = b.d_counters;
     a.frame = ctx->d_frame;
@@ -320,21 +386,14 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 pfh_status run_batch(pfh_context *ctx, Batch &b) {
     pfh::BatchArgs a = batch_args(ctx, b);
     hipStream_t s = ctx->stream;
-    bool ev = ctx->timing && b.have_events;
-    if (ev) (void)hipEventRecord(b.ev[0], s);
-    pfh::launch_batch_init(a, s);
-    pfh::launch_dice(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_DICE], s);
-    pfh::launch_bound(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_BOUND], s);
-    pfh::launch_bin(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_BIN], s);
-    pfh::launch_propagate(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_PROPAGATE], s);
-    pfh::launch_fill(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_FILL], s);
-    pfh::launch_sort(a, s);
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_SORT], s);
+    pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
+    pfh::launch_batch_init(a, s, p);
+    pfh::launch_dice(a, s, p);
+    pfh::launch_bound(a, s, p);
+    pfh::launch_bin(a, s, p);
+    pfh::launch_propagate(a, s, p);
+    pfh::launch_fill(a, s, p);
+    pfh::launch_sort(a, s, p);
     if (b.draw) {
         pfh::TileArgs t;
         t.tiles = b.d_tiles;
@@ -349,9 +408,8 @@ pfh_status run_batch(pfh_context *ctx, Batch &b) {
         t.fb_height_tiles = ctx->fbh;
         t.load = b.load;
         for (int i = 0; i < 4; i++) t.clear[i] = b.clear[i];
-        pfh::launch_tile(t, s);
+        pfh::launch_tile(t, s, p);
     }
-    if (ev) (void)hipEventRecord(b.ev[1 + STAGE_TILE], s);
     HIP_TRY(ctx, hipGetLastError());
     return PFH_OK;
 }
@@ -424,12 +482,9 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
         PFH_OK)
         return s;
     if ((s = dalloc(ctx, &b.d_row_counts, (size_t)ctx->fbh)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_list_stats, fbn)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_counters, 1)) != PFH_OK) return s;
     if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
-    if (ctx->timing) {
-        for (auto &e : b.ev) HIP_TRY(ctx, hipEventCreate(&e));
-        b.have_events = true;
-    }
     ctx->batches.push_back(std::move(bp));
     return run_batch(ctx, b);
 }
@@ -451,7 +506,7 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     std::unique_ptr<pfh_context> ctx(new (std::nothrow) pfh_context());
     if (!ctx) return PFH_ERROR_OUT_OF_MEMORY;
     ctx->device = opt.device;
-    ctx->timing = opt.enable_timing != 0;
+    ctx->prof.mode = opt.enable_timing ? PFH_PROFILE_ALL : PFH_PROFILE_OFF;
     if (opt.initial_microline_capacity) ctx->cap_m = next_pow2(opt.initial_microline_capacity);
     if (opt.initial_fill_capacity) ctx->cap_f = next_pow2(opt.initial_fill_capacity);
     if (hipSetDevice(opt.device) != hipSuccess) return PFH_ERROR_NO_DEVICE;
@@ -648,23 +703,6 @@ pfh_status pfh_end_scene(pfh_context *ctx) {
         st.segment_count += b->host.segment_count;
         st.column_count += b->host.backdrops.size();
         st.drawcall_count += b->draw ? 7 : 6;
-        if (b->have_events) {
-            float ms[STAGE_COUNT];
-            for (int i = 0; i < STAGE_COUNT; i++) {
-                ms[i] = 0;
-                (void)hipEventElapsedTime(&ms[i], b->ev[i], b->ev[i + 1]);
-            }
-            st.ms_dice += ms[STAGE_DICE];
-            st.ms_bound += ms[STAGE_BOUND];
-            st.ms_bin += ms[STAGE_BIN];
-            st.ms_propagate += ms[STAGE_PROPAGATE];
-            st.ms_fill += ms[STAGE_FILL];
-            st.ms_sort += ms[STAGE_SORT];
-            st.ms_tile += ms[STAGE_TILE];
-            float f = 0;
-            (void)hipEventElapsedTime(&f, b->ev[0], b->ev[STAGE_COUNT]);
-            st.ms_frame += f;
-        }
     }
     st.gpu_bytes_allocated = ctx->pool.bytes();
     st.overflow_replays = ctx->replays;
@@ -748,7 +786,68 @@ pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
 
 pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats) {
     if (!ctx || !stats) return PFH_ERROR_INVALID_ARGUMENT;
-    *stats = ctx->stats;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    pfh_render_stats &st = ctx->stats;
+    st.linked_tile_count = st.composited_tile_count = st.composited_alpha_tile_count = 0;
+    st.clipped_alpha_tile_count = 0;
+    size_t fbn = (size_t)ctx->fbw * ctx->fbh;
+    std::vector<uint4> ls(fbn);
+    for (auto &b : ctx->batches) {
+        if (fbn)
+            HIP_TRY(ctx, hipMemcpy(ls.data(), b->d_list_stats, fbn * sizeof(uint4),
+                                   hipMemcpyDeviceToHost));
+        for (const uint4 &v : ls) {
+            st.linked_tile_count += v.x;
+            if (b->draw) {
+                st.composited_tile_count += v.y;
+                st.composited_alpha_tile_count += v.z;
+            }
+        }
+        std::vector<uint2> at(b->counters.alpha);
+        if (!at.empty())
+            HIP_TRY(ctx, hipMemcpy(at.data(), b->d_alpha, at.size() * sizeof(uint2),
+                                   hipMemcpyDeviceToHost));
+        for (const uint2 &v : at) st.clipped_alpha_tile_count += (int32_t)v.y >= 0 ? 1 : 0;
+    }
+    *stats = st;
+    return PFH_OK;
+}
+
+uint32_t pfh_profile_kernel_count(void) { return pfh::K_COUNT; }
+
+const char *pfh_profile_kernel_name(uint32_t kernel) {
+    return kernel < (uint32_t)pfh::K_COUNT ? pfh::kKernelNames[kernel] : nullptr;
+}
+
+pfh_status pfh_profile_set(pfh_context *ctx, int32_t mode, int32_t kernel) {
+    if (!ctx || mode < PFH_PROFILE_OFF || mode > PFH_PROFILE_ONE ||
+        (mode == PFH_PROFILE_ONE && (kernel < 0 || kernel >= pfh::K_COUNT)))
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad profile mode %d / kernel %d", mode,
+                    kernel);
+    ctx->prof.mode = mode;
+    ctx->prof.only = kernel;
+    return PFH_OK;
+}
+
+pfh_status pfh_profile_reset(pfh_context *ctx) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->prof.reset();
+    return PFH_OK;
+}
+
+pfh_status pfh_profile_read(pfh_context *ctx, pfh_kernel_time *out, uint32_t capacity,
+                            uint32_t *count) {
+    if (!ctx || !count) return PFH_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->prof.collect();
+    *count = pfh::K_COUNT;
+    if (!out) return PFH_OK;
+    for (uint32_t i = 0; i < (uint32_t)pfh::K_COUNT && i < capacity; i++) {
+        out[i].name = pfh::kKernelNames[i];
+        out[i].launches = ctx->prof.launches[i];
+        out[i].total_ms = ctx->prof.total_ms[i];
+    }
     return PFH_OK;
 }
 

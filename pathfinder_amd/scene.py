@@ -577,4 +577,27 @@ class Renderer:
     def stats(self):
         s = L.RenderStats()
         self._check(self._lib.pfh_get_stats(self._h, C.byref(s)))
-        return {name: getattr(s, name) for name, _ in L.RenderStats._fields_}
+        return {name: getattr(s, name) for name, _ in L.RenderStats._fields_ if name != "pad"}
+
+    @staticmethod
+    def kernel_names():
+        lib = L.load()
+        return [lib.pfh_profile_kernel_name(i).decode()
+                for i in range(lib.pfh_profile_kernel_count())]
+
+    def profile(self, mode, kernel=None):
+        """PFH_PROFILE_OFF / ALL / ONE (by kernel name or id)."""
+        if isinstance(kernel, str):
+            kernel = self.kernel_names().index(kernel)
+        self._check(self._lib.pfh_profile_set(self._h, mode, -1 if kernel is None else kernel))
+
+    def profile_reset(self):
+        self._check(self._lib.pfh_profile_reset(self._h))
+
+    def profile_read(self):
+        """{kernel: (launches, total_ms)} since the last reset."""
+        n = self._lib.pfh_profile_kernel_count()
+        arr = (L.KernelTime * n)()
+        got = C.c_uint32()
+        self._check(self._lib.pfh_profile_read(self._h, arr, n, C.byref(got)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n)}

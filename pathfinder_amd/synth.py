@@ -62,7 +62,7 @@ def small_scene(seed=SEED_BASE + 0, paths=256, size=1024):
     return scene, (size, size), ColorF(1.0, 1.0, 1.0, 1.0)
 
 
-def _blobs(rng, cx, cy, radius, k, quad_frac, jitter):
+def _blobs(rng, cx, cy, radius, k, quad_frac, jitter, ctrl_jitter):
     """Closed star-shaped contours, one per path, path i with k[i] mixed
     quadratic/cubic segments. Returns (points, flags, contour_ends)."""
     k = np.asarray(k, np.int64)
@@ -76,8 +76,14 @@ def _blobs(rng, cx, cy, radius, k, quad_frac, jitter):
     th1 = np.where(last, theta[first[path]] + 2.0 * math.pi, np.roll(theta, -1))
     rad = radius[path]
     r_on = rad * rng.uniform(1.0 - jitter, 1.0, total)
+    r_next = np.where(last, r_on[first[path]], np.roll(r_on, -1))
     quad = rng.random(total) < quad_frac
-    r_c = rad[:, None] * rng.uniform(1.0 - jitter, 1.0 + 0.5 * jitter, (total, 2))
+    # control points follow the arc between the two on-curve radii, so the
+    # curvature (and the dice count) comes from the path size and segment count
+    lerp = lambda f: r_on + f * (r_next - r_on)
+    r_c = np.stack([np.where(quad, lerp(0.5), lerp(1.0 / 3.0)), lerp(2.0 / 3.0)], 1)
+    r_c = r_c / np.cos(np.where(quad, 0.5, 1.0 / 3.0)[:, None] * (th1 - theta)[:, None] * 0.5)
+    r_c = r_c * rng.uniform(1.0 - ctrl_jitter, 1.0 + ctrl_jitter, (total, 2))
     npts = np.where(quad, 2, 3)
     off = np.concatenate([[0], np.cumsum(npts)[:-1]])
     n = int(npts.sum())
@@ -120,7 +126,7 @@ def _clip_shape(kind, cx, cy, hw, hh):
 
 def scene_4k(seed=SEED_BASE + 1, paths=20000, clips=500, width=3840, height=2160,
              clipped_paths=2000, seg_lo=10, seg_hi=600, size_lo=4.0, size_hi=2000.0,
-             jitter=0.25):
+             jitter=0.15, ctrl_jitter=0.008):
     """20k overlapping star-shaped paths with 500 clip paths on a 4K canvas."""
     rng = np.random.default_rng(seed)
     scene = Scene()
@@ -156,7 +162,7 @@ def scene_4k(seed=SEED_BASE + 1, paths=20000, clips=500, width=3840, height=2160
                                          np.asarray(cends, np.uint32),
                                          np.asarray(cpends, np.uint32)))
 
-    pts, fl, ends = _blobs(rng, cx, cy, 0.5 * sizes, segs, 0.5, jitter)
+    pts, fl, ends = _blobs(rng, cx, cy, 0.5 * sizes, segs, 0.5, jitter, ctrl_jitter)
     pends = np.arange(1, paths + 1, dtype=np.uint32)
     arr = PathArrays(pts, fl, ends, pends, paints,
                      np.where(even_odd, L.FILL_RULE_EVEN_ODD, L.FILL_RULE_WINDING).astype(np.int32),
