@@ -529,8 +529,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_emit(BatchArgs a) {
 // one workgroup per path sorts its fills by (tile, fill index) in LDS chunks
 // and stitches the runs; chunks of one path run in order, so the exchange on
 // the tile head carries a list across chunks.
-constexpr uint32_t kLinkChunk = 2048;
+// Paths with at most kLinkSmall fills sort in 16 KiB of LDS; larger ones in a
+// second launch with 64 KiB chunks, so the common case keeps occupancy.
+constexpr uint32_t kLinkSmall = 2048;
+constexpr uint32_t kLinkBig = 8192;
 
+template <uint32_t kLinkChunk, bool kBig>
 __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     __shared__ unsigned long long keys[kLinkChunk];
     uint32_t path = blockIdx.x;
@@ -543,6 +547,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     if (s1 < s0) s1 = s0;
     uint32_t m0 = min(a.seg_offsets[s0], mcount), m1 = min(a.seg_offsets[s1], mcount);
     uint32_t f0 = min(a.ml_fill_offsets[m0], fcount), f1 = min(a.ml_fill_offsets[m1], fcount);
+    if (kBig ? (f1 - f0 <= kLinkSmall) : (f1 - f0 > kLinkSmall)) return;
+    // A path sorted in one chunk leaves each tile's fills contiguous in
+    // sorted_fills; k_fill reads them from there instead of walking the list.
+    bool single = f1 - f0 <= kLinkChunk;
     for (uint32_t base = f0; base < f1; base += kLinkChunk) {
         uint32_t n = min(kLinkChunk, f1 - base);
         uint32_t size = 64;
@@ -582,10 +590,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
                 }
                 uint32_t last_f = (uint32_t)keys[lo];
                 link = atomicExch(&a.tiles[tile].y, last_f);
+                a.tile_runs[tile] = single ? make_uint2(base + i, lo - i + 1) : make_uint2(0, 0);
             } else {
                 link = (uint32_t)keys[i - 1];
             }
             a.fills[(size_t)f * 3 + 2] = link;
+            if (single)
+                a.sorted_fills[base + i] =
+                    make_uint2(a.fills[(size_t)f * 3 + 0], a.fills[(size_t)f * 3 + 1]);
         }
         __syncthreads();
     }
@@ -858,14 +870,14 @@ __device__ __forceinline__ float4 sample_lut(const float4 *__restrict__ lut, flo
     float s = u * 256.0f - 0.5f, t = v * 256.0f - 0.5f;
     float fs = floorf(s), ft = floorf(t);
     float wa = s - fs, wb = t - ft;
-    int32_t i0 = f2i_sat(fs), j0 = f2i_sat(ft);
-    int32_t i1 = i0 < 2147483647 ? i0 + 1 : i0, j1 = j0 < 2147483647 ? j0 + 1 : j0;
-    i0 = min(max(i0, 0), 255);
-    i1 = min(max(i1, 0), 255);
-    j0 = min(max(j0, 0), 255);
-    j1 = min(max(j1, 0), 255);
-    float4 t00 = lut[j0 * 256 + i0], t10 = lut[j0 * 256 + i1];
-    float4 t01 = lut[j1 * 256 + i0], t11 = lut[j1 * 256 + i1];
+    // CLAMP_TO_EDGE: texel indices clamp to [0, 255] (also for |fs| >= 2^31, where
+    // the reference's int conversion saturates first)
+    uint32_t i0 = (uint32_t)fminf(fmaxf(fs, 0.0f), 255.0f);
+    uint32_t i1 = (uint32_t)fminf(fmaxf(fs + 1.0f, 0.0f), 255.0f);
+    uint32_t j0 = (uint32_t)fminf(fmaxf(ft, 0.0f), 255.0f);
+    uint32_t j1 = (uint32_t)fminf(fmaxf(ft + 1.0f, 0.0f), 255.0f);
+    float4 t00 = lut[(j0 << 8) | i0], t10 = lut[(j0 << 8) | i1];
+    float4 t01 = lut[(j1 << 8) | i0], t11 = lut[(j1 << 8) | i1];
     float ia = 1.0f - wa, ib = 1.0f - wb;
     float4 r;
     r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
@@ -890,38 +902,118 @@ __device__ __forceinline__ float4 compute_coverage(const float4 *__restrict__ lu
     return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
 }
 
-// fill.cs.glsl:61-88 + fill_compute.inc.glsl:11-25. One wavefront per alpha
-// tile; lane (lx, ly) owns the four pixels of column lx, rows 4ly..4ly+3,
-// which are the four channels of mask texel (lx, ly).
+// computeCoverage with the slope d precomputed (it is the same for every pixel).
+__device__ __forceinline__ float4 coverage_with_slope(const float4 *__restrict__ lut, float fx,
+                                                      float fy, float tx, float ty, float d) {
+    bool lt = fx < tx;
+    float lx = lt ? fx : tx, ly = lt ? fy : ty, rx = lt ? tx : fx, ry = lt ? ty : fy;
+    float wx = clamp_glsl(fx, -0.5f, 0.5f), wy = clamp_glsl(tx, -0.5f, 0.5f);
+    float offset = mixf(wx, wy, 0.5f) - lx;
+    float t = offset / (rx - lx);
+    float yy = mixf(ly, ry, t);
+    float dX = wx - wy;
+    float4 s = sample_lut(lut, (yy + 8.0f) / 16.0f, fabsf(d * dX) / 16.0f);
+    return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
+}
+
+// Coverage of one alpha tile whose fills sit at sorted_fills[top - k], k < n
+// (descending fill index, at most 1024): the list walk of fill_compute.inc.glsl
+// visits them in this order. A fill adds exactly +-0 to a pixel column its
+// window does not see (dX = 0: both ends at or left of the column's left edge,
+// or both at or right of its right edge) and the sum ends in abs(), so each
+// lane walks only the fills that touch its column, in the same order. `first`
+// holds the first 64 fills, loaded ahead by the caller.
+__device__ __forceinline__ float4 accumulate_run(const BatchArgs &a, uint32_t top, uint32_t n,
+                                                 uint2 first, uint32_t lane, float fragx,
+                                                 float fragy) {
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t col = lane & 15u;
+    for (uint32_t c = 0; c < n; c += 64) {
+        uint32_t k = c + lane;
+        bool have = k < n;
+        uint2 fw = c == 0 ? first : (have ? a.sorted_fills[top - k] : make_uint2(0, 0));
+        uint32_t fxi = fw.x & 0xffffu, txi = fw.y & 0xffffu;
+        uint32_t lo = min(fxi, txi), hi = max(fxi, txi);
+        uint32_t c0 = lo >> 8, c1 = hi > 0 ? (hi - 1) >> 8 : 0;
+        bool any = have && hi > lo;
+        // d = (right.y - left.y) / (right.x - left.x) does not depend on the pixel:
+        // the pixel offset cancels exactly in both differences
+        bool lt = fxi < txi;
+        float lyf = (float)(lt ? (fw.x >> 16) : (fw.y >> 16)) / 256.0f;
+        float ryf = (float)(lt ? (fw.y >> 16) : (fw.x >> 16)) / 256.0f;
+        float dxw = (float)(hi - lo) / 256.0f;
+        float dfill = any ? (ryf - lyf) / dxw : 0.0f;
+        unsigned long long mine = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 16; q++) {
+            unsigned long long m = __ballot(any && c0 <= q && q <= c1);
+            mine = q == col ? m : mine;
+        }
+        while (__ballot(mine != 0ull)) {
+            bool act = mine != 0ull;
+            uint32_t j = act ? (uint32_t)(__ffsll((long long)mine) - 1) : 0u;
+            mine = act ? (mine & (mine - 1ull)) : mine;
+            uint32_t fw0 = __shfl(fw.x, j, 64), fw1 = __shfl(fw.y, j, 64);
+            float d = __shfl(dfill, j, 64);
+            if (act) {
+                float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
+                float sy = (float)(fw0 >> 16) / 256.0f - fragy;
+                float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
+                float ey = (float)(fw1 >> 16) / 256.0f - fragy;
+                float4 cv = coverage_with_slope(a.lut, sx, sy, ex, ey, d);
+                acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
+            }
+        }
+    }
+    return acc;
+}
+
+// The reference's list walk (fill_compute.inc.glsl:11-25), for tiles of paths
+// too large to sort in one chunk.
+__device__ float4 accumulate_list(const BatchArgs &a, int32_t f, float fragx, float fragy) {
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int it = 0;
+    do {
+        if ((uint32_t)f >= a.fill_capacity) break;
+        uint32_t fw0 = a.fills[(size_t)f * 3 + 0], fw1 = a.fills[(size_t)f * 3 + 1];
+        int32_t link = (int32_t)a.fills[(size_t)f * 3 + 2];
+        float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
+        float sy = (float)(fw0 >> 16) / 256.0f - fragy;
+        float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
+        float ey = (float)(fw1 >> 16) / 256.0f - fragy;
+        float4 cv = compute_coverage(a.lut, sx, sy, ex, ey);
+        acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
+        f = link;
+        it++;
+    } while (f >= 0 && it < 1024);
+    return acc;
+}
+
+// fill.cs.glsl:61-88. One wavefront per alpha tile; lane (lx, ly) owns the
+// four pixels of column lx, rows 4ly..4ly+3 = the four channels of mask texel
+// (lx, ly).
 __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     uint32_t lane = lane_id();
     uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-    uint32_t n = a.counters->alpha;
+    uint32_t n_alpha = a.counters->alpha;
     uint32_t start = a.counters->alpha_start;
     float fragx = (float)(lane & 15u) + 0.5f, fragy = (float)((lane >> 4) * 4u) + 0.5f;
-    for (uint32_t ba = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; ba < n; ba += waves) {
+    for (uint32_t ba = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; ba < n_alpha; ba += waves) {
         uint32_t alpha_index = start + ba;
         if (alpha_index >= a.mask_capacity) break;
         uint2 at = a.alpha_tiles[ba];
         uint4 tile = a.tiles[at.x];
         if (sext24(tile.z) < 0) continue;
-        int32_t f = (int32_t)tile.y;
         int32_t backdrop = (int32_t)tile.w >> 24;
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        int it = 0;
-        do {
-            if ((uint32_t)f >= a.fill_capacity) break;
-            uint32_t fw0 = a.fills[(size_t)f * 3 + 0], fw1 = a.fills[(size_t)f * 3 + 1];
-            int32_t link = (int32_t)a.fills[(size_t)f * 3 + 2];
-            float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
-            float sy = (float)(fw0 >> 16) / 256.0f - fragy;
-            float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
-            float ey = (float)(fw1 >> 16) / 256.0f - fragy;
-            float4 c = compute_coverage(a.lut, sx, sy, ex, ey);
-            acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
-            f = link;
-            it++;
-        } while (f >= 0 && it < 1024);
+        uint2 run = a.tile_runs[at.x];
+        float4 acc;
+        if (run.y != 0) {
+            uint32_t top = run.x + run.y - 1, n = min(run.y, 1024u);
+            uint2 first = lane < n ? a.sorted_fills[top - lane] : make_uint2(0, 0);
+            acc = accumulate_run(a, top, n, first, lane, fragx, fragy);
+        } else {
+            acc = accumulate_list(a, (int32_t)tile.y, fragx, fragy);
+        }
         float bd = (float)backdrop;
         float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
         float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
@@ -1116,7 +1208,10 @@ void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
     if (a.path_count) {
         Scope t(p, s, K_FILL_LINK);
-        hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_fill_link<kLinkSmall, false>), dim3(a.path_count), dim3(kBlock), 0,
+                           s, a);
+        hipLaunchKernelGGL((k_fill_link<kLinkBig, true>), dim3(a.path_count), dim3(kBlock), 0, s,
+                           a);
     }
 }
 

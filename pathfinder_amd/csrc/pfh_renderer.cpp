@@ -184,6 +184,8 @@ struct Batch {
     uint32_t *d_ml_fill_offsets = nullptr;
     uint32_t *d_fills = nullptr;
     uint32_t *d_fill_tiles = nullptr;
+    uint2 *d_sorted_fills = nullptr;
+    uint2 *d_tile_runs = nullptr;
     uint4 *d_tiles = nullptr;
     uint32_t *d_col_alpha = nullptr;
     uint2 *d_alpha = nullptr;
@@ -272,7 +274,8 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
-    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_tile_runs);
+    dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
     dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
@@ -297,7 +300,8 @@ pfh_status upload(pfh_context *ctx, T **dst, const std::vector<T> &src) {
 pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if (b.cap_m == ctx->cap_m && b.cap_f == ctx->cap_f && b.d_microlines) return PFH_OK;
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets);
-    dfree(ctx, b.d_fills); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_scan);
+    dfree(ctx, b.d_fills); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills);
+    dfree(ctx, b.d_scan);
     b.cap_m = ctx->cap_m;
     b.cap_f = ctx->cap_f;
     pfh_status s;
@@ -305,6 +309,7 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if ((s = dalloc(ctx, &b.d_ml_fill_offsets, (size_t)b.cap_m + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_fills, (size_t)b.cap_f * 3)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_fill_tiles, b.cap_f)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_sorted_fills, b.cap_f)) != PFH_OK) return s;
     uint32_t scan_n = std::max({b.host.segment_count, b.cap_m, (uint32_t)b.host.backdrops.size()});
     if ((s = dalloc(ctx, &b.d_scan, pfh::scan_partials_words(scan_n))) != PFH_OK) return s;
     return PFH_OK;
@@ -362,6 +367,8 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.fills = b.d_fills;
     a.fill_tiles = b.d_fill_tiles;
     a.fill_capacity = b.cap_f;
+    a.sorted_fills = b.d_sorted_fills;
+    a.tile_runs = b.d_tile_runs;
     a.tiles = b.d_tiles;
     a.col_alpha_offsets = b.d_col_alpha;
     a.alpha_tiles = b.d_alpha;
@@ -474,6 +481,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_backdrops, h.backdrops.size())) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_alpha, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_zbuffer, fbn + 8)) != PFH_OK) return s;
