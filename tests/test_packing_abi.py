@@ -1,0 +1,131 @@
+"""The C++ scene front-end against the packing restatement, and the C ABI
+surface of libpfhip.so (loadable and complete without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import packing
+from pathfinder_amd import BuildOptions, Transform2F, synth
+from pathfinder_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pfhip.h")
+
+
+def _compare(scene, transform=None):
+    t = transform or (1, 0, 0, 1, 0, 0)
+    want = packing.build(scene, t)
+    cl = scene.build(BuildOptions(Transform2F(*t)))
+    d, c = cl.segments()
+    for got, exp in ((d, want["draw"]), (c, want["clip"])):
+        assert np.array_equal(got["points"], exp["points"])
+        assert np.array_equal(got["indices"], exp["indices"])
+    assert np.array_equal(cl.paints(), want["paints"])
+    got_b = cl.batches()
+    assert [k for k, _ in got_b] == [k for k, _ in want["batches"]]
+    for (_, gb), (_, wb) in zip(got_b, want["batches"]):
+        for key, gv in gb.items():
+            wv = wb[key]
+            if isinstance(gv, np.ndarray):
+                assert np.array_equal(gv, np.asarray(wv).astype(gv.dtype).reshape(gv.shape)), key
+            else:
+                assert gv == wv, key
+    return want
+
+
+@pytest.mark.parametrize("name", sorted(synth.tiny_scenes()))
+def test_packing_tiny(name):
+    scene, _, _ = synth.tiny_scenes()[name]
+    _compare(scene)
+
+
+def test_packing_small_and_transformed():
+    scene, _, _ = synth.small_scene()
+    _compare(scene)
+    _compare(scene, (0.8, 0.3, -0.25, 0.9, 100.5, -40.25))
+
+
+def test_packing_4k_subset():
+    scene, _, _ = synth.scene_4k(paths=1500, clips=60, clipped_paths=300)
+    cmds = _compare(scene)
+    kinds = [k for k, _ in cmds["batches"]]
+    assert kinds == [packing.PREPARE_CLIP, packing.DRAW]
+
+
+def test_packing_culls_paths_outside_view_box():
+    scene, _, _ = synth.tiny_scenes()["offscreen"]
+    from pathfinder_amd import PathArrays, ColorU
+    pid = scene.push_paint(ColorU(1, 2, 3, 255))
+    pts = np.asarray([(100, 100), (120, 100), (110, 130)], np.float32)
+    scene.push_draw_paths(PathArrays(pts, np.zeros(3, np.uint8), [3], [1], [pid]))
+    cmds = _compare(scene)
+    assert cmds["batches"][-1][1]["path_count"] == 1  # the second path is culled
+
+
+def test_segments_close_contours():
+    # SegmentsD3D11::add_path: a contour of k on-curve points gives k segments
+    # and its first point is appended (builder.rs:804-840)
+    scene, _, _ = synth.tiny_scenes()["square"]
+    d, _ = scene.build().segments()
+    assert len(d["indices"]) == 4
+    assert np.array_equal(d["points"][4], d["points"][0])
+    assert list(d["indices"][:, 1]) == [0, 0, 0, 0]
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pfh_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the binding declares exactly those
+    assert sorted(L.SIGNATURES) == names
+
+
+def test_struct_layouts_match_reference_sizes():
+    # byte sizes of the #[repr(C)] structs of renderer/src/gpu_data.rs
+    src = """#include "pfhip.h"
+    #include <stddef.h>
+    size_t sizes[] = {sizeof(pfh_tile), sizeof(pfh_fill), sizeof(pfh_microline),
+      sizeof(pfh_alpha_tile), sizeof(pfh_tile_path_info), sizeof(pfh_propagate_metadata),
+      sizeof(pfh_dice_metadata), sizeof(pfh_backdrop_info), sizeof(pfh_segment_indices)};
+    """
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        so = os.path.join(d, "s.so")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-shared", "-fPIC", "-I", os.path.join(ROOT, "include"), c,
+                        "-o", so], check=True)
+        lib = ctypes.CDLL(so)
+        sizes = (ctypes.c_size_t * 9).in_dll(lib, "sizes")
+        assert list(sizes) == [16, 12, 16, 8, 16, 48, 16, 12, 8]
+
+
+def test_no_device_is_reported_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    opts = L.ContextOptions()
+    opts.level = L.RENDERER_LEVEL_D3D11
+    st = L.load().pfh_context_create(ctypes.byref(opts), ctypes.byref(h))
+    assert st == 5  # PFH_ERROR_NO_DEVICE
+
+
+def test_null_arguments_are_rejected():
+    lib = L.load()
+    assert lib.pfh_begin_scene(None, None) == 1
+    assert lib.pfh_end_scene(None) == 1
+    assert lib.pfh_scene_build(None, None, None) == 1
+    assert lib.pfh_render_commands(None, None) == 1
