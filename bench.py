@@ -56,6 +56,24 @@ def algorithmic_bytes(st, fb_tiles, pixels, points, load_action):
     }
 
 
+def copy_calibration(nbytes):
+    """Three device-to-device copies of nbytes with the HIP runtime's own copy
+    kernel: a copy of known size for scaling memory counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print("bench: copy calibration: 3 device copies of %d bytes" % nbytes, file=sys.stderr,
+          flush=True)
+
+
 def cpu_baseline(cmds, size, bg, microlines):
     import oracle
     threads = min(16, os.cpu_count() or 1)
@@ -78,6 +96,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="4k", choices=["4k", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the frames, run a plain 256 MiB device-to-device copy so a "
+                         "counter run can scale FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -183,6 +204,8 @@ def main():
         cmds = packing.build(scene)
         line["cpu_baseline"] = cpu_baseline(cmds, size, bg, M)
     r.close()
+    if args.copy_calibration:
+        copy_calibration(256 << 20)
     print(json.dumps(line), flush=True)
 
 

@@ -62,7 +62,8 @@ struct BatchArgs {
     uint32_t *fill_tiles;          // tile index per fill
     uint32_t fill_capacity;
     uint2 *sorted_fills;           // fill words 0-1 sorted by (tile, fill index), fill_capacity
-    uint2 *tile_runs;              // per tile with fills: (first sorted index, count) or (0, 0)
+    uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)
+    uint2 *run_prev;               // at a run's first sorted index: the previous run or (0, 0)
     uint4 *tiles;                  // tile_count
     uint32_t *col_alpha_offsets;   // column_count + 1
     uint2 *alpha_tiles;            // tile_count

@@ -362,25 +362,82 @@ __global__ void k_finalize_count(uint32_t *total, uint32_t cap, uint32_t *clampe
 
 // ------------------------------------------------------------------ bound
 
-// bound.cs.glsl:49-84.
-__global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < a.tile_count;
-         t += gridDim.x * blockDim.x) {
-        if (a.path_count == 0) break;
-        uint32_t lo = 0, hi = a.path_count;
-        int it = 0;
-        while (it < 1024 && lo + 1 < hi) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            uint32_t mid_tile = a.tile_path_info[mid].z;
-            if (t < mid_tile) {
-                hi = mid;
-            } else {
-                lo = mid;
-                if (t == mid_tile) break;
-            }
-            it++;
+// The path search of bound.cs.glsl:53-66: binary search of the tile's path
+// by first tile index, stopping early when a probed path starts at the tile.
+__device__ __forceinline__ uint32_t reference_tile_search(const BatchArgs &a, uint32_t t) {
+    uint32_t lo = 0, hi = a.path_count;
+    int it = 0;
+    while (it < 1024 && lo + 1 < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        uint32_t mid_tile = a.tile_path_info[mid].z;
+        if (t < mid_tile) {
+            hi = mid;
+        } else {
+            lo = mid;
+            if (t == mid_tile) break;
         }
-        a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, a.tile_path_info[lo].w);
+        it++;
+    }
+    return lo;
+}
+
+// Largest path index whose first tile index is <= t.
+__device__ __forceinline__ uint32_t last_path_at_or_before(const BatchArgs &a, uint32_t t) {
+    uint32_t lo = 0, hi = a.path_count;
+    while (lo + 1 < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a.tile_path_info[mid].z <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// bound.cs.glsl:49-84. A workgroup initialises 1024 consecutive tiles: it
+// stages the first tile indices of the paths they belong to in LDS and looks
+// each tile up there. The reference search returns the same path, the last
+// one starting at or before the tile, except when it lands on an empty path
+// that starts at the tile itself; that case takes the reference search.
+constexpr uint32_t kBoundItems = 4;
+
+__global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
+    constexpr uint32_t kPer = kBlock * kBoundItems;
+    __shared__ uint32_t zs[kPer + 1];
+    __shared__ uint32_t qrange[2];
+    if (a.path_count == 0) return;
+    for (uint32_t b0 = blockIdx.x * kPer; b0 < a.tile_count; b0 += gridDim.x * kPer) {
+        uint32_t b1 = min(b0 + kPer, a.tile_count) - 1;
+        if (threadIdx.x == 0) {
+            qrange[0] = last_path_at_or_before(a, b0);
+            qrange[1] = last_path_at_or_before(a, b1);
+        }
+        __syncthreads();
+        uint32_t q0 = qrange[0], q1 = qrange[1];
+        bool staged = q1 - q0 < kPer;
+        if (staged)
+            for (uint32_t i = threadIdx.x; i <= q1 - q0; i += blockDim.x)
+                zs[i] = a.tile_path_info[q0 + i].z;
+        __syncthreads();
+        for (uint32_t k = 0; k < kBoundItems; k++) {
+            uint32_t t = b0 + k * kBlock + threadIdx.x;
+            if (t > b1) break;
+            uint32_t p;
+            if (staged) {
+                uint32_t lo = 0, hi = q1 - q0 + 1;
+                while (lo + 1 < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (zs[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                p = q0 + lo;
+                if (p > 0 && zs[lo] == t &&
+                    (lo > 0 ? zs[lo - 1] : a.tile_path_info[p - 1].z) == t)
+                    p = reference_tile_search(a, t);
+            } else {
+                p = reference_tile_search(a, t);
+            }
+            a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, a.tile_path_info[p].w);
+        }
+        __syncthreads();
     }
 }
 
@@ -528,13 +585,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_emit(BatchArgs a) {
 // one. A path's fills are contiguous and only touch the path's own tiles, so
 // one workgroup per path sorts its fills by (tile, fill index) in LDS chunks
 // and stitches the runs; chunks of one path run in order, so the exchange on
-// the tile head carries a list across chunks.
-// Paths with at most kLinkSmall fills sort in 16 KiB of LDS; larger ones in a
-// second launch with 64 KiB chunks, so the common case keeps occupancy.
-constexpr uint32_t kLinkSmall = 2048;
-constexpr uint32_t kLinkBig = 8192;
+// the tile head carries a list across chunks. The sorted order is also kept:
+// sorted_fills holds each run's fill words contiguously, tile_runs[t] the
+// tile's last run (first index, count; bit 31 of the count set when an
+// earlier run exists) and run_prev[first index of a run] the run before it.
+constexpr uint32_t kLinkChunk = 1024;
 
-template <uint32_t kLinkChunk, bool kBig>
 __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     __shared__ unsigned long long keys[kLinkChunk];
     uint32_t path = blockIdx.x;
@@ -547,10 +603,6 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     if (s1 < s0) s1 = s0;
     uint32_t m0 = min(a.seg_offsets[s0], mcount), m1 = min(a.seg_offsets[s1], mcount);
     uint32_t f0 = min(a.ml_fill_offsets[m0], fcount), f1 = min(a.ml_fill_offsets[m1], fcount);
-    if (kBig ? (f1 - f0 <= kLinkSmall) : (f1 - f0 > kLinkSmall)) return;
-    // A path sorted in one chunk leaves each tile's fills contiguous in
-    // sorted_fills; k_fill reads them from there instead of walking the list.
-    bool single = f1 - f0 <= kLinkChunk;
     for (uint32_t base = f0; base < f1; base += kLinkChunk) {
         uint32_t n = min(kLinkChunk, f1 - base);
         uint32_t size = 64;
@@ -590,14 +642,22 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
                 }
                 uint32_t last_f = (uint32_t)keys[lo];
                 link = atomicExch(&a.tiles[tile].y, last_f);
-                a.tile_runs[tile] = single ? make_uint2(base + i, lo - i + 1) : make_uint2(0, 0);
+                // bit 31 of the count: an earlier run of this tile exists
+                uint32_t count = (lo - i + 1) | (link != 0xffffffffu ? 0x80000000u : 0u);
+                unsigned long long run =
+                    ((unsigned long long)count << 32) | (unsigned long long)(base + i);
+                unsigned long long prev = atomicExch(
+                    reinterpret_cast<unsigned long long *>(&a.tile_runs[tile]), run);
+                // the tile's first run has no predecessor (tile_runs starts undefined)
+                a.run_prev[base + i] = link == 0xffffffffu
+                                           ? make_uint2(0, 0)
+                                           : make_uint2((uint32_t)prev, (uint32_t)(prev >> 32));
             } else {
                 link = (uint32_t)keys[i - 1];
             }
             a.fills[(size_t)f * 3 + 2] = link;
-            if (single)
-                a.sorted_fills[base + i] =
-                    make_uint2(a.fills[(size_t)f * 3 + 0], a.fills[(size_t)f * 3 + 1]);
+            a.sorted_fills[base + i] =
+                make_uint2(a.fills[(size_t)f * 3 + 0], a.fills[(size_t)f * 3 + 1]);
         }
         __syncthreads();
     }
@@ -767,13 +827,21 @@ __global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
 // keeps tiles with index >= z in ascending order and leaves culled tiles with
 // their pre-sort next. One wavefront walks the covering paths from the top
 // down, 64 at a time, and writes exactly those words.
-__global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
-    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
-    if (wave >= fbn) return;
+// A workgroup takes kSortBand framebuffer tiles of one row (a wavefront each)
+// and stages the row's path list in LDS 64 entries at a time for all of them.
+constexpr uint32_t kSortBand = 16;
+
+__global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
+    __shared__ int32_t s_miny[64], s_minx[64], s_maxx[64];
+    __shared__ uint32_t s_first[64], s_zw[64];
+    uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
+    int32_t y = (int32_t)(blockIdx.x / bands);
+    uint32_t band = blockIdx.x % bands;
+    if (y >= a.fb_height_tiles) return;
     uint32_t lane = lane_id();
-    int32_t x = (int32_t)(wave % (uint32_t)a.fb_width_tiles);
-    int32_t y = (int32_t)(wave / (uint32_t)a.fb_width_tiles);
+    int32_t x = (int32_t)(band * kSortBand + (threadIdx.x >> 6));
+    bool active = x < a.fb_width_tiles;
+    uint32_t wave = (uint32_t)y * (uint32_t)a.fb_width_tiles + (uint32_t)x;
     const uint32_t *list = a.row_paths + (size_t)y * a.path_count;
     uint32_t len = a.row_counts[y];
 
@@ -786,15 +854,22 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
     for (uint32_t end = len; end > 0;) {
         uint32_t begin = end > 64 ? end - 64 : 0;
         uint32_t k = begin + lane;
-        bool valid = k < end;
+        if (threadIdx.x < 64 && k < end) {
+            uint32_t p = list[k];
+            uint4 info = a.tile_path_info[p];
+            s_miny[lane] = (int32_t)(info.x) >> 16;
+            s_minx[lane] = (int32_t)(info.x << 16) >> 16;
+            s_maxx[lane] = (int32_t)(info.y << 16) >> 16;
+            s_first[lane] = info.z;
+            s_zw[lane] = a.prop_md[(size_t)p * 12 + 6] != 0;
+        }
+        __syncthreads();
+        bool valid = active && k < end;
         int32_t t = -1;
         bool linked = false, occl = false, alpha_tile = false;
         if (valid) {
-            uint32_t p = list[k];
-            uint4 info = a.tile_path_info[p];
-            int32_t miny = (int32_t)(info.x) >> 16;
-            int32_t minx = (int32_t)(info.x << 16) >> 16, maxx = (int32_t)(info.y << 16) >> 16;
-            uint32_t ti = info.z + (uint32_t)(y - miny) * (uint32_t)(maxx - minx) +
+            int32_t miny = s_miny[lane], minx = s_minx[lane], maxx = s_maxx[lane];
+            uint32_t ti = s_first[lane] + (uint32_t)(y - miny) * (uint32_t)(maxx - minx) +
                           (uint32_t)(x - minx);
             if (x >= minx && x < maxx && ti < a.tile_count) {
                 t = (int32_t)ti;
@@ -803,9 +878,10 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
                 int32_t bd = (int32_t)w.w >> 24;
                 linked = bd != 0 || alpha >= 0;
                 alpha_tile = alpha >= 0;
-                occl = linked && a.prop_md[(size_t)p * 12 + 6] != 0 && bd != 0 && alpha < 0;
+                occl = linked && s_zw[lane] != 0 && bd != 0 && alpha < 0;
             }
         }
+        __syncthreads();
         unsigned long long lmask = __ballot(linked);
         unsigned long long omask = __ballot(occl);
         unsigned long long amask = __ballot(alpha_tile);
@@ -855,7 +931,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_tiles(BatchArgs a) {
         }
         end = begin;
     }
-    if (lane == 0) {
+    if (lane == 0 && active) {
         if (pending >= 0) a.tiles[pending].x = 0xffffffffu;
         a.first_tile_map[wave] = lowest_survivor;
         a.zbuffer[8 + wave] = z_found ? z : 0;
@@ -916,77 +992,49 @@ __device__ __forceinline__ float4 coverage_with_slope(const float4 *__restrict__
     return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
 }
 
-// Coverage of one alpha tile whose fills sit at sorted_fills[top - k], k < n
-// (descending fill index, at most 1024): the list walk of fill_compute.inc.glsl
-// visits them in this order. A fill adds exactly +-0 to a pixel column its
-// window does not see (dX = 0: both ends at or left of the column's left edge,
-// or both at or right of its right edge) and the sum ends in abs(), so each
-// lane walks only the fills that touch its column, in the same order. `first`
-// holds the first 64 fills, loaded ahead by the caller.
-__device__ __forceinline__ float4 accumulate_run(const BatchArgs &a, uint32_t top, uint32_t n,
-                                                 uint2 first, uint32_t lane, float fragx,
-                                                 float fragy) {
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// Adds the coverage of up to 64 fills held one per lane (fill k in lane k, in
+// the order the list walk of fill_compute.inc.glsl visits them). A fill adds
+// exactly +-0 to a pixel column its window does not see (dX = 0: both ends at
+// or left of the column's left edge, or both at or right of its right edge)
+// and the sum ends in abs(), so each lane walks only the fills that touch its
+// column, in the same order.
+__device__ __forceinline__ void accumulate_chunk(const BatchArgs &a, uint2 fw, uint32_t n,
+                                                 uint32_t lane, float fragx, float fragy,
+                                                 float4 &acc) {
     uint32_t col = lane & 15u;
-    for (uint32_t c = 0; c < n; c += 64) {
-        uint32_t k = c + lane;
-        bool have = k < n;
-        uint2 fw = c == 0 ? first : (have ? a.sorted_fills[top - k] : make_uint2(0, 0));
-        uint32_t fxi = fw.x & 0xffffu, txi = fw.y & 0xffffu;
-        uint32_t lo = min(fxi, txi), hi = max(fxi, txi);
-        uint32_t c0 = lo >> 8, c1 = hi > 0 ? (hi - 1) >> 8 : 0;
-        bool any = have && hi > lo;
-        // d = (right.y - left.y) / (right.x - left.x) does not depend on the pixel:
-        // the pixel offset cancels exactly in both differences
-        bool lt = fxi < txi;
-        float lyf = (float)(lt ? (fw.x >> 16) : (fw.y >> 16)) / 256.0f;
-        float ryf = (float)(lt ? (fw.y >> 16) : (fw.x >> 16)) / 256.0f;
-        float dxw = (float)(hi - lo) / 256.0f;
-        float dfill = any ? (ryf - lyf) / dxw : 0.0f;
-        unsigned long long mine = 0;
+    bool have = lane < n;
+    uint32_t fxi = fw.x & 0xffffu, txi = fw.y & 0xffffu;
+    uint32_t lo = min(fxi, txi), hi = max(fxi, txi);
+    uint32_t c0 = lo >> 8, c1 = hi > 0 ? (hi - 1) >> 8 : 0;
+    bool any = have && hi > lo;
+    // d = (right.y - left.y) / (right.x - left.x) does not depend on the pixel:
+    // the pixel offset cancels exactly in both differences
+    bool lt = fxi < txi;
+    float lyf = (float)(lt ? (fw.x >> 16) : (fw.y >> 16)) / 256.0f;
+    float ryf = (float)(lt ? (fw.y >> 16) : (fw.x >> 16)) / 256.0f;
+    float dxw = (float)(hi - lo) / 256.0f;
+    float dfill = any ? (ryf - lyf) / dxw : 0.0f;
+    unsigned long long mine = 0;
 #pragma unroll
-        for (uint32_t q = 0; q < 16; q++) {
-            unsigned long long m = __ballot(any && c0 <= q && q <= c1);
-            mine = q == col ? m : mine;
-        }
-        while (__ballot(mine != 0ull)) {
-            bool act = mine != 0ull;
-            uint32_t j = act ? (uint32_t)(__ffsll((long long)mine) - 1) : 0u;
-            mine = act ? (mine & (mine - 1ull)) : mine;
-            uint32_t fw0 = __shfl(fw.x, j, 64), fw1 = __shfl(fw.y, j, 64);
-            float d = __shfl(dfill, j, 64);
-            if (act) {
-                float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
-                float sy = (float)(fw0 >> 16) / 256.0f - fragy;
-                float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
-                float ey = (float)(fw1 >> 16) / 256.0f - fragy;
-                float4 cv = coverage_with_slope(a.lut, sx, sy, ex, ey, d);
-                acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
-            }
+    for (uint32_t q = 0; q < 16; q++) {
+        unsigned long long m = __ballot(any && c0 <= q && q <= c1);
+        mine = q == col ? m : mine;
+    }
+    while (__ballot(mine != 0ull)) {
+        bool act = mine != 0ull;
+        uint32_t j = act ? (uint32_t)(__ffsll((long long)mine) - 1) : 0u;
+        mine = act ? (mine & (mine - 1ull)) : mine;
+        uint32_t fw0 = __shfl(fw.x, j, 64), fw1 = __shfl(fw.y, j, 64);
+        float d = __shfl(dfill, j, 64);
+        if (act) {
+            float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
+            float sy = (float)(fw0 >> 16) / 256.0f - fragy;
+            float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
+            float ey = (float)(fw1 >> 16) / 256.0f - fragy;
+            float4 cv = coverage_with_slope(a.lut, sx, sy, ex, ey, d);
+            acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
         }
     }
-    return acc;
-}
-
-// The reference's list walk (fill_compute.inc.glsl:11-25), for tiles of paths
-// too large to sort in one chunk.
-__device__ float4 accumulate_list(const BatchArgs &a, int32_t f, float fragx, float fragy) {
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    int it = 0;
-    do {
-        if ((uint32_t)f >= a.fill_capacity) break;
-        uint32_t fw0 = a.fills[(size_t)f * 3 + 0], fw1 = a.fills[(size_t)f * 3 + 1];
-        int32_t link = (int32_t)a.fills[(size_t)f * 3 + 2];
-        float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
-        float sy = (float)(fw0 >> 16) / 256.0f - fragy;
-        float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
-        float ey = (float)(fw1 >> 16) / 256.0f - fragy;
-        float4 cv = compute_coverage(a.lut, sx, sy, ex, ey);
-        acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
-        f = link;
-        it++;
-    } while (f >= 0 && it < 1024);
-    return acc;
 }
 
 // fill.cs.glsl:61-88. One wavefront per alpha tile; lane (lx, ly) owns the
@@ -1005,14 +1053,34 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
         uint4 tile = a.tiles[at.x];
         if (sext24(tile.z) < 0) continue;
         int32_t backdrop = (int32_t)tile.w >> 24;
+        // the tile's fills from the highest index down, at most 1024: its runs
+        // in sorted_fills, latest first, gathered 64 at a time
         uint2 run = a.tile_runs[at.x];
-        float4 acc;
-        if (run.y != 0) {
-            uint32_t top = run.x + run.y - 1, n = min(run.y, 1024u);
-            uint2 first = lane < n ? a.sorted_fills[top - lane] : make_uint2(0, 0);
-            acc = accumulate_run(a, top, n, first, lane, fragx, fragy);
-        } else {
-            acc = accumulate_list(a, (int32_t)tile.y, fragx, fragy);
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t budget = 1024, used = 0;
+        uint32_t rn = run.y & 0x7fffffffu;
+        if (run.x + rn > a.fill_capacity) rn = 0;
+        while (budget != 0 && rn != 0) {
+            uint2 fw = make_uint2(0, 0);
+            uint32_t filled = 0;
+            while (filled < 64 && budget != 0 && rn != 0) {
+                uint32_t take = min(min(rn - used, 64u - filled), budget);
+                if (lane >= filled && lane < filled + take)
+                    fw = a.sorted_fills[run.x + rn - 1 - used - (lane - filled)];
+                filled += take;
+                used += take;
+                budget -= take;
+                if (used == rn) {
+                    used = 0;
+                    rn = 0;
+                    if (run.y & 0x80000000u) {
+                        run = a.run_prev[run.x];
+                        rn = run.y & 0x7fffffffu;
+                        if (run.x + rn > a.fill_capacity) rn = 0;
+                    }
+                }
+            }
+            accumulate_chunk(a, fw, filled, lane, fragx, fragy, acc);
         }
         float bd = (float)backdrop;
         float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
@@ -1186,7 +1254,8 @@ void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
 void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (a.tile_count == 0) return;
     Scope t(p, s, K_BOUND);
-    hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock * kBoundItems)), dim3(kBlock),
+                       0, s, a);
 }
 
 void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
@@ -1208,10 +1277,7 @@ void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
     if (a.path_count) {
         Scope t(p, s, K_FILL_LINK);
-        hipLaunchKernelGGL((k_fill_link<kLinkSmall, false>), dim3(a.path_count), dim3(kBlock), 0,
-                           s, a);
-        hipLaunchKernelGGL((k_fill_link<kLinkBig, true>), dim3(a.path_count), dim3(kBlock), 0, s,
-                           a);
+        hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
     }
 }
 
@@ -1242,7 +1308,9 @@ void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p) {
                            a);
     }
     Scope t(p, s, K_SORT_TILES);
-    hipLaunchKernelGGL(k_sort_tiles, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+    uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
+    hipLaunchKernelGGL(k_sort_tiles, dim3((uint32_t)a.fb_height_tiles * bands),
+                       dim3(kSortBand * 64), 0, s, a);
 }
 
 void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p) {

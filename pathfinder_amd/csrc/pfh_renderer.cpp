@@ -185,6 +185,7 @@ struct Batch {
     uint32_t *d_fills = nullptr;
     uint32_t *d_fill_tiles = nullptr;
     uint2 *d_sorted_fills = nullptr;
+    uint2 *d_run_prev = nullptr;
     uint2 *d_tile_runs = nullptr;
     uint4 *d_tiles = nullptr;
     uint32_t *d_col_alpha = nullptr;
@@ -274,7 +275,8 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
-    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_tile_runs);
+    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
+    dfree(ctx, b.d_tile_runs);
     dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
@@ -301,7 +303,7 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if (b.cap_m == ctx->cap_m && b.cap_f == ctx->cap_f && b.d_microlines) return PFH_OK;
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets);
     dfree(ctx, b.d_fills); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills);
-    dfree(ctx, b.d_scan);
+    dfree(ctx, b.d_run_prev); dfree(ctx, b.d_scan);
     b.cap_m = ctx->cap_m;
     b.cap_f = ctx->cap_f;
     pfh_status s;
@@ -310,6 +312,7 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if ((s = dalloc(ctx, &b.d_fills, (size_t)b.cap_f * 3)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_fill_tiles, b.cap_f)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_sorted_fills, b.cap_f)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_run_prev, b.cap_f)) != PFH_OK) return s;
     uint32_t scan_n = std::max({b.host.segment_count, b.cap_m, (uint32_t)b.host.backdrops.size()});
     if ((s = dalloc(ctx, &b.d_scan, pfh::scan_partials_words(scan_n))) != PFH_OK) return s;
     return PFH_OK;
@@ -369,6 +372,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.fill_capacity = b.cap_f;
     a.sorted_fills = b.d_sorted_fills;
     a.tile_runs = b.d_tile_runs;
+    a.run_prev = b.d_run_prev;
     a.tiles = b.d_tiles;
     a.col_alpha_offsets = b.d_col_alpha;
     a.alpha_tiles = b.d_alpha;
