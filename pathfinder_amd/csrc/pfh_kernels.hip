@@ -207,6 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_init(BatchArgs a) {
     for (uint32_t i = gid; i < fbn + 8; i += stride) a.zbuffer[i] = 0;
     for (uint32_t i = gid; i < fbn; i += stride) a.first_tile_map[i] = -1;
     for (uint32_t i = gid; i < a.column_count * 3; i += stride) a.backdrops[i] = a.backdrops_init[i];
+    for (uint32_t i = gid; i <= a.column_count; i += stride) a.col_alpha_offsets[i] = 0;
 }
 
 __global__ void k_frame_reset(FrameState *f) {
@@ -603,6 +604,10 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     if (s1 < s0) s1 = s0;
     uint32_t m0 = min(a.seg_offsets[s0], mcount), m1 = min(a.seg_offsets[s1], mcount);
     uint32_t f0 = min(a.ml_fill_offsets[m0], fcount), f1 = min(a.ml_fill_offsets[m1], fcount);
+    const int32_t *pm = a.prop_md + (size_t)path * 12;
+    bool unclipped = pm[7] < 0;
+    uint32_t width = (uint32_t)(pm[2] - pm[0]);
+    if (width == 0) width = 1;
     for (uint32_t base = f0; base < f1; base += kLinkChunk) {
         uint32_t n = min(kLinkChunk, f1 - base);
         uint32_t size = 64;
@@ -642,6 +647,13 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
                 }
                 uint32_t last_f = (uint32_t)keys[lo];
                 link = atomicExch(&a.tiles[tile].y, last_f);
+                // An unclipped path gives a tile an alpha tile exactly when it has
+                // fills (propagate.cs.glsl:130-131), so the column's alpha count is
+                // its number of tiles with fills.
+                if (link == 0xffffffffu && unclipped) {
+                    uint32_t local = tile - (uint32_t)pm[4];
+                    atomicAdd(&a.col_alpha_offsets[(uint32_t)pm[8] + local % width], 1u);
+                }
                 // bit 31 of the count: an earlier run of this tile exists
                 uint32_t count = (lo - i + 1) | (link != 0xffffffffu ? 0x80000000u : 0u);
                 unsigned long long run =
@@ -756,10 +768,15 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
     return count;
 }
 
+// Alpha tile counts of the columns of clipped paths (k_fill_link has counted
+// the others).
 __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
-         c += gridDim.x * blockDim.x)
+         c += gridDim.x * blockDim.x) {
+        uint32_t dpath = (uint32_t)a.backdrops[(size_t)c * 3 + 2];
+        if (dpath < a.path_count && a.prop_md[(size_t)dpath * 12 + 7] < 0) continue;
         a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
+    }
 }
 
 // Allocates the batch's alpha tile range in the frame (uFirstAlphaTileIndex,
@@ -830,10 +847,13 @@ __global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
 // A workgroup takes kSortBand framebuffer tiles of one row (a wavefront each)
 // and stages the row's path list in LDS 64 entries at a time for all of them.
 constexpr uint32_t kSortBand = 16;
+constexpr uint32_t kBandCap = 8192;  // band list entries held in LDS
 
 __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
     __shared__ int32_t s_miny[64], s_minx[64], s_maxx[64];
     __shared__ uint32_t s_first[64], s_zw[64];
+    __shared__ uint32_t s_band[kBandCap];
+    __shared__ uint32_t s_wave[kSortBand + 1];
     uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
     int32_t y = (int32_t)(blockIdx.x / bands);
     uint32_t band = blockIdx.x % bands;
@@ -844,6 +864,35 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
     uint32_t wave = (uint32_t)y * (uint32_t)a.fb_width_tiles + (uint32_t)x;
     const uint32_t *list = a.row_paths + (size_t)y * a.path_count;
     uint32_t len = a.row_counts[y];
+    // The row's paths that overlap this band, in order, compacted into LDS.
+    int32_t bx0 = (int32_t)(band * kSortBand), bx1 = bx0 + (int32_t)kSortBand;
+    uint32_t band_len = 0;
+    for (uint32_t base = 0; base < len; base += blockDim.x) {
+        uint32_t k = base + threadIdx.x;
+        bool keep = false;
+        uint32_t p = 0;
+        if (k < len) {
+            p = list[k];
+            uint4 info = a.tile_path_info[p];
+            int32_t minx = (int32_t)(info.x << 16) >> 16, maxx = (int32_t)(info.y << 16) >> 16;
+            keep = minx < bx1 && maxx > bx0;
+        }
+        unsigned long long m = __ballot(keep);
+        if (lane == 0) s_wave[threadIdx.x >> 6] = __popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < kSortBand; w++) {
+            uint32_t c = s_wave[w];
+            before += w < (threadIdx.x >> 6) ? c : 0u;
+            total += c;
+        }
+        uint32_t pos = band_len + before + __popcll(m & ((1ull << lane) - 1ull));
+        if (keep && pos < kBandCap) s_band[pos] = p;
+        band_len += total;
+        __syncthreads();
+    }
+    bool in_lds = band_len <= kBandCap;
+    if (in_lds) len = band_len;
 
     bool z_found = false;
     int32_t z = 0;
@@ -855,7 +904,7 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
         uint32_t begin = end > 64 ? end - 64 : 0;
         uint32_t k = begin + lane;
         if (threadIdx.x < 64 && k < end) {
-            uint32_t p = list[k];
+            uint32_t p = in_lds ? s_band[k] : list[k];
             uint4 info = a.tile_path_info[p];
             s_miny[lane] = (int32_t)(info.x) >> 16;
             s_minx[lane] = (int32_t)(info.x << 16) >> 16;
@@ -1228,7 +1277,7 @@ void launch_frame_reset(FrameState *frame, hipStream_t s) {
 
 void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint64_t n = (uint64_t)a.fb_width_tiles * a.fb_height_tiles + 8;
-    if ((uint64_t)a.column_count * 3 > n) n = (uint64_t)a.column_count * 3;
+    if ((uint64_t)a.column_count * 3 + 1 > n) n = (uint64_t)a.column_count * 3 + 1;
     Scope t(p, s, K_BATCH_INIT);
     hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a);
 }
