@@ -990,19 +990,33 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
 
 // ------------------------------------------------------------------ fill
 
-// texture(uAreaLUT, (u, v)) with GL_LINEAR + CLAMP_TO_EDGE and fp32 weights.
-__device__ __forceinline__ float4 sample_lut(const float4 *__restrict__ lut, float u, float v) {
-    float s = u * 256.0f - 0.5f, t = v * 256.0f - 0.5f;
-    float fs = floorf(s), ft = floorf(t);
-    float wa = s - fs, wb = t - ft;
-    // CLAMP_TO_EDGE: texel indices clamp to [0, 255] (also for |fs| >= 2^31, where
-    // the reference's int conversion saturates first)
-    uint32_t i0 = (uint32_t)fminf(fmaxf(fs, 0.0f), 255.0f);
-    uint32_t i1 = (uint32_t)fminf(fmaxf(fs + 1.0f, 0.0f), 255.0f);
-    uint32_t j0 = (uint32_t)fminf(fmaxf(ft, 0.0f), 255.0f);
-    uint32_t j1 = (uint32_t)fminf(fmaxf(ft + 1.0f, 0.0f), 255.0f);
-    float4 t00 = lut[(j0 << 8) | i0], t10 = lut[(j0 << 8) | i1];
-    float4 t01 = lut[(j1 << 8) | i0], t11 = lut[(j1 << 8) | i1];
+// One fill in tile coordinates (pixels), the values every pixel of the tile
+// shares: both x ends in list order, the left end (smaller x) and its y, the
+// rise from the left to the right end, the width w = right.x - left.x, 1/w
+// rounded to nearest, and the slope rise / w.
+struct FillRecord {
+    float fx, tx, lx, ly;
+    float dy, w, rw, d;
+};
+
+// texture(uAreaLUT, .) at texel coordinates (s, u) = (256 x - 0.5, 256 y - 0.5)
+// with GL_LINEAR, CLAMP_TO_EDGE and fp32 weights; lut is 256 x 256 float4
+// texels, row-major in y.
+__device__ __forceinline__ float4 sample_lut_texel(const char *__restrict__ lut, float s,
+                                                   float u) {
+    float fs = floorf(s), fu = floorf(u);
+    float wa = s - fs, wb = u - fu;
+    // |s|, |u| < 2^21 for every packed fill, so the integer conversion is exact;
+    // CLAMP_TO_EDGE clamps each texel index to [0, 255]
+    int32_t is = (int32_t)fs, iu = (int32_t)fu;
+    uint32_t i0 = (uint32_t)min(max(is, 0), 255) << 4;
+    uint32_t i1 = (uint32_t)min(max(is + 1, 0), 255) << 4;
+    uint32_t j0 = (uint32_t)min(max(iu, 0), 255) << 12;
+    uint32_t j1 = (uint32_t)min(max(iu + 1, 0), 255) << 12;
+    float4 t00 = *(const float4 *)(lut + (j0 | i0));
+    float4 t10 = *(const float4 *)(lut + (j0 | i1));
+    float4 t01 = *(const float4 *)(lut + (j1 | i0));
+    float4 t11 = *(const float4 *)(lut + (j1 | i1));
     float ia = 1.0f - wa, ib = 1.0f - wb;
     float4 r;
     r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
@@ -1012,131 +1026,159 @@ __device__ __forceinline__ float4 sample_lut(const float4 *__restrict__ lut, flo
     return r;
 }
 
-// fill_area.inc.glsl:11-27 computeCoverage.
-__device__ __forceinline__ float4 compute_coverage(const float4 *__restrict__ lut, float fx,
-                                                   float fy, float tx, float ty) {
-    bool lt = fx < tx;
-    float lx = lt ? fx : tx, ly = lt ? fy : ty, rx = lt ? tx : fx, ry = lt ? ty : fy;
-    float wx = clamp_glsl(fx, -0.5f, 0.5f), wy = clamp_glsl(tx, -0.5f, 0.5f);
-    float offset = mixf(wx, wy, 0.5f) - lx;
-    float t = offset / (rx - lx);
-    float yy = mixf(ly, ry, t);
-    float d = (ry - ly) / (rx - lx);
+// computeCoverage (fill_area.inc.glsl:11-27) of one fill for the pixel column
+// centred at fragx and the four rows from fragy, bit-identical to the plain
+// statement of the function on pixel-relative coordinates:
+//  - packed coordinates are multiples of 1/256 below 256, so the differences
+//    right - left (x and y) and every pixel-relative coordinate are exact and
+//    the pixel offset cancels in w, dy and the slope;
+//  - the window middle and offset are exact; the products by 0.5 and 16 are
+//    exact, so fusing them into fma changes no rounding;
+//  - offset / w comes from the rounded reciprocal and one residual step,
+//    which returns the correctly rounded quotient for every w = k / 256,
+//    k in [1, 65535], and offset in [0, 2w] (checked for all of them); a
+//    column the fill touches always has offset in [0, w].
+__device__ __forceinline__ float4 coverage_record(const char *__restrict__ lut,
+                                                  const FillRecord &f, float fragx, float fragy) {
+    float wx = __builtin_amdgcn_fmed3f(f.fx - fragx, -0.5f, 0.5f);
+    float wy = __builtin_amdgcn_fmed3f(f.tx - fragx, -0.5f, 0.5f);
+    float offset = __builtin_fmaf(0.5f, wy - wx, wx) - (f.lx - fragx);
+    float q0 = offset * f.rw;
+    float t = __builtin_fmaf(__builtin_fmaf(-q0, f.w, offset), f.rw, q0);
+    float yy = (f.ly - fragy) + t * f.dy;
     float dX = wx - wy;
-    float4 s = sample_lut(lut, (yy + 8.0f) / 16.0f, fabsf(d * dX) / 16.0f);
+    // (yy + 8) / 16 * 256 - 0.5 and |d dX| / 16 * 256 - 0.5
+    float4 s = sample_lut_texel(lut, __builtin_fmaf(yy + 8.0f, 16.0f, -0.5f),
+                                __builtin_fmaf(fabsf(f.d * dX), 16.0f, -0.5f));
     return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
 }
 
-// computeCoverage with the slope d precomputed (it is the same for every pixel).
-__device__ __forceinline__ float4 coverage_with_slope(const float4 *__restrict__ lut, float fx,
-                                                      float fy, float tx, float ty, float d) {
-    bool lt = fx < tx;
-    float lx = lt ? fx : tx, ly = lt ? fy : ty, rx = lt ? tx : fx, ry = lt ? ty : fy;
-    float wx = clamp_glsl(fx, -0.5f, 0.5f), wy = clamp_glsl(tx, -0.5f, 0.5f);
-    float offset = mixf(wx, wy, 0.5f) - lx;
-    float t = offset / (rx - lx);
-    float yy = mixf(ly, ry, t);
-    float dX = wx - wy;
-    float4 s = sample_lut(lut, (yy + 8.0f) / 16.0f, fabsf(d * dX) / 16.0f);
-    return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
-}
-
-// Adds the coverage of up to 64 fills held one per lane (fill k in lane k, in
-// the order the list walk of fill_compute.inc.glsl visits them). A fill adds
-// exactly +-0 to a pixel column its window does not see (dX = 0: both ends at
-// or left of the column's left edge, or both at or right of its right edge)
-// and the sum ends in abs(), so each lane walks only the fills that touch its
-// column, in the same order.
-__device__ __forceinline__ void accumulate_chunk(const BatchArgs &a, uint2 fw, uint32_t n,
-                                                 uint32_t lane, float fragx, float fragy,
+// Adds the coverage of n <= 64 fills held one per lane (fill k in lane k, in
+// the order of the list walk of fill_compute.inc.glsl) to this lane's pixels.
+// A fill adds exactly +-0 to a pixel column its window does not see (dX = 0:
+// both ends at or left of the column's left edge, or both at or right of its
+// right edge) and the sum ends in abs(), so each lane walks only the fills
+// that touch its column, in list order. rec is this wavefront's 64 records
+// in LDS.
+__device__ __forceinline__ void accumulate_chunk(const char *__restrict__ lut, FillRecord *rec,
+                                                 uint2 fw, uint32_t n, uint32_t lane,
+                                                 uint32_t col, float fragx, float fragy,
                                                  float4 &acc) {
-    uint32_t col = lane & 15u;
-    bool have = lane < n;
-    uint32_t fxi = fw.x & 0xffffu, txi = fw.y & 0xffffu;
-    uint32_t lo = min(fxi, txi), hi = max(fxi, txi);
-    uint32_t c0 = lo >> 8, c1 = hi > 0 ? (hi - 1) >> 8 : 0;
-    bool any = have && hi > lo;
-    // d = (right.y - left.y) / (right.x - left.x) does not depend on the pixel:
-    // the pixel offset cancels exactly in both differences
-    bool lt = fxi < txi;
-    float lyf = (float)(lt ? (fw.x >> 16) : (fw.y >> 16)) / 256.0f;
-    float ryf = (float)(lt ? (fw.y >> 16) : (fw.x >> 16)) / 256.0f;
-    float dxw = (float)(hi - lo) / 256.0f;
-    float dfill = any ? (ryf - lyf) / dxw : 0.0f;
-    unsigned long long mine = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < 16; q++) {
-        unsigned long long m = __ballot(any && c0 <= q && q <= c1);
-        mine = q == col ? m : mine;
+    uint32_t x0 = fw.x & 0xffffu, x1 = fw.y & 0xffffu;
+    uint32_t lo = min(x0, x1), hi = max(x0, x1);
+    bool any = lane < n && hi > lo;
+    // columns c0..c1 of the tile (clipped to 15) see the fill
+    uint32_t c0 = lo >> 8, c1 = min(any ? (hi - 1u) >> 8 : 0u, 15u);
+    uint32_t cm = (any && c0 <= 15u) ? (2u << c1) - (1u << c0) : 0u;
+    if (lane < n) {
+        bool lt = x0 < x1;
+        uint32_t y0 = fw.x >> 16, y1 = fw.y >> 16;
+        FillRecord f;
+        f.fx = (float)x0 * (1.0f / 256.0f);
+        f.tx = (float)x1 * (1.0f / 256.0f);
+        f.lx = (float)lo * (1.0f / 256.0f);
+        f.ly = (float)(lt ? y0 : y1) * (1.0f / 256.0f);
+        float ry = (float)(lt ? y1 : y0) * (1.0f / 256.0f);
+        f.dy = ry - f.ly;
+        f.w = (float)(hi - lo) * (1.0f / 256.0f);
+        f.rw = any ? 1.0f / f.w : 0.0f;
+        f.d = any ? f.dy / f.w : 0.0f;
+        rec[lane] = f;
     }
-    while (__ballot(mine != 0ull)) {
-        bool act = mine != 0ull;
-        uint32_t j = act ? (uint32_t)(__ffsll((long long)mine) - 1) : 0u;
-        mine = act ? (mine & (mine - 1ull)) : mine;
-        uint32_t fw0 = __shfl(fw.x, j, 64), fw1 = __shfl(fw.y, j, 64);
-        float d = __shfl(dfill, j, 64);
-        if (act) {
-            float sx = (float)(fw0 & 0xffffu) / 256.0f - fragx;
-            float sy = (float)(fw0 >> 16) / 256.0f - fragy;
-            float ex = (float)(fw1 & 0xffffu) / 256.0f - fragx;
-            float ey = (float)(fw1 >> 16) / 256.0f - fragy;
-            float4 cv = coverage_with_slope(a.lut, sx, sy, ex, ey, d);
-            acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
+    // the records are read by other lanes of this wavefront only
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long mine = 0;
+    if (n <= 16) {
+        for (uint32_t k = 0; k < n; k++) {
+            uint32_t m = __builtin_amdgcn_readlane(cm, k);
+            mine |= (unsigned long long)((m >> col) & 1u) << k;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < 16; q++) {
+            unsigned long long m = __ballot((cm >> q) & 1u);
+            mine = q == col ? m : mine;
         }
     }
+    while (mine != 0ull) {
+        uint32_t j = (uint32_t)__ffsll((long long)mine) - 1u;
+        mine &= mine - 1ull;
+        float4 cv = coverage_record(lut, rec[j], fragx, fragy);
+        acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
+    }
+    // every lane is done with the records before the next chunk rewrites them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // fill.cs.glsl:61-88. One wavefront per alpha tile; lane (lx, ly) owns the
 // four pixels of column lx, rows 4ly..4ly+3 = the four channels of mask texel
 // (lx, ly).
 __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
+    __shared__ FillRecord s_rec[kBlock];
     uint32_t lane = lane_id();
+    FillRecord *rec = s_rec + (threadIdx.x & ~63u);
+    const char *lut = (const char *)a.lut;
     uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-    uint32_t n_alpha = a.counters->alpha;
-    uint32_t start = a.counters->alpha_start;
-    float fragx = (float)(lane & 15u) + 0.5f, fragy = (float)((lane >> 4) * 4u) + 0.5f;
-    for (uint32_t ba = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; ba < n_alpha; ba += waves) {
+    uint32_t n_alpha = uniform(a.counters->alpha);
+    uint32_t start = uniform(a.counters->alpha_start);
+    uint32_t col = lane & 15u;
+    float fragx = (float)col + 0.5f, fragy = (float)((lane >> 4) * 4u) + 0.5f;
+    for (uint32_t ba = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6); ba < n_alpha;
+         ba += waves) {
         uint32_t alpha_index = start + ba;
         if (alpha_index >= a.mask_capacity) break;
         uint2 at = a.alpha_tiles[ba];
-        uint4 tile = a.tiles[at.x];
-        if (sext24(tile.z) < 0) continue;
-        int32_t backdrop = (int32_t)tile.w >> 24;
+        uint32_t tile_index = uniform(at.x);
+        int32_t clip = (int32_t)uniform(at.y);
+        uint4 tile = a.tiles[tile_index];
+        uint32_t tz = uniform(tile.z), tw = uniform(tile.w);
+        if (sext24(tz) < 0) continue;
+        int32_t backdrop = (int32_t)tw >> 24;
         // the tile's fills from the highest index down, at most 1024: its runs
         // in sorted_fills, latest first, gathered 64 at a time
-        uint2 run = a.tile_runs[at.x];
+        uint2 run = a.tile_runs[tile_index];
+        uint32_t rx = uniform(run.x), ry = uniform(run.y);
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t budget = 1024, used = 0;
-        uint32_t rn = run.y & 0x7fffffffu;
-        if (run.x + rn > a.fill_capacity) rn = 0;
+        uint32_t rn = ry & 0x7fffffffu;
+        if (rx + rn > a.fill_capacity) rn = 0;
         while (budget != 0 && rn != 0) {
             uint2 fw = make_uint2(0, 0);
             uint32_t filled = 0;
             while (filled < 64 && budget != 0 && rn != 0) {
                 uint32_t take = min(min(rn - used, 64u - filled), budget);
                 if (lane >= filled && lane < filled + take)
-                    fw = a.sorted_fills[run.x + rn - 1 - used - (lane - filled)];
+                    fw = a.sorted_fills[rx + rn - 1 - used - (lane - filled)];
                 filled += take;
                 used += take;
                 budget -= take;
                 if (used == rn) {
                     used = 0;
                     rn = 0;
-                    if (run.y & 0x80000000u) {
-                        run = a.run_prev[run.x];
-                        rn = run.y & 0x7fffffffu;
-                        if (run.x + rn > a.fill_capacity) rn = 0;
+                    if (ry & 0x80000000u) {
+                        uint2 prev = a.run_prev[rx];
+                        rx = uniform(prev.x);
+                        ry = uniform(prev.y);
+                        rn = ry & 0x7fffffffu;
+                        if (rx + rn > a.fill_capacity) rn = 0;
                     }
                 }
             }
-            accumulate_chunk(a, fw, filled, lane, fragx, fragy, acc);
+            accumulate_chunk(lut, rec, fw, filled, lane, col, fragx, fragy, acc);
         }
         float bd = (float)backdrop;
         float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
         float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
         float c2 = clamp_glsl(fabsf(bd + acc.z), 0.0f, 1.0f);
         float c3 = clamp_glsl(fabsf(bd + acc.w), 0.0f, 1.0f);
-        int32_t clip = (int32_t)at.y;
         if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
             uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
             c0 = min_glsl(c0, (float)(ct & 0xffu) / 255.0f);
