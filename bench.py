@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "4K frame ms & Msegments/s vs llvmpipe ref; fwd+bwd/fwd ratio at ≤1 LSB"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per kernel per frame from two rocprofv3 counter runs of this
+# benchmark (FETCH_SIZE, WRITE_SIZE), scaled by a device copy of known size
+TRAFFIC_TABLE = {"4k": os.path.join(ROOT, "profiles", "traffic_4k.md")}
 
 
 def algorithmic_bytes(st, fb_tiles, pixels, points, load_action):
@@ -153,6 +156,7 @@ def main():
     dt = time.perf_counter() - t0
     prof_t = r.profile_read()[dominant]
     r.profile(L.PROFILE_OFF)
+    launches = prof_t[0] / args.steps
 
     ms_per_step = dt / args.steps * 1e3
     M = st["microline_count"]
@@ -163,6 +167,9 @@ def main():
     k_ms = prof_t[1] / args.steps
     bytes_k = alg.get(dominant)
     achieved = bytes_k / (k_ms * 1e-3) / 1e9 if bytes_k else None
+    from pathfinder_amd import profiling
+    measured = profiling.read_traffic(TRAFFIC_TABLE.get(args.workload, ""))
+    traffic = measured.get("k_" + dominant)
 
     print("bench: first frame %.1f ms (with buffer growth), frame with uploads %.2f ms, "
           "build %.1f ms" % (t_first * 1e3, t_upload_frame * 1e3, t_build * 1e3),
@@ -195,9 +202,15 @@ def main():
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None,
+                     "traffic": traffic[3] if traffic else None,
+                     "scope": "all launches of the kernel in one frame (%g per frame); "
+                              "achieved = algorithmic bytes / their summed duration"
+                              % launches,
                      "kernel_ms_per_frame": round(k_ms, 4),
-                     "algorithmic_bytes_per_frame": bytes_k},
+                     "kernel_us_per_launch": round(k_ms * 1e3 / max(launches, 1), 2),
+                     "algorithmic_bytes_per_frame": bytes_k,
+                     "traffic_source": (os.path.relpath(TRAFFIC_TABLE[args.workload], ROOT)
+                                        if traffic else None)},
     }
     if not args.no_cpu_baseline:
         from oracle import packing
