@@ -1053,6 +1053,60 @@ __device__ __forceinline__ float4 coverage_record(const char *__restrict__ lut,
     return make_float4(s.x * dX, s.y * dX, s.z * dX, s.w * dX);
 }
 
+// Orders this wavefront's LDS accesses: what one lane wrote before the call
+// is visible to every lane of the wavefront after it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+// The pixel columns c0..c1 of the tile (0..15) that a fill's window sees; an
+// empty range (c0 > c1) when it sees none.
+struct FillColumns {
+    uint32_t c0, c1;
+};
+
+// Builds the record of packed fill fw (Fill words 0-1) and its column range.
+__device__ __forceinline__ FillColumns make_record(uint2 fw, FillRecord &f) {
+    uint32_t x0 = fw.x & 0xffffu, x1 = fw.y & 0xffffu;
+    uint32_t lo = min(x0, x1), hi = max(x0, x1);
+    bool any = hi > lo;
+    bool lt = x0 < x1;
+    uint32_t y0 = fw.x >> 16, y1 = fw.y >> 16;
+    f.fx = (float)x0 * (1.0f / 256.0f);
+    f.tx = (float)x1 * (1.0f / 256.0f);
+    f.lx = (float)lo * (1.0f / 256.0f);
+    f.ly = (float)(lt ? y0 : y1) * (1.0f / 256.0f);
+    float ry = (float)(lt ? y1 : y0) * (1.0f / 256.0f);
+    f.dy = ry - f.ly;
+    f.w = (float)(hi - lo) * (1.0f / 256.0f);
+    f.rw = any ? 1.0f / f.w : 0.0f;
+    f.d = any ? f.dy / f.w : 0.0f;
+    FillColumns c;
+    c.c0 = lo >> 8;
+    c.c1 = any ? min((hi - 1u) >> 8, 15u) : 0u;
+    if (!any || c.c0 > 15u) {
+        c.c0 = 1u;
+        c.c1 = 0u;
+    }
+    return c;
+}
+
 // Adds the coverage of n <= 64 fills held one per lane (fill k in lane k, in
 // the order of the list walk of fill_compute.inc.glsl) to this lane's pixels.
 // A fill adds exactly +-0 to a pixel column its window does not see (dX = 0:
@@ -1064,31 +1118,12 @@ __device__ __forceinline__ void accumulate_chunk(const char *__restrict__ lut, F
                                                  uint2 fw, uint32_t n, uint32_t lane,
                                                  uint32_t col, float fragx, float fragy,
                                                  float4 &acc) {
-    uint32_t x0 = fw.x & 0xffffu, x1 = fw.y & 0xffffu;
-    uint32_t lo = min(x0, x1), hi = max(x0, x1);
-    bool any = lane < n && hi > lo;
-    // columns c0..c1 of the tile (clipped to 15) see the fill
-    uint32_t c0 = lo >> 8, c1 = min(any ? (hi - 1u) >> 8 : 0u, 15u);
-    uint32_t cm = (any && c0 <= 15u) ? (2u << c1) - (1u << c0) : 0u;
-    if (lane < n) {
-        bool lt = x0 < x1;
-        uint32_t y0 = fw.x >> 16, y1 = fw.y >> 16;
-        FillRecord f;
-        f.fx = (float)x0 * (1.0f / 256.0f);
-        f.tx = (float)x1 * (1.0f / 256.0f);
-        f.lx = (float)lo * (1.0f / 256.0f);
-        f.ly = (float)(lt ? y0 : y1) * (1.0f / 256.0f);
-        float ry = (float)(lt ? y1 : y0) * (1.0f / 256.0f);
-        f.dy = ry - f.ly;
-        f.w = (float)(hi - lo) * (1.0f / 256.0f);
-        f.rw = any ? 1.0f / f.w : 0.0f;
-        f.d = any ? f.dy / f.w : 0.0f;
-        rec[lane] = f;
-    }
-    // the records are read by other lanes of this wavefront only
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    FillRecord f;
+    FillColumns fc = make_record(fw, f);
+    bool have = lane < n;
+    uint32_t cm = (have && fc.c0 <= fc.c1) ? (2u << fc.c1) - (1u << fc.c0) : 0u;
+    if (have) rec[lane] = f;
+    wave_lds_sync();
     unsigned long long mine = 0;
     if (n <= 16) {
         for (uint32_t k = 0; k < n; k++) {
@@ -1109,85 +1144,263 @@ __device__ __forceinline__ void accumulate_chunk(const char *__restrict__ lut, F
         acc.x += cv.x; acc.y += cv.y; acc.z += cv.z; acc.w += cv.w;
     }
     // every lane is done with the records before the next chunk rewrites them
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
 }
 
-__device__ __forceinline__ uint32_t uniform(uint32_t v) {
-    return __builtin_amdgcn_readfirstlane(v);
+// byte / 255 rounded to nearest, from the rounded reciprocal and one residual
+// step (equal to the IEEE quotient for every byte).
+__device__ __forceinline__ float byte_to_unit(uint32_t c) {
+    const float r = 1.0f / 255.0f;
+    float x = (float)c;
+    float q0 = x * r;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 255.0f, x), r, q0);
 }
 
-// fill.cs.glsl:61-88. One wavefront per alpha tile; lane (lx, ly) owns the
-// four pixels of column lx, rows 4ly..4ly+3 = the four channels of mask texel
-// (lx, ly).
+// fill.cs.glsl:80-87: backdrop + coverage, abs, clamp, min with the clip
+// mask texel, unorm8 store of the four rows of this lane's mask texel.
+__device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_index,
+                                           int32_t backdrop, int32_t clip, uint32_t lane,
+                                           float4 acc) {
+    float bd = (float)backdrop;
+    float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
+    float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
+    float c2 = clamp_glsl(fabsf(bd + acc.z), 0.0f, 1.0f);
+    float c3 = clamp_glsl(fabsf(bd + acc.w), 0.0f, 1.0f);
+    if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
+        uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+        c0 = min_glsl(c0, byte_to_unit(ct & 0xffu));
+        c1 = min_glsl(c1, byte_to_unit((ct >> 8) & 0xffu));
+        c2 = min_glsl(c2, byte_to_unit((ct >> 16) & 0xffu));
+        c3 = min_glsl(c3, byte_to_unit(ct >> 24));
+    }
+    a.mask[(size_t)alpha_index * 64 + lane] =
+        unorm8(c0) | (unorm8(c1) << 8) | (unorm8(c2) << 16) | (unorm8(c3) << 24);
+}
+
+// The tile slot (0..63) of stream position q = pos + lane of a wavefront
+// group (the last tile i with S_i <= q, S = exclusive scan of the tiles' fill
+// counts n), that position's fill words, and whether it is its tile's last
+// fill; nothing is loaded past total.
+__device__ __forceinline__ void fill_lookup(const BatchArgs &a, uint32_t pos, uint32_t total,
+                                            uint32_t S, uint32_t rx, uint32_t n, uint32_t lane,
+                                            uint32_t &ts, uint2 &fw, bool &last) {
+    uint32_t q = pos + lane;
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t step = 32; step != 0; step >>= 1) {
+        uint32_t s = __shfl(S, t + step, 64);
+        if (s <= q) t += step;
+    }
+    uint32_t trx = __shfl(rx, t, 64), tn = __shfl(n, t, 64), tS = __shfl(S, t, 64);
+    ts = t;
+    last = q + 1u == tS + tn;
+    fw = make_uint2(0, 0);
+    if (q < total) fw = a.sorted_fills[trx + tn - 1u - (q - tS)];
+}
+
+constexpr uint32_t kFillGroup = 64;     // alpha tiles per wavefront group
+constexpr uint32_t kFillPairCap = 48;   // (fill, pixel column) pairs per round
+
+// fill.cs.glsl:61-88 with fill_compute.inc.glsl:11-25. Lane (lx, ly) of a
+// wavefront owns the four pixels of column lx, rows 4ly..4ly+3 of an alpha
+// tile (the four channels of mask texel (lx, ly)).
+//
+// A wavefront takes 64 consecutive alpha tiles and streams their fills: each
+// tile's fills from the highest index down (its run in sorted_fills), tile
+// after tile, 64 at a time. Lane k turns fill k of a chunk into a record and
+// the chunk's (fill, pixel column) pairs are numbered in stream order. Round
+// by round (kFillPairCap pairs), the wavefront evaluates computeCoverage for
+// every (pair, row group) into LDS, 64 at a time; then each lane adds, in
+// stream order, the values of the round's fills that touch its column (a
+// fill adds exactly +-0 to a column it does not touch, and the sum ends in
+// abs()), storing a tile's mask when the stream moves on to the next tile.
+// A pixel meets each fill in one pair only, so its sum runs over the same
+// fills in the same order as the list walk.
+// Tiles whose fills are split over several runs (rare) take the list walk
+// one chunk at a time instead (accumulate_chunk).
 __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
-    __shared__ FillRecord s_rec[kBlock];
-    uint32_t lane = lane_id();
-    FillRecord *rec = s_rec + (threadIdx.x & ~63u);
+    __shared__ FillRecord s_rec[kBlock / 64][64];
+    __shared__ float4 s_res[kBlock / 64][4 * kFillPairCap];
+    __shared__ int32_t s_cadj[kBlock / 64][64];
+    __shared__ uint8_t s_pair[kBlock / 64][64 * 16];
+    uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+    FillRecord *rec = s_rec[wv];
+    float4 *res = s_res[wv];
+    int32_t *cadj = s_cadj[wv];
+    uint8_t *pair_fill = s_pair[wv];
     const char *lut = (const char *)a.lut;
-    uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     uint32_t n_alpha = uniform(a.counters->alpha);
     uint32_t start = uniform(a.counters->alpha_start);
-    uint32_t col = lane & 15u;
-    float fragx = (float)col + 0.5f, fragy = (float)((lane >> 4) * 4u) + 0.5f;
-    for (uint32_t ba = uniform((blockIdx.x * blockDim.x + threadIdx.x) >> 6); ba < n_alpha;
-         ba += waves) {
-        uint32_t alpha_index = start + ba;
-        if (alpha_index >= a.mask_capacity) break;
-        uint2 at = a.alpha_tiles[ba];
-        uint32_t tile_index = uniform(at.x);
-        int32_t clip = (int32_t)uniform(at.y);
-        uint4 tile = a.tiles[tile_index];
-        uint32_t tz = uniform(tile.z), tw = uniform(tile.w);
-        if (sext24(tz) < 0) continue;
-        int32_t backdrop = (int32_t)tw >> 24;
-        // the tile's fills from the highest index down, at most 1024: its runs
-        // in sorted_fills, latest first, gathered 64 at a time
-        uint2 run = a.tile_runs[tile_index];
-        uint32_t rx = uniform(run.x), ry = uniform(run.y);
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t budget = 1024, used = 0;
+    uint32_t col = lane & 15u, grp = lane >> 4;
+    float fragx = (float)col + 0.5f, fragy = (float)(grp * 4u) + 0.5f;
+    uint32_t stride = ((gridDim.x * blockDim.x) >> 6) * kFillGroup;
+    for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kFillGroup);
+         g0 < n_alpha; g0 += stride) {
+        // lane i: alpha tile g0 + i
+        uint32_t ba = g0 + lane;
+        bool valid = ba < n_alpha && start + ba < a.mask_capacity;
+        uint32_t t_clip = 0xffffffffu, t_w = 0, rx = 0, ry = 0;
+        if (valid) {
+            uint2 at = a.alpha_tiles[ba];
+            t_clip = at.y;
+            uint2 zw = ((const uint2 *)(a.tiles + at.x))[1];
+            valid = sext24(zw.x) >= 0;
+            t_w = zw.y;
+            if (valid) {
+                uint2 run = a.tile_runs[at.x];
+                rx = run.x;
+                ry = run.y;
+            }
+        }
         uint32_t rn = ry & 0x7fffffffu;
         if (rx + rn > a.fill_capacity) rn = 0;
-        while (budget != 0 && rn != 0) {
-            uint2 fw = make_uint2(0, 0);
-            uint32_t filled = 0;
-            while (filled < 64 && budget != 0 && rn != 0) {
-                uint32_t take = min(min(rn - used, 64u - filled), budget);
-                if (lane >= filled && lane < filled + take)
-                    fw = a.sorted_fills[rx + rn - 1 - used - (lane - filled)];
-                filled += take;
-                used += take;
-                budget -= take;
-                if (used == rn) {
-                    used = 0;
-                    rn = 0;
-                    if (ry & 0x80000000u) {
-                        uint2 prev = a.run_prev[rx];
-                        rx = uniform(prev.x);
-                        ry = uniform(prev.y);
-                        rn = ry & 0x7fffffffu;
-                        if (rx + rn > a.fill_capacity) rn = 0;
+        bool chained = valid && rn != 0 && (ry & 0x80000000u) != 0u;
+        uint32_t n = (valid && !chained) ? min(rn, 1024u) : 0u;
+        uint32_t incl = wave_inclusive_scan(n);
+        uint32_t S = incl - n;
+        uint32_t total = uniform(__shfl(incl, 63, 64));
+
+        int32_t cur = -1;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t ts;
+        uint2 fw;
+        bool tlast;
+        fill_lookup(a, 0u, total, S, rx, n, lane, ts, fw, tlast);
+        for (uint32_t pos = 0; pos < total; pos += 64u) {
+            bool have = pos + lane < total;
+            uint32_t nf = min(total - pos, 64u);
+            FillRecord f;
+            FillColumns fc = make_record(fw, f);
+            // every fill takes at least one pair (a fill that sees no column
+            // gets a pair no lane reads), so each fill lies in some round
+            uint32_t span = have ? (fc.c0 <= fc.c1 ? fc.c1 - fc.c0 + 1u : 1u) : 0u;
+            uint32_t pe = wave_inclusive_scan(span);
+            uint32_t pb = pe - span;
+            uint32_t pairs = uniform(__shfl(pe, 63, 64));
+            if (have) {
+                rec[lane] = f;
+                cadj[lane] = (int32_t)fc.c0 - (int32_t)pb;
+                for (uint32_t j = 0; j < span; j++) pair_fill[pb + j] = (uint8_t)lane;
+            }
+            // the fills (bit k = fill k) whose window sees this lane's column
+            unsigned long long mine = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < 16; q++) {
+                unsigned long long m = __ballot(have && fc.c0 <= q && q <= fc.c1);
+                mine = q == col ? m : mine;
+            }
+            // a round may end after a fill that ends its tile or the chunk
+            bool cut = have && (tlast || lane == nf - 1u);
+            // the next chunk's fills load meanwhile
+            uint32_t nts;
+            uint2 nfw;
+            bool ntlast;
+            fill_lookup(a, pos + 64u, total, S, rx, n, lane, nts, nfw, ntlast);
+            wave_lds_sync();
+            for (uint32_t r0 = 0; r0 < pairs;) {
+                unsigned long long cuts = __ballot(cut && pe > r0 && pe <= r0 + kFillPairCap);
+                uint32_t r1 = cuts != 0ull
+                                  ? uniform(__shfl(pe, 63u - (uint32_t)__clzll((long long)cuts), 64))
+                                  : min(r0 + kFillPairCap, pairs);
+                // computeCoverage of every (pair, row group) of the round
+                for (uint32_t it = lane; it < 4u * (r1 - r0); it += 64u) {
+                    uint32_t p = r0 + (it >> 2), g = it & 3u;
+                    uint32_t k = pair_fill[p];
+                    float x = (float)((int32_t)p + cadj[k]) + 0.5f;
+                    res[it] = coverage_record(lut, rec[k], x, (float)(g * 4u) + 0.5f);
+                }
+                wave_lds_sync();
+                // the fills with a pair in [r0, r1), tile by tile in stream order;
+                // each lane adds the values of the fills that touch its column
+                uint32_t ka = (uint32_t)__popcll(__ballot(have && pe <= r0));
+                uint32_t kb = (uint32_t)__popcll(__ballot(have && pb < r1));
+                for (uint32_t k = ka; k < kb;) {
+                    int32_t t = (int32_t)__builtin_amdgcn_readlane(ts, k);
+                    unsigned long long seg = __ballot(have && (int32_t)ts == t);
+                    uint32_t ke = min(64u - (uint32_t)__clzll((long long)seg), kb);
+                    if (t != cur) {
+                        if (cur >= 0)
+                            store_mask(a, start + g0 + (uint32_t)cur,
+                                       (int32_t)__builtin_amdgcn_readlane(t_w, cur) >> 24,
+                                       (int32_t)__builtin_amdgcn_readlane(t_clip, cur), lane,
+                                       acc);
+                        cur = t;
+                        acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                    unsigned long long below = ke == 64u ? ~0ull : (1ull << ke) - 1ull;
+                    unsigned long long m = mine & below;
+                    while (m != 0ull) {
+                        uint32_t j = (uint32_t)__ffsll((long long)m) - 1u;
+                        uint32_t pp = col - (uint32_t)cadj[j];
+                        if (pp >= r1) break;
+                        m &= m - 1ull;
+                        float4 v = res[(pp - r0) * 4u + grp];
+                        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+                    }
+                    mine = (mine & ~below) | m;
+                    k = ke;
+                }
+                // every lane is done with this round's values
+                wave_lds_sync();
+                r0 = r1;
+            }
+            ts = nts;
+            fw = nfw;
+            tlast = ntlast;
+        }
+        if (cur >= 0)
+            store_mask(a, start + g0 + (uint32_t)cur,
+                       (int32_t)__builtin_amdgcn_readlane(t_w, cur) >> 24,
+                       (int32_t)__builtin_amdgcn_readlane(t_clip, cur), lane, acc);
+
+        // tiles without fills: backdrop (and clip) only
+        unsigned long long zero = __ballot(valid && !chained && n == 0u);
+        while (zero != 0ull) {
+            uint32_t i = (uint32_t)__ffsll((long long)zero) - 1u;
+            zero &= zero - 1ull;
+            store_mask(a, start + g0 + i, (int32_t)__builtin_amdgcn_readlane(t_w, i) >> 24,
+                       (int32_t)__builtin_amdgcn_readlane(t_clip, i), lane,
+                       make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        }
+
+        // tiles with chained runs: walk the runs latest first, at most 1024
+        // fills, 64 at a time
+        unsigned long long chain = __ballot(chained);
+        while (chain != 0ull) {
+            uint32_t i = (uint32_t)__ffsll((long long)chain) - 1u;
+            chain &= chain - 1ull;
+            uint32_t crx = __builtin_amdgcn_readlane(rx, i);
+            uint32_t cry = __builtin_amdgcn_readlane(ry, i);
+            uint32_t crn = __builtin_amdgcn_readlane(rn, i);
+            float4 cacc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t budget = 1024, used = 0;
+            while (budget != 0 && crn != 0) {
+                uint2 fw = make_uint2(0, 0);
+                uint32_t filled = 0;
+                while (filled < 64 && budget != 0 && crn != 0) {
+                    uint32_t take = min(min(crn - used, 64u - filled), budget);
+                    if (lane >= filled && lane < filled + take)
+                        fw = a.sorted_fills[crx + crn - 1 - used - (lane - filled)];
+                    filled += take;
+                    used += take;
+                    budget -= take;
+                    if (used == crn) {
+                        used = 0;
+                        crn = 0;
+                        if (cry & 0x80000000u) {
+                            uint2 prev = a.run_prev[crx];
+                            crx = uniform(prev.x);
+                            cry = uniform(prev.y);
+                            crn = cry & 0x7fffffffu;
+                            if (crx + crn > a.fill_capacity) crn = 0;
+                        }
                     }
                 }
+                accumulate_chunk(lut, rec, fw, filled, lane, col, fragx, fragy, cacc);
             }
-            accumulate_chunk(lut, rec, fw, filled, lane, col, fragx, fragy, acc);
+            store_mask(a, start + g0 + i, (int32_t)__builtin_amdgcn_readlane(t_w, i) >> 24,
+                       (int32_t)__builtin_amdgcn_readlane(t_clip, i), lane, cacc);
         }
-        float bd = (float)backdrop;
-        float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
-        float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
-        float c2 = clamp_glsl(fabsf(bd + acc.z), 0.0f, 1.0f);
-        float c3 = clamp_glsl(fabsf(bd + acc.w), 0.0f, 1.0f);
-        if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
-            uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
-            c0 = min_glsl(c0, (float)(ct & 0xffu) / 255.0f);
-            c1 = min_glsl(c1, (float)((ct >> 8) & 0xffu) / 255.0f);
-            c2 = min_glsl(c2, (float)((ct >> 16) & 0xffu) / 255.0f);
-            c3 = min_glsl(c3, (float)(ct >> 24) / 255.0f);
-        }
-        a.mask[(size_t)alpha_index * 64 + lane] =
-            unorm8(c0) | (unorm8(c1) << 8) | (unorm8(c2) << 16) | (unorm8(c3) << 24);
     }
 }
 

@@ -77,3 +77,37 @@ def test_planted_wrong_step_is_caught(tmp_path):
                     str(src), "-lm"], check=True)
     out = subprocess.run([str(exe), "4096"], check=True, capture_output=True, text=True).stdout
     assert int(out.split()[1]) > 0
+
+
+BYTE_SRC = r"""
+#include <math.h>
+#include <stdio.h>
+int main(void) {
+    int bad = 0;
+    const float r = 1.0f / 255.0f;
+    for (int c = 0; c < 256; c++) {
+        volatile float xv = (float)c;
+        float x = xv;
+        float q0 = x * r;
+        float q = fmaf(fmaf(-q0, 255.0f, x), r, q0);
+        bad += q != x / 255.0f;
+    }
+    printf("%d\n", bad);
+    return 0;
+}
+"""
+
+
+def test_byte_over_255_from_reciprocal(tmp_path):
+    """The fill kernel turns clip-mask bytes into floats with the same residual
+    step (fill.cs.glsl:84 loads the clip texel as byte / 255); all 256 bytes
+    must give the IEEE quotient."""
+    if not _has_fma():
+        pytest.skip("host has no hardware fma")
+    src = tmp_path / "byte.c"
+    src.write_text(BYTE_SRC)
+    exe = tmp_path / "byte"
+    subprocess.run(["gcc", "-O2", "-mfma", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"],
+                   check=True)
+    assert subprocess.run([str(exe)], check=True, capture_output=True,
+                          text=True).stdout.strip() == "0"
