@@ -60,6 +60,7 @@ struct BatchArgs {
     uint32_t *ml_fill_offsets;     // microline_capacity + 1
     uint32_t *fills;               // 3 words per fill, fill_capacity
     uint32_t *fill_tiles;          // tile index per fill
+    uint32_t *path_fills;          // path_count + 1: first fill of each path
     uint32_t fill_capacity;
     uint2 *sorted_fills;           // fill words 0-1 sorted by (tile, fill index), fill_capacity
     uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)

@@ -580,30 +580,45 @@ __global__ void __launch_bounds__(kBlock) k_bin_emit(BatchArgs a) {
         bin_microline<true>(a, a.microlines[m], a.ml_fill_offsets[m]);
 }
 
+// First fill of each path (path_count + 1 entries, the last one the fill
+// count): the fills of a path are those of its segments' microlines.
+__global__ void __launch_bounds__(kBlock) k_path_fills(BatchArgs a) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > a.path_count) return;
+    uint32_t mcount = a.counters->microline_count, fcount = a.counters->fill_count;
+    uint32_t seg = p < a.path_count ? a.dice_md[p].z : a.segment_count;
+    if (seg > a.segment_count) seg = a.segment_count;
+    uint32_t m = min(a.seg_offsets[seg], mcount);
+    a.path_fills[p] = min(a.ml_fill_offsets[m], fcount);
+}
+
 // Per-tile fill lists (bin.cs.glsl:113-117): the reference pushes each fill
 // onto its tile's list with atomicExchange, so in sequential order fill f
 // links to the previous fill of the same tile and the tile heads the last
 // one. A path's fills are contiguous and only touch the path's own tiles, so
 // one workgroup per path sorts its fills by (tile, fill index) in LDS chunks
-// and stitches the runs; chunks of one path run in order, so the exchange on
-// the tile head carries a list across chunks. The sorted order is also kept:
-// sorted_fills holds each run's fill words contiguously, tile_runs[t] the
-// tile's last run (first index, count; bit 31 of the count set when an
-// earlier run exists) and run_prev[first index of a run] the run before it.
+// and stitches the runs. No other workgroup touches the path's tiles and the
+// chunks run in order, so the tile head and last run are read and written
+// with plain loads and stores, ordered by the barrier between chunks. The
+// sorted order is also kept: sorted_fills holds each run's fill words
+// contiguously, tile_runs[t] the tile's last run (first index, count; bit 31
+// of the count set when an earlier run exists) and run_prev[first index of a
+// run] the run before it.
 constexpr uint32_t kLinkChunk = 1024;
 
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, uint32_t m) {
+    uint32_t lo = __shfl_xor((uint32_t)v, (int)m, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), (int)m, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
-    __shared__ unsigned long long keys[kLinkChunk];
+    __shared__ unsigned long long keys[2][kLinkChunk];
+    __shared__ uint2 words[kLinkChunk];
     uint32_t path = blockIdx.x;
     if (path >= a.path_count) return;
-    uint32_t mcount = a.counters->microline_count, fcount = a.counters->fill_count;
-    uint32_t s0 = a.dice_md[path].z;
-    uint32_t s1 = (path + 1 < a.path_count) ? a.dice_md[path + 1].z : a.segment_count;
-    if (s0 > a.segment_count) s0 = a.segment_count;
-    if (s1 > a.segment_count) s1 = a.segment_count;
-    if (s1 < s0) s1 = s0;
-    uint32_t m0 = min(a.seg_offsets[s0], mcount), m1 = min(a.seg_offsets[s1], mcount);
-    uint32_t f0 = min(a.ml_fill_offsets[m0], fcount), f1 = min(a.ml_fill_offsets[m1], fcount);
+    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
     const int32_t *pm = a.prop_md + (size_t)path * 12;
     bool unclipped = pm[7] < 0;
     uint32_t width = (uint32_t)(pm[2] - pm[0]);
@@ -612,41 +627,94 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
         uint32_t n = min(kLinkChunk, f1 - base);
         uint32_t size = 64;
         while (size < n) size <<= 1;
-        for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
-            keys[i] = i < n ? (((unsigned long long)a.fill_tiles[base + i] << 32) |
-                               (unsigned long long)(base + i))
-                            : ~0ull;
+        // Bitonic sort of the chunk's (tile, fill index) keys, padded to size.
+        // Key e lives in register r of lane l of wave w, e = r * 64W + w * 64
+        // + l with W = min(size / 64, 4) waves: partners at distance < 64 are
+        // exchanged across lanes, at distance >= 64W within the thread, and
+        // the distances in between through LDS (double-buffered, one
+        // barrier per step).
+        uint32_t lw = size >= 256 ? 2u : (size == 128 ? 1u : 0u);
+        uint32_t nw = 1u << lw, span = 64u << lw, nr = size >> (6 + lw);
+        bool active = wave < nw;
+        unsigned long long v[kLinkChunk / kBlock];
+#pragma unroll
+        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
+            uint32_t e = r * span + wave * 64u + lane;
+            v[r] = ~0ull;
+            if (active && r < nr && e < n) {
+                v[r] = ((unsigned long long)a.fill_tiles[base + e] << 32) |
+                       (unsigned long long)(base + e);
+                words[e] = make_uint2(a.fills[(size_t)(base + e) * 3 + 0],
+                                      a.fills[(size_t)(base + e) * 3 + 1]);
+            }
         }
-        __syncthreads();
+        uint32_t buf = 0;
         for (uint32_t k = 2; k <= size; k <<= 1) {
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < size; i += blockDim.x) {
-                    uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        unsigned long long x = keys[i], y = keys[ixj];
-                        bool up = (i & k) == 0;
-                        if ((x > y) == up) {
-                            keys[i] = y;
-                            keys[ixj] = x;
+                if (j < 64u) {
+#pragma unroll
+                    for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
+                        unsigned long long o = shfl_xor64(v[r], j);
+                        uint32_t e = r * span + wave * 64u + lane;
+                        bool keep_min = ((lane & j) == 0) == ((e & k) == 0);
+                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
+                    }
+                } else if (j < span) {
+                    if (active) {
+#pragma unroll
+                        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++)
+                            if (r < nr) keys[buf][r * span + wave * 64u + lane] = v[r];
+                    }
+                    __syncthreads();
+                    if (active) {
+#pragma unroll
+                        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
+                            if (r < nr) {
+                                uint32_t e = r * span + wave * 64u + lane;
+                                unsigned long long o = keys[buf][e ^ j];
+                                bool keep_min = ((e & j) == 0) == ((e & k) == 0);
+                                v[r] = keep_min ? min(v[r], o) : max(v[r], o);
+                            }
+                        }
+                    }
+                    buf ^= 1u;
+                } else {
+                    uint32_t rj = j / span;
+#pragma unroll
+                    for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
+                        if ((r & rj) == 0 && (r | rj) < kLinkChunk / kBlock) {
+                            uint32_t e = r * span + wave * 64u + lane;
+                            unsigned long long x = v[r], y = v[r | rj];
+                            bool up = (e & k) == 0;
+                            v[r] = up ? min(x, y) : max(x, y);
+                            v[r | rj] = up ? max(x, y) : min(x, y);
                         }
                     }
                 }
-                __syncthreads();
             }
         }
+        // the last LDS step read keys[buf ^ 1]; the sorted keys go to keys[buf]
+        unsigned long long *sk = keys[buf];
+        if (active) {
+#pragma unroll
+            for (uint32_t r = 0; r < kLinkChunk / kBlock; r++)
+                if (r < nr) sk[r * span + wave * 64u + lane] = v[r];
+        }
+        __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            unsigned long long cur = keys[i];
+            unsigned long long cur = sk[i];
             uint32_t tile = (uint32_t)(cur >> 32), f = (uint32_t)cur;
             uint32_t link;
-            if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) {
+            if (i == 0 || (uint32_t)(sk[i - 1] >> 32) != tile) {
                 uint32_t lo = i, hi = n;  // last index with this tile: upper bound - 1
                 while (lo + 1 < hi) {
                     uint32_t mid = (lo + hi) >> 1;
-                    if ((uint32_t)(keys[mid] >> 32) == tile) lo = mid;
+                    if ((uint32_t)(sk[mid] >> 32) == tile) lo = mid;
                     else hi = mid;
                 }
-                uint32_t last_f = (uint32_t)keys[lo];
-                link = atomicExch(&a.tiles[tile].y, last_f);
+                link = a.tiles[tile].y;
+                uint2 prev = a.tile_runs[tile];
+                a.tiles[tile].y = (uint32_t)sk[lo];
                 // An unclipped path gives a tile an alpha tile exactly when it has
                 // fills (propagate.cs.glsl:130-131), so the column's alpha count is
                 // its number of tiles with fills.
@@ -656,26 +724,22 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
                 }
                 // bit 31 of the count: an earlier run of this tile exists
                 uint32_t count = (lo - i + 1) | (link != 0xffffffffu ? 0x80000000u : 0u);
-                unsigned long long run =
-                    ((unsigned long long)count << 32) | (unsigned long long)(base + i);
-                unsigned long long prev = atomicExch(
-                    reinterpret_cast<unsigned long long *>(&a.tile_runs[tile]), run);
+                a.tile_runs[tile] = make_uint2(base + i, count);
                 // the tile's first run has no predecessor (tile_runs starts undefined)
-                a.run_prev[base + i] = link == 0xffffffffu
-                                           ? make_uint2(0, 0)
-                                           : make_uint2((uint32_t)prev, (uint32_t)(prev >> 32));
+                a.run_prev[base + i] = link == 0xffffffffu ? make_uint2(0, 0) : prev;
             } else {
-                link = (uint32_t)keys[i - 1];
+                link = (uint32_t)sk[i - 1];
             }
             a.fills[(size_t)f * 3 + 2] = link;
-            a.sorted_fills[base + i] =
-                make_uint2(a.fills[(size_t)f * 3 + 0], a.fills[(size_t)f * 3 + 1]);
+            a.sorted_fills[base + i] = words[f - base];
         }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ propagate
+
+constexpr uint32_t kPropRows = 8;  // tile rows a column walk fetches at a time
 
 // propagate.cs.glsl:103-218 for one column. EMIT = false counts the alpha
 // tiles the column allocates; EMIT = true writes tile words and alpha tiles
@@ -706,64 +770,91 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
         crx1 = (uint32_t)cm[2]; cry1 = (uint32_t)cm[3];
         cofs = (uint32_t)cm[4];
     }
+    bool cload = clip && a.clip_tile_count != 0;
     uint32_t count = 0;
-    for (uint32_t y = 0; y < height; y++) {
-        uint32_t t = dofs + y * width + (uint32_t)tile_x;
-        if (t >= a.tile_count) {
-            atomicOr(&a.counters->error, 4u);
-            break;
-        }
-        uint4 tile = a.tiles[t];
-        int32_t draw_alpha = -1, clip_alpha = -1;
-        int32_t first_fill = (int32_t)tile.y;
-        int32_t delta = (int32_t)tile.z >> 24;
-        uint32_t word = tile.w & 0x00ffffffu;
-        int32_t tile_backdrop = backdrop;
-        bool have_mask = first_fill >= 0;
-        bool need_alpha = have_mask;
-        if (clip) {
-            uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;  // unsigned, as the shader
-            if (ux >= crx0 && uy >= cry0 && ux < crx1 && uy < cry1) {
+    // rows are fetched kPropRows at a time (the loads do not depend on the
+    // running backdrop), then processed in order
+    for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
+        uint4 tl[kPropRows];
+        uint2 cl[kPropRows];
+        // unconditional loads at clamped indices (a load under a branch is
+        // waited for before the branch joins); rows past the column or the
+        // buffers are not used below
+#pragma unroll
+        for (uint32_t u = 0; u < kPropRows; u++) {
+            uint32_t y = y0 + u;
+            uint32_t t = dofs + y * width + (uint32_t)tile_x;
+            tl[u] = a.tiles[min(t, a.tile_count - 1u)];
+            if (cload) {
+                uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;
                 uint32_t ct = cofs + (uy - cry0) * (crx1 - crx0) + (ux - crx0);
-                uint4 ctile = ct < a.clip_tile_count ? a.clip_tiles[ct] : make_uint4(0, 0, 0, 0);
-                int32_t this_clip_alpha = sext24(ctile.z);
-                int32_t clip_backdrop = (int32_t)ctile.w >> 24;
-                if (this_clip_alpha >= 0) {
-                    if (have_mask) {
-                        clip_alpha = this_clip_alpha;
-                        need_alpha = true;
-                    } else if (tile_backdrop != 0) {
-                        draw_alpha = this_clip_alpha;
-                        clip_alpha = -1;
-                        need_alpha = false;
-                    } else {
-                        draw_alpha = -1;
-                        clip_alpha = -1;
+                cl[u] = ((const uint2 *)(a.clip_tiles + min(ct, a.clip_tile_count - 1u)))[1];
+            }
+        }
+        bool stop = false;
+#pragma unroll
+        for (uint32_t u = 0; u < kPropRows; u++) {
+            uint32_t y = y0 + u;
+            if (y >= height) break;
+            uint32_t t = dofs + y * width + (uint32_t)tile_x;
+            if (t >= a.tile_count) {
+                atomicOr(&a.counters->error, 4u);
+                stop = true;
+                break;
+            }
+            uint4 tile = tl[u];
+            int32_t draw_alpha = -1, clip_alpha = -1;
+            int32_t first_fill = (int32_t)tile.y;
+            int32_t delta = (int32_t)tile.z >> 24;
+            uint32_t word = tile.w & 0x00ffffffu;
+            int32_t tile_backdrop = backdrop;
+            bool have_mask = first_fill >= 0;
+            bool need_alpha = have_mask;
+            if (clip) {
+                uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;  // unsigned, as the shader
+                if (ux >= crx0 && uy >= cry0 && ux < crx1 && uy < cry1) {
+                    uint32_t ct = cofs + (uy - cry0) * (crx1 - crx0) + (ux - crx0);
+                    uint2 cw = ct < a.clip_tile_count ? cl[u] : make_uint2(0, 0);
+                    int32_t this_clip_alpha = sext24(cw.x);
+                    int32_t clip_backdrop = (int32_t)cw.y >> 24;
+                    if (this_clip_alpha >= 0) {
+                        if (have_mask) {
+                            clip_alpha = this_clip_alpha;
+                            need_alpha = true;
+                        } else if (tile_backdrop != 0) {
+                            draw_alpha = this_clip_alpha;
+                            clip_alpha = -1;
+                            need_alpha = false;
+                        } else {
+                            draw_alpha = -1;
+                            clip_alpha = -1;
+                            need_alpha = false;
+                        }
+                    } else if (clip_backdrop == 0) {
+                        tile_backdrop = 0;
                         need_alpha = false;
                     }
-                } else if (clip_backdrop == 0) {
+                } else {
                     tile_backdrop = 0;
                     need_alpha = false;
                 }
-            } else {
-                tile_backdrop = 0;
-                need_alpha = false;
             }
-        }
-        if (need_alpha) {
+            if (need_alpha) {
+                if (EMIT) {
+                    uint32_t ai = alpha_base + count;
+                    a.alpha_tiles[ai] = make_uint2(t, (uint32_t)clip_alpha);
+                    draw_alpha = (int32_t)(ai + first_alpha);
+                }
+                count++;
+            }
             if (EMIT) {
-                uint32_t ai = alpha_base + count;
-                a.alpha_tiles[ai] = make_uint2(t, (uint32_t)clip_alpha);
-                draw_alpha = (int32_t)(ai + first_alpha);
+                uint2 zw = make_uint2(((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24),
+                                      word | ((uint32_t)tile_backdrop << 24));
+                *reinterpret_cast<uint2 *>(&a.tiles[t].z) = zw;
             }
-            count++;
+            backdrop += delta;
         }
-        if (EMIT) {
-            uint2 zw = make_uint2(((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24),
-                                  word | ((uint32_t)tile_backdrop << 24));
-            *reinterpret_cast<uint2 *>(&a.tiles[t].z) = zw;
-        }
-        backdrop += delta;
+        if (stop) break;
     }
     return count;
 }
@@ -1581,6 +1672,8 @@ void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
     if (a.path_count) {
         Scope t(p, s, K_FILL_LINK);
+        hipLaunchKernelGGL(k_path_fills, dim3((a.path_count + kBlock) / kBlock), dim3(kBlock), 0,
+                           s, a);
         hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
     }
 }

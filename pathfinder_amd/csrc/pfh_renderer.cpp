@@ -184,6 +184,7 @@ struct Batch {
     uint32_t *d_ml_fill_offsets = nullptr;
     uint32_t *d_fills = nullptr;
     uint32_t *d_fill_tiles = nullptr;
+    uint32_t *d_path_fills = nullptr;
     uint2 *d_sorted_fills = nullptr;
     uint2 *d_run_prev = nullptr;
     uint2 *d_tile_runs = nullptr;
@@ -276,6 +277,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
     dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
+    dfree(ctx, b.d_path_fills);
     dfree(ctx, b.d_tile_runs);
     dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
@@ -373,6 +375,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.sorted_fills = b.d_sorted_fills;
     a.tile_runs = b.d_tile_runs;
     a.run_prev = b.d_run_prev;
+    a.path_fills = b.d_path_fills;
     a.tiles = b.d_tiles;
     a.col_alpha_offsets = b.d_col_alpha;
     a.alpha_tiles = b.d_alpha;
@@ -484,6 +487,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = upload(ctx, &b.d_backdrops_init, h.backdrops)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_backdrops, h.backdrops.size())) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
