@@ -170,6 +170,7 @@ class EventProfiler : public pfh::Profiler {
 struct Batch {
     pfh::BatchData host;
     bool draw = false;
+    hipEvent_t front_done = nullptr;  // the batch's front stages finished
     int32_t load = 1;
     float clear[4] = {0, 0, 0, 0};
     // device inputs
@@ -206,7 +207,12 @@ struct Batch {
 
 struct pfh_context {
     int device = 0;
+    // Back stages (propagate onwards) run in batch order on `stream`; each
+    // batch's front stages (init, dice, bound, bin, fill links) run on one of
+    // the two side streams, so they overlap the previous batch's back stages.
     hipStream_t stream = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t frame_start = nullptr;
     std::string error;
     Pool pool;
     EventProfiler prof;
@@ -273,6 +279,8 @@ void dfree(pfh_context *ctx, T *&p) {
 }
 
 void free_batch(pfh_context *ctx, Batch &b) {
+    if (b.front_done) (void)hipEventDestroy(b.front_done);
+    b.front_done = nullptr;
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
@@ -291,12 +299,13 @@ void free_batches(pfh_context *ctx) {
 }
 
 template <class T>
-pfh_status upload(pfh_context *ctx, T **dst, const std::vector<T> &src) {
+pfh_status upload(pfh_context *ctx, T **dst, const std::vector<T> &src,
+                  hipStream_t stream = nullptr) {
     pfh_status s = dalloc(ctx, dst, src.size());
     if (s != PFH_OK) return s;
     if (!src.empty())
         HIP_TRY(ctx, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T),
-                                    hipMemcpyHostToDevice, ctx->stream));
+                                    hipMemcpyHostToDevice, stream ? stream : ctx->stream));
     return PFH_OK;
 }
 
@@ -396,15 +405,20 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 }
 
 // prepare_tiles (d3d11/renderer.rs:427-542) and, for draw batches, draw_tiles
-// (:711-782), enqueued on the context stream.
-pfh_status run_batch(pfh_context *ctx, Batch &b) {
+// (:711-782). The stages that only touch the batch's own buffers run on the
+// side stream `front`; propagate and later (the clip batch's tiles and
+// masks, the frame's alpha tile count, the framebuffer) follow on the
+// context stream once they are done.
+pfh_status run_batch(pfh_context *ctx, Batch &b, hipStream_t front) {
     pfh::BatchArgs a = batch_args(ctx, b);
-    hipStream_t s = ctx->stream;
     pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
-    pfh::launch_batch_init(a, s, p);
-    pfh::launch_dice(a, s, p);
-    pfh::launch_bound(a, s, p);
-    pfh::launch_bin(a, s, p);
+    pfh::launch_batch_init(a, front, p);
+    pfh::launch_dice(a, front, p);
+    pfh::launch_bound(a, front, p);
+    pfh::launch_bin(a, front, p);
+    HIP_TRY(ctx, hipEventRecord(b.front_done, front));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipStreamWaitEvent(s, b.front_done, 0));
     pfh::launch_propagate(a, s, p);
     pfh::launch_fill(a, s, p);
     pfh::launch_sort(a, s, p);
@@ -428,15 +442,25 @@ pfh_status run_batch(pfh_context *ctx, Batch &b) {
     return PFH_OK;
 }
 
+// Orders the side streams after everything enqueued on the context stream so
+// far (the previous frame reads the batches' buffers until it ends).
+pfh_status start_frame_on_side_streams(pfh_context *ctx) {
+    HIP_TRY(ctx, hipEventRecord(ctx->frame_start, ctx->stream));
+    for (hipStream_t side : ctx->side) HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->frame_start, 0));
+    return PFH_OK;
+}
+
 pfh_status replay_frame(pfh_context *ctx) {
     pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
     if (ctx->d_fb_saved)
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_fb, ctx->d_fb_saved, ctx->fb_words * 4,
                                     hipMemcpyDeviceToDevice, ctx->stream));
-    for (auto &b : ctx->batches) {
-        pfh_status s = size_capacity_buffers(ctx, *b);
-        if (s != PFH_OK) return s;
-        if ((s = run_batch(ctx, *b)) != PFH_OK) return s;
+    pfh_status s = start_frame_on_side_streams(ctx);
+    if (s != PFH_OK) return s;
+    for (size_t i = 0; i < ctx->batches.size(); i++) {
+        Batch &b = *ctx->batches[i];
+        if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
+        if ((s = run_batch(ctx, b, ctx->side[i & 1])) != PFH_OK) return s;
     }
     return PFH_OK;
 }
@@ -480,11 +504,17 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if (ctx->cap_f < ctx->cap_m) ctx->cap_f = ctx->cap_m;
 
     size_t fbn = (size_t)ctx->fbw * ctx->fbh;
+    // the batch's inputs go up on the side stream its front stages run on
+    hipStream_t front = ctx->side[ctx->batches.size() & 1];
+    if (hipEventCreateWithFlags(&b.front_done, hipEventDisableTiming) != hipSuccess) {
+        b.front_done = nullptr;
+        return fail(ctx, PFH_ERROR_HIP, "hipEventCreateWithFlags failed");
+    }
     pfh_status s;
-    if ((s = upload(ctx, &b.d_dice, h.dice_metadata)) != PFH_OK) return s;
-    if ((s = upload(ctx, &b.d_tpi, h.tile_path_info)) != PFH_OK) return s;
-    if ((s = upload(ctx, &b.d_prop, h.propagate_metadata)) != PFH_OK) return s;
-    if ((s = upload(ctx, &b.d_backdrops_init, h.backdrops)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_dice, h.dice_metadata, front)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_tpi, h.tile_path_info, front)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_prop, h.propagate_metadata, front)) != PFH_OK) return s;
+    if ((s = upload(ctx, &b.d_backdrops_init, h.backdrops, front)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_backdrops, h.backdrops.size())) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
@@ -502,7 +532,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_counters, 1)) != PFH_OK) return s;
     if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
     ctx->batches.push_back(std::move(bp));
-    return run_batch(ctx, b);
+    return run_batch(ctx, b, front);
 }
 
 }  // namespace
@@ -531,6 +561,11 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PFH_ERROR_NO_DEVICE;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return PFH_ERROR_HIP;
+    for (hipStream_t &side : ctx->side)
+        if (hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess)
+            return PFH_ERROR_HIP;
+    if (hipEventCreateWithFlags(&ctx->frame_start, hipEventDisableTiming) != hipSuccess)
+        return PFH_ERROR_HIP;
     // Renderer::new uploads the area LUT (gpu/renderer.rs:207-222).
     uint8_t *d_bytes = nullptr;
     if (dalloc(ctx.get(), &d_bytes, 256 * 256 * 4) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
@@ -551,8 +586,13 @@ void pfh_context_destroy(pfh_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t side : ctx->side)
+        if (side) (void)hipStreamSynchronize(side);
     free_batches(ctx);
     ctx->pool.release_all();
+    for (hipStream_t side : ctx->side)
+        if (side) (void)hipStreamDestroy(side);
+    if (ctx->frame_start) (void)hipEventDestroy(ctx->frame_start);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -566,6 +606,7 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
                     frame->height);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (hipStream_t side : ctx->side) HIP_TRY(ctx, hipStreamSynchronize(side));
     free_batches(ctx);
     int32_t fbw = (frame->width + 15) / 16, fbh = (frame->height + 15) / 16;
     size_t words = (size_t)fbw * fbh * 256;
@@ -602,6 +643,7 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
     pfh_status s = ensure_mask(ctx, 0);
     if (s != PFH_OK) return s;
     pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
+    if ((s = start_frame_on_side_streams(ctx)) != PFH_OK) return s;
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
     ctx->in_scene = true;
     return PFH_OK;
@@ -656,7 +698,8 @@ pfh_status pfh_upload_scene(pfh_context *ctx, const pfh_segments *draw, const pf
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_clip_idx, ctx->clip_idx_host.data(),
                                     (size_t)clip->index_count * 8, hipMemcpyHostToDevice,
                                     ctx->stream));
-    return PFH_OK;
+    // the batches' front stages (side streams) read the scene
+    return start_frame_on_side_streams(ctx);
 }
 
 pfh_status pfh_prepare_tiles(pfh_context *ctx, const pfh_tile_batch *batch) {
