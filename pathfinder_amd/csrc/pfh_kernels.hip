@@ -115,9 +115,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t *data,
     uint32_t base = blockIdx.x * kScanTile;
     if (base >= n) return;
     uint32_t s = 0;
+#pragma unroll
     for (uint32_t k = 0; k < kScanTile / kBlock; k++) {
         uint32_t i = base + k * kBlock + threadIdx.x;
-        if (i < n) s += data[i];
+        uint32_t v = data[min(i, n - 1u)];  // unconditional: the loads overlap
+        s += i < n ? v : 0u;
     }
     uint32_t t = block_reduce_sum(s, lds4);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -217,6 +219,26 @@ __global__ void k_frame_reset(FrameState *f) {
     }
 }
 
+// The largest i < n with key(i) <= t (0 when there is none) for keys that do
+// not decrease, searched by a whole wavefront 64 pivots at a time (three
+// rounds of parallel loads for 2^18 keys instead of a chain of 18). t and n
+// must be the same in every lane.
+template <class Key>
+__device__ __forceinline__ uint32_t wave_last_at_or_before(uint32_t n, uint32_t t, Key key) {
+    uint32_t lo = 0, len = n, lane = lane_id();
+    while (len > 1) {
+        uint32_t step = (len + 63u) / 64u;
+        uint32_t idx = lo + (lane + 1u) * step;
+        bool in = idx < lo + len;
+        uint32_t k = key(min(idx, n - 1u));
+        uint32_t cnt = (uint32_t)__popcll(__ballot(in && k <= t));
+        uint32_t end = lo + len;
+        lo += cnt * step;
+        len = min(step, end - lo);
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------ dice
 
 struct DiceSeg {
@@ -232,9 +254,10 @@ __device__ __forceinline__ float2 get_point(const BatchArgs &a, uint32_t i) {
                        (a.xform[2] * p.x + a.xform[3] * p.y) + a.xform[5]);
 }
 
-// dice.cs.glsl:130-202: path search, point fetch and segment count.
-__device__ bool dice_setup(const BatchArgs &a, uint32_t seg, DiceSeg &d) {
-    if (a.path_count == 0) return false;
+// dice.cs.glsl:130-148: the path of segment seg, by the shader's binary
+// search over the paths' first batch segments (it stops at the first probe
+// equal to seg, which can be an empty path).
+__device__ uint32_t reference_segment_search(const BatchArgs &a, uint32_t seg) {
     uint32_t lo = 0, hi = a.path_count;
     int it = 0;
     while (it < 1024 && lo + 1 < hi) {
@@ -248,8 +271,54 @@ __device__ bool dice_setup(const BatchArgs &a, uint32_t seg, DiceSeg &d) {
         }
         it++;
     }
-    d.path = lo;
-    uint4 md = a.dice_md[lo];
+    return lo;
+}
+
+// The path of segment s0 + threadIdx.x, for a block of consecutive segments.
+// Two wavefronts find the paths of the first and last segment, the block
+// stages the first segments of the paths in between in LDS and each thread
+// searches there. The result is the last path starting at or before the
+// segment, which is what the shader's search returns unless two paths start
+// at the segment (an empty one before it); those take the shader's search.
+// Every thread of the block must call it.
+__device__ uint32_t block_segment_path(const BatchArgs &a, uint32_t s0, uint32_t *zs,
+                                       uint32_t *qrange) {
+    uint32_t seg = s0 + threadIdx.x, wave = threadIdx.x >> 6;
+    uint32_t s1 = min(s0 + kBlock, a.segment_count) - 1;
+    auto first_seg = [&](uint32_t i) { return a.dice_md[i].z; };
+    if (wave < 2) {
+        uint32_t q = wave_last_at_or_before(a.path_count, wave == 0 ? s0 : s1, first_seg);
+        if (lane_id() == 0) qrange[wave] = q;
+    }
+    __syncthreads();
+    uint32_t q0 = qrange[0], q1 = qrange[1];
+    bool staged = q1 - q0 < kBlock;
+    if (staged)
+        for (uint32_t i = threadIdx.x; i <= q1 - q0; i += blockDim.x) zs[i] = a.dice_md[q0 + i].z;
+    __syncthreads();
+    uint32_t path;
+    if (staged) {
+        uint32_t lo = 0, hi = q1 - q0 + 1;
+        while (lo + 1 < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (zs[mid] <= seg) lo = mid;
+            else hi = mid;
+        }
+        path = q0 + lo;
+        if (path > 0 && zs[lo] == seg && (lo > 0 ? zs[lo - 1] : a.dice_md[path - 1].z) == seg)
+            path = reference_segment_search(a, seg);
+    } else {
+        path = reference_segment_search(a, seg);
+    }
+    __syncthreads();  // zs is rewritten by the next call
+    return path;
+}
+
+// dice.cs.glsl:149-202: point fetch and segment count of segment seg of path
+// `path`.
+__device__ bool dice_setup(const BatchArgs &a, uint32_t seg, uint32_t path, DiceSeg &d) {
+    d.path = path;
+    uint4 md = a.dice_md[path];
     uint32_t gseg = seg - md.z + md.y;
     if (gseg >= a.scene.index_count) return false;
     uint2 ix = a.scene.indices[gseg];
@@ -292,11 +361,22 @@ __device__ bool dice_setup(const BatchArgs &a, uint32_t seg, DiceSeg &d) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_dice_count(BatchArgs a) {
-    for (uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x; seg < a.segment_count;
-         seg += gridDim.x * blockDim.x) {
+    __shared__ uint32_t zs[kBlock];
+    __shared__ uint32_t qrange[2];
+    for (uint32_t s0 = blockIdx.x * kBlock; s0 < a.segment_count; s0 += gridDim.x * kBlock) {
+        uint32_t seg = s0 + threadIdx.x;
+        if (a.path_count == 0) {
+            if (seg < a.segment_count) {
+                atomicOr(&a.counters->error, 1u);
+                a.seg_offsets[seg] = 0;
+            }
+            continue;
+        }
+        uint32_t path = block_segment_path(a, s0, zs, qrange);
+        if (seg >= a.segment_count) continue;
         DiceSeg d;
         uint32_t n = 0;
-        if (dice_setup(a, seg, d)) {
+        if (dice_setup(a, seg, path, d)) {
             n = (uint32_t)d.n;
         } else {
             atomicOr(&a.counters->error, 1u);
@@ -324,10 +404,15 @@ __device__ __forceinline__ uint4 pack_microline(float x0, float y0, float x1, fl
 }
 
 __global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
-    for (uint32_t seg = blockIdx.x * blockDim.x + threadIdx.x; seg < a.segment_count;
-         seg += gridDim.x * blockDim.x) {
+    __shared__ uint32_t zs[kBlock];
+    __shared__ uint32_t qrange[2];
+    if (a.path_count == 0) return;
+    for (uint32_t s0 = blockIdx.x * kBlock; s0 < a.segment_count; s0 += gridDim.x * kBlock) {
+        uint32_t seg = s0 + threadIdx.x;
+        uint32_t path = block_segment_path(a, s0, zs, qrange);
+        if (seg >= a.segment_count) continue;
         DiceSeg d;
-        if (!dice_setup(a, seg, d)) continue;
+        if (!dice_setup(a, seg, path, d)) continue;
         uint32_t first = a.seg_offsets[seg];
         float2 prev = d.p0;
         for (int32_t i = 0; i < d.n; i++) {
@@ -382,17 +467,6 @@ __device__ __forceinline__ uint32_t reference_tile_search(const BatchArgs &a, ui
     return lo;
 }
 
-// Largest path index whose first tile index is <= t.
-__device__ __forceinline__ uint32_t last_path_at_or_before(const BatchArgs &a, uint32_t t) {
-    uint32_t lo = 0, hi = a.path_count;
-    while (lo + 1 < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a.tile_path_info[mid].z <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // bound.cs.glsl:49-84. A workgroup initialises 1024 consecutive tiles: it
 // stages the first tile indices of the paths they belong to in LDS and looks
 // each tile up there. The reference search returns the same path, the last
@@ -403,25 +477,32 @@ constexpr uint32_t kBoundItems = 4;
 __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
     constexpr uint32_t kPer = kBlock * kBoundItems;
     __shared__ uint32_t zs[kPer + 1];
+    __shared__ uint32_t ws[kPer + 1];
     __shared__ uint32_t qrange[2];
     if (a.path_count == 0) return;
+    uint32_t wave = threadIdx.x >> 6;
+    auto first_tile = [&](uint32_t i) { return a.tile_path_info[i].z; };
     for (uint32_t b0 = blockIdx.x * kPer; b0 < a.tile_count; b0 += gridDim.x * kPer) {
         uint32_t b1 = min(b0 + kPer, a.tile_count) - 1;
-        if (threadIdx.x == 0) {
-            qrange[0] = last_path_at_or_before(a, b0);
-            qrange[1] = last_path_at_or_before(a, b1);
+        // wavefront 0 finds the path of the first tile, wavefront 1 that of the last
+        if (wave < 2) {
+            uint32_t q = wave_last_at_or_before(a.path_count, wave == 0 ? b0 : b1, first_tile);
+            if (lane_id() == 0) qrange[wave] = q;
         }
         __syncthreads();
         uint32_t q0 = qrange[0], q1 = qrange[1];
         bool staged = q1 - q0 < kPer;
         if (staged)
-            for (uint32_t i = threadIdx.x; i <= q1 - q0; i += blockDim.x)
-                zs[i] = a.tile_path_info[q0 + i].z;
+            for (uint32_t i = threadIdx.x; i <= q1 - q0; i += blockDim.x) {
+                uint4 tpi = a.tile_path_info[q0 + i];
+                zs[i] = tpi.z;
+                ws[i] = tpi.w;
+            }
         __syncthreads();
         for (uint32_t k = 0; k < kBoundItems; k++) {
             uint32_t t = b0 + k * kBlock + threadIdx.x;
             if (t > b1) break;
-            uint32_t p;
+            uint32_t p, w;
             if (staged) {
                 uint32_t lo = 0, hi = q1 - q0 + 1;
                 while (lo + 1 < hi) {
@@ -430,13 +511,17 @@ __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
                     else hi = mid;
                 }
                 p = q0 + lo;
+                w = ws[lo];
                 if (p > 0 && zs[lo] == t &&
-                    (lo > 0 ? zs[lo - 1] : a.tile_path_info[p - 1].z) == t)
+                    (lo > 0 ? zs[lo - 1] : a.tile_path_info[p - 1].z) == t) {
                     p = reference_tile_search(a, t);
+                    w = a.tile_path_info[p].w;
+                }
             } else {
                 p = reference_tile_search(a, t);
+                w = a.tile_path_info[p].w;
             }
-            a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, a.tile_path_info[p].w);
+            a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, w);
         }
         __syncthreads();
     }
