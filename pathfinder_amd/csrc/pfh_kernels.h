@@ -119,6 +119,7 @@ void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_row_paths(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p);

@@ -1781,14 +1781,16 @@ void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
 }
 
+void launch_row_paths(const BatchArgs &a, hipStream_t s, Profiler *p) {
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (fbn == 0) return;
+    Scope t(p, s, K_ROW_PATHS);
+    hipLaunchKernelGGL(k_row_paths, dim3((uint32_t)a.fb_height_tiles), dim3(kBlock), 0, s, a);
+}
+
 void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     if (fbn == 0) return;
-    {
-        Scope t(p, s, K_ROW_PATHS);
-        hipLaunchKernelGGL(k_row_paths, dim3((uint32_t)a.fb_height_tiles), dim3(kBlock), 0, s,
-                           a);
-    }
     Scope t(p, s, K_SORT_TILES);
     uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
     hipLaunchKernelGGL(k_sort_tiles, dim3((uint32_t)a.fb_height_tiles * bands),

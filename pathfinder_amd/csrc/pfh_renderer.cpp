@@ -171,6 +171,8 @@ struct Batch {
     pfh::BatchData host;
     bool draw = false;
     hipEvent_t front_done = nullptr;  // the batch's front stages finished
+    hipEvent_t prop_done = nullptr;   // its propagate stage finished
+    hipEvent_t sort_done = nullptr;   // its sort stage finished
     int32_t load = 1;
     float clear[4] = {0, 0, 0, 0};
     // device inputs
@@ -279,8 +281,10 @@ void dfree(pfh_context *ctx, T *&p) {
 }
 
 void free_batch(pfh_context *ctx, Batch &b) {
-    if (b.front_done) (void)hipEventDestroy(b.front_done);
-    b.front_done = nullptr;
+    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
@@ -405,23 +409,30 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 }
 
 // prepare_tiles (d3d11/renderer.rs:427-542) and, for draw batches, draw_tiles
-// (:711-782). The stages that only touch the batch's own buffers run on the
-// side stream `front`; propagate and later (the clip batch's tiles and
-// masks, the frame's alpha tile count, the framebuffer) follow on the
-// context stream once they are done.
+// (:711-782). The stages that only touch the batch's own buffers (init, dice,
+// bound, the per-row path lists, bin) run on the side stream `front`;
+// propagate, fill and tile (the clip batch's tiles and masks, the frame's
+// alpha tile count, the framebuffer) follow in batch order on the context
+// stream. Sort reads what propagate writes but not the masks, so it runs on
+// `front` beside fill; tile and the next batch wait for it.
 pfh_status run_batch(pfh_context *ctx, Batch &b, hipStream_t front) {
     pfh::BatchArgs a = batch_args(ctx, b);
     pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
     pfh::launch_batch_init(a, front, p);
     pfh::launch_dice(a, front, p);
     pfh::launch_bound(a, front, p);
+    pfh::launch_row_paths(a, front, p);
     pfh::launch_bin(a, front, p);
     HIP_TRY(ctx, hipEventRecord(b.front_done, front));
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipStreamWaitEvent(s, b.front_done, 0));
     pfh::launch_propagate(a, s, p);
+    HIP_TRY(ctx, hipEventRecord(b.prop_done, s));
+    HIP_TRY(ctx, hipStreamWaitEvent(front, b.prop_done, 0));
+    pfh::launch_sort(a, front, p);
+    HIP_TRY(ctx, hipEventRecord(b.sort_done, front));
     pfh::launch_fill(a, s, p);
-    pfh::launch_sort(a, s, p);
+    HIP_TRY(ctx, hipStreamWaitEvent(s, b.sort_done, 0));
     if (b.draw) {
         pfh::TileArgs t;
         t.tiles = b.d_tiles;
@@ -506,9 +517,11 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     size_t fbn = (size_t)ctx->fbw * ctx->fbh;
     // the batch's inputs go up on the side stream its front stages run on
     hipStream_t front = ctx->side[ctx->batches.size() & 1];
-    if (hipEventCreateWithFlags(&b.front_done, hipEventDisableTiming) != hipSuccess) {
-        b.front_done = nullptr;
-        return fail(ctx, PFH_ERROR_HIP, "hipEventCreateWithFlags failed");
+    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done}) {
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+            *e = nullptr;
+            return fail(ctx, PFH_ERROR_HIP, "hipEventCreateWithFlags failed");
+        }
     }
     pfh_status s;
     if ((s = upload(ctx, &b.d_dice, h.dice_metadata, front)) != PFH_OK) return s;
