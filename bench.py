@@ -92,6 +92,20 @@ def cpu_baseline(cmds, size, bg, microlines):
                       "bound/sort/fill/tile): %.2f s" % dt}
 
 
+def cpu_baseline_d3d9(scene, size, bg, microlines):
+    from oracle import d3d9
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    d3d9.render(scene, size, bg, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(microlines / dt / 1e6, 4), "unit": "Msegments/s", "cores": threads,
+            "kind": "port",
+            "sample": "one whole frame of the same workload through the D3D9-level CPU "
+                      "restatement (tiler.rs flattening and tiling on a thread per path, "
+                      "binary16 mask raster, per-tile RGBA8 composite), rated on the D3D11 "
+                      "microline count: %.2f s" % dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +230,7 @@ def main():
         from oracle import packing
         cmds = packing.build(scene)
         line["cpu_baseline"] = cpu_baseline(cmds, size, bg, M)
+        line["cpu_baseline_d3d9"] = cpu_baseline_d3d9(scene, size, bg, M)
     r.close()
     if args.copy_calibration:
         copy_calibration(256 << 20)

@@ -112,6 +112,35 @@ float orc_f16_to_f32(uint16_t h);
 /* fill_area.inc.glsl computeCoverage for one (from, to) pair; out[4]. */
 void orc_compute_coverage(const orc_ctx *ctx, float fx, float fy, float tx, float ty, float out[4]);
 
+/* ------------------------------------------------------------ D3D9 level
+ * (pf_d3d9.c): the CPU tiler and mask raster of the reference's D3D9 level,
+ * a second whole-frame opinion that agrees with the D3D11 path only to a
+ * tolerance. Paths are given as segments packed like SegmentsD3D11 (points,
+ * {first point, flags} per segment) with the end of each path's segments. */
+typedef struct orc_d3d9_paths {
+    const float *points;         /* x, y per point */
+    const uint32_t *indices;     /* first point, flags per segment */
+    const uint32_t *path_seg_end;/* per path: one past its last segment */
+    const uint32_t *paint;       /* per path (draw paths) */
+    const int32_t *fill_rule;    /* per path: 0 nonzero, 1 even-odd */
+    const int32_t *clip;         /* per path: clip path index or -1 */
+    uint32_t n_paths;
+} orc_d3d9_paths;
+
+typedef struct orc_d3d9_scene {
+    float view_box[4];           /* min x, min y, max x, max y */
+    float transform[6];          /* m00 m01 m10 m11 tx ty */
+    int32_t width, height, has_background;
+    float background[4];
+    const uint8_t *paints_rgba;
+    uint32_t n_paints;
+    orc_d3d9_paths draw, clip;
+} orc_d3d9_scene;
+
+/* Renders the scene into fb (RGBA8, ceil(w/16)*16 x ceil(h/16)*16 words,
+ * scene rows) with the LUT of ctx. Returns 0 on success. */
+int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *scene, uint32_t *fb, int threads);
+
 #ifdef __cplusplus
 }
 #endif
