@@ -243,8 +243,8 @@ typedef struct pfh_render_stats {
 } pfh_render_stats;
 pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats);
 
-/* Per-kernel device timing: HIP events recorded on the context stream around
- * kernel launches (the reference times draw calls with GL timer queries per
+/* Per-kernel device timing: HIP events recorded around kernel launches on the
+ * stream each kernel runs on (the reference times draw calls with GL timer queries per
  * TimeCategory, gpu/perf.rs:78-219). PFH_PROFILE_ONE times only `kernel`. */
 #define PFH_PROFILE_OFF 0
 #define PFH_PROFILE_ALL 1

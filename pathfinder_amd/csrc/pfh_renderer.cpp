@@ -95,7 +95,7 @@ class Pool {
     uint64_t bytes_ = 0;
 };
 
-// Per-kernel HIP-event timing on the context stream.
+// Per-kernel HIP-event timing on the stream each kernel is launched on.
 class EventProfiler : public pfh::Profiler {
   public:
     int mode = PFH_PROFILE_OFF;
