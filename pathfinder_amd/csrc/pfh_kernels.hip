@@ -681,145 +681,284 @@ __global__ void __launch_bounds__(kBlock) k_path_fills(BatchArgs a) {
 // onto its tile's list with atomicExchange, so in sequential order fill f
 // links to the previous fill of the same tile and the tile heads the last
 // one. A path's fills are contiguous and only touch the path's own tiles, so
-// one workgroup per path sorts its fills by (tile, fill index) in LDS chunks
-// and stitches the runs. No other workgroup touches the path's tiles and the
-// chunks run in order, so the tile head and last run are read and written
-// with plain loads and stores, ordered by the barrier between chunks. The
-// sorted order is also kept: sorted_fills holds each run's fill words
-// contiguously, tile_runs[t] the tile's last run (first index, count; bit 31
-// of the count set when an earlier run exists) and run_prev[first index of a
-// run] the run before it.
-constexpr uint32_t kLinkChunk = 1024;
+// one workgroup per path orders its fills by (tile, fill index) chunk by
+// chunk and stitches the chunks. Bin emits a microline's fills tile by tile,
+// so consecutive fills often share a tile: a chunk is cut into such runs and
+// only the runs are sorted, by (tile, run index); a run keeps its fills in
+// index order. No other workgroup touches the path's tiles and the chunks
+// run in order, so the tile head and last run are read and written with
+// plain loads and stores, ordered by the barrier between chunks. The sorted
+// order is also kept: sorted_fills holds each tile's fills of a chunk
+// contiguously, tile_runs[t] the tile's last such range (first index, count;
+// bit 31 of the count set when an earlier range exists) and run_prev[first
+// index of a range] the range before it.
 
 __device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, uint32_t m) {
     uint32_t lo = __shfl_xor((uint32_t)v, (int)m, 64);
     uint32_t hi = __shfl_xor((uint32_t)(v >> 32), (int)m, 64);
     return ((unsigned long long)hi << 32) | lo;
 }
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, uint32_t m) {
+    return __shfl_xor(v, (int)m, 64);
+}
+__device__ __forceinline__ unsigned long long shfl_xor_key(unsigned long long v, uint32_t m) {
+    return shfl_xor64(v, m);
+}
 
-__global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
-    __shared__ unsigned long long keys[2][kLinkChunk];
-    __shared__ uint2 words[kLinkChunk];
-    uint32_t path = blockIdx.x;
-    if (path >= a.path_count) return;
-    uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
-    const int32_t *pm = a.prop_md + (size_t)path * 12;
-    bool unclipped = pm[7] < 0;
-    uint32_t width = (uint32_t)(pm[2] - pm[0]);
-    if (width == 0) width = 1;
-    for (uint32_t base = f0; base < f1; base += kLinkChunk) {
-        uint32_t n = min(kLinkChunk, f1 - base);
+// Sort keys: (tile - path's first tile, run index). 32 bits when the path's
+// tile rect has at most 2^21 tiles (chunks of 2048 fills, 11 run bits),
+// 64 bits otherwise (chunks of 1024 fills).
+template <typename Key> struct LinkKey;
+template <> struct LinkKey<uint32_t> {
+    static constexpr uint32_t kChunk = 2048;
+    __device__ static uint32_t make(uint32_t tile, uint32_t run) { return tile << 11 | run; }
+    __device__ static uint32_t tile(uint32_t k) { return k >> 11; }
+    __device__ static uint32_t run(uint32_t k) { return k & 2047u; }
+};
+template <> struct LinkKey<unsigned long long> {
+    static constexpr uint32_t kChunk = 1024;
+    __device__ static unsigned long long make(uint32_t tile, uint32_t run) {
+        return (unsigned long long)tile << 32 | run;
+    }
+    __device__ static uint32_t tile(unsigned long long k) { return (uint32_t)(k >> 32); }
+    __device__ static uint32_t run(unsigned long long k) { return (uint32_t)k; }
+};
+constexpr uint32_t kLinkRect32 = 1u << 21;  // largest tile rect with 32-bit keys
+constexpr uint32_t kLinkLds = 2048;          // u16 entries per LDS table
+
+struct LinkLds {
+    unsigned long long keys[2][1024];     // sort buffers (2 x 2048 u32 or 2 x 1024 u64)
+    uint16_t run_first[kLinkLds + 2];     // first fill of each run; [runs] = n
+    uint16_t run_of[kLinkLds];            // run of each fill
+    uint16_t rank[kLinkLds];              // sorted position of each run
+    uint16_t out[kLinkLds];               // first output slot of each sorted run
+    uint32_t scan[4];
+};
+
+template <typename Key>
+__device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t f1,
+                          uint32_t tile_base, uint32_t width, uint32_t col_base,
+                          bool unclipped) {
+    using K = LinkKey<Key>;
+    constexpr uint32_t kChunk = K::kChunk;
+    constexpr uint32_t kPer = kChunk / kBlock;  // fills (and keys) per thread
+    Key *keys0 = reinterpret_cast<Key *>(L.keys[0]);
+    Key *keys1 = reinterpret_cast<Key *>(L.keys[1]);
+    const Key kPad = ~(Key)0;
+    uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    for (uint32_t base = f0; base < f1; base += kChunk) {
+        uint32_t n = min(kChunk, f1 - base);
+        // Runs: thread tid holds fills tid * kPer .. + kPer - 1. Loads are
+        // unconditional at clamped indices so they are all in flight at once.
+        uint32_t i0 = tid * kPer;
+        uint32_t tl[kPer];
+        uint32_t before = a.fill_tiles[base + (i0 > 0 ? min(i0, n) - 1 : 0)];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) tl[k] = a.fill_tiles[base + min(i0 + k, n - 1)];
+        uint32_t starts = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            uint32_t prev = k == 0 ? before : tl[k - 1];
+            bool s = i0 + k < n && (i0 + k == 0 || tl[k] != prev);
+            starts |= (uint32_t)s << k;
+        }
+        uint32_t runs;
+        uint32_t r = block_exclusive_scan(__popc(starts), L.scan, &runs);
+        // run ids: the run of fill i is the number of starts up to i, minus 1
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            uint32_t i = i0 + k;
+            if (i < n) {
+                if (starts >> k & 1u) {
+                    L.run_first[r] = (uint16_t)i;
+                    keys1[r] = K::make(tl[k] - tile_base, r);
+                    r++;
+                }
+                L.run_of[i] = (uint16_t)(r - 1);
+            }
+        }
+        if (tid == 0) L.run_first[runs] = (uint16_t)n;
+        __syncthreads();
+
+        // Bitonic sort of the runs' keys, padded to size. Key e lives in
+        // register q of lane l of wave w, e = q * 64W + w * 64 + l with
+        // W = min(size / 64, 4) waves: partners at distance < 64 are exchanged
+        // across lanes, at distance >= 64W within the thread, and the
+        // distances in between through LDS (double-buffered, one barrier per
+        // step). The input is in keys1 and the first LDS step writes keys0.
         uint32_t size = 64;
-        while (size < n) size <<= 1;
-        // Bitonic sort of the chunk's (tile, fill index) keys, padded to size.
-        // Key e lives in register r of lane l of wave w, e = r * 64W + w * 64
-        // + l with W = min(size / 64, 4) waves: partners at distance < 64 are
-        // exchanged across lanes, at distance >= 64W within the thread, and
-        // the distances in between through LDS (double-buffered, one
-        // barrier per step).
+        while (size < runs) size <<= 1;
         uint32_t lw = size >= 256 ? 2u : (size == 128 ? 1u : 0u);
         uint32_t nw = 1u << lw, span = 64u << lw, nr = size >> (6 + lw);
         bool active = wave < nw;
-        unsigned long long v[kLinkChunk / kBlock];
+        Key v[kPer];
 #pragma unroll
-        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
-            uint32_t e = r * span + wave * 64u + lane;
-            v[r] = ~0ull;
-            if (active && r < nr && e < n) {
-                v[r] = ((unsigned long long)a.fill_tiles[base + e] << 32) |
-                       (unsigned long long)(base + e);
-                words[e] = make_uint2(a.fills[(size_t)(base + e) * 3 + 0],
-                                      a.fills[(size_t)(base + e) * 3 + 1]);
-            }
+        for (uint32_t q = 0; q < kPer; q++) {
+            uint32_t e = q * span + wave * 64u + lane;
+            v[q] = (active && q < nr && e < runs) ? keys1[e] : kPad;
         }
         uint32_t buf = 0;
         for (uint32_t k = 2; k <= size; k <<= 1) {
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
                 if (j < 64u) {
-#pragma unroll
-                    for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
-                        unsigned long long o = shfl_xor64(v[r], j);
-                        uint32_t e = r * span + wave * 64u + lane;
-                        bool keep_min = ((lane & j) == 0) == ((e & k) == 0);
-                        v[r] = keep_min ? min(v[r], o) : max(v[r], o);
-                    }
-                } else if (j < span) {
                     if (active) {
 #pragma unroll
-                        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++)
-                            if (r < nr) keys[buf][r * span + wave * 64u + lane] = v[r];
+                        for (uint32_t q = 0; q < kPer; q++) {
+                            if (q < nr) {
+                                Key o = shfl_xor_key(v[q], j);
+                                uint32_t e = q * span + wave * 64u + lane;
+                                bool keep_min = ((lane & j) == 0) == ((e & k) == 0);
+                                v[q] = keep_min ? min(v[q], o) : max(v[q], o);
+                            }
+                        }
+                    }
+                } else if (j < span) {
+                    Key *kb = reinterpret_cast<Key *>(L.keys[buf]);
+                    if (active) {
+#pragma unroll
+                        for (uint32_t q = 0; q < kPer; q++)
+                            if (q < nr) kb[q * span + wave * 64u + lane] = v[q];
                     }
                     __syncthreads();
                     if (active) {
 #pragma unroll
-                        for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
-                            if (r < nr) {
-                                uint32_t e = r * span + wave * 64u + lane;
-                                unsigned long long o = keys[buf][e ^ j];
+                        for (uint32_t q = 0; q < kPer; q++) {
+                            if (q < nr) {
+                                uint32_t e = q * span + wave * 64u + lane;
+                                Key o = kb[e ^ j];
                                 bool keep_min = ((e & j) == 0) == ((e & k) == 0);
-                                v[r] = keep_min ? min(v[r], o) : max(v[r], o);
+                                v[q] = keep_min ? min(v[q], o) : max(v[q], o);
                             }
                         }
                     }
                     buf ^= 1u;
                 } else {
                     uint32_t rj = j / span;
+                    if (active) {
 #pragma unroll
-                    for (uint32_t r = 0; r < kLinkChunk / kBlock; r++) {
-                        if ((r & rj) == 0 && (r | rj) < kLinkChunk / kBlock) {
-                            uint32_t e = r * span + wave * 64u + lane;
-                            unsigned long long x = v[r], y = v[r | rj];
-                            bool up = (e & k) == 0;
-                            v[r] = up ? min(x, y) : max(x, y);
-                            v[r | rj] = up ? max(x, y) : min(x, y);
+                        for (uint32_t q = 0; q < kPer; q++) {
+                            if ((q & rj) == 0 && (q | rj) < nr) {
+                                uint32_t e = q * span + wave * 64u + lane;
+                                Key x = v[q], y = v[q | rj];
+                                bool up = (e & k) == 0;
+                                v[q] = up ? min(x, y) : max(x, y);
+                                v[q | rj] = up ? max(x, y) : min(x, y);
+                            }
                         }
                     }
                 }
             }
         }
-        // the last LDS step read keys[buf ^ 1]; the sorted keys go to keys[buf]
-        unsigned long long *sk = keys[buf];
+        // The last LDS step read keys[buf ^ 1] (keys1 holds the input when
+        // there was none); the sorted keys go to keys[buf].
+        Key *sk = reinterpret_cast<Key *>(L.keys[buf]);
         if (active) {
 #pragma unroll
-            for (uint32_t r = 0; r < kLinkChunk / kBlock; r++)
-                if (r < nr) sk[r * span + wave * 64u + lane] = v[r];
+            for (uint32_t q = 0; q < kPer; q++)
+                if (q < nr) sk[q * span + wave * 64u + lane] = v[q];
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            unsigned long long cur = sk[i];
-            uint32_t tile = (uint32_t)(cur >> 32), f = (uint32_t)cur;
+
+        // Output slots: exclusive scan of the run lengths in sorted order
+        // (thread tid holds sorted runs tid * kPer .. + kPer - 1).
+        uint32_t len[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            uint32_t p = i0 + k;
+            len[k] = 0;
+            if (p < runs) {
+                uint32_t rr = K::run(sk[p]);
+                len[k] = (uint32_t)L.run_first[rr + 1] - L.run_first[rr];
+                L.rank[rr] = (uint16_t)p;
+            }
+            sum += len[k];
+        }
+        uint32_t total;
+        uint32_t o = block_exclusive_scan(sum, L.scan, &total);
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            if (i0 + k < runs) L.out[i0 + k] = (uint16_t)o;
+            o += len[k];
+        }
+        __syncthreads();
+
+        // Tile ranges and the links of run starts, one sorted run per thread.
+        for (uint32_t p = tid; p < runs; p += kBlock) {
+            Key cur = sk[p];
+            uint32_t lt = K::tile(cur), rr = K::run(cur);
+            uint32_t first = L.run_first[rr];
             uint32_t link;
-            if (i == 0 || (uint32_t)(sk[i - 1] >> 32) != tile) {
-                uint32_t lo = i, hi = n;  // last index with this tile: upper bound - 1
+            if (p == 0 || K::tile(sk[p - 1]) != lt) {
+                // last sorted run of this tile: galloping, then binary search
+                uint32_t lo = p, hi = runs, step = 1;
+                for (;;) {
+                    uint32_t nx = lo + step;
+                    if (nx >= runs) break;
+                    if (K::tile(sk[nx]) != lt) {
+                        hi = nx;
+                        break;
+                    }
+                    lo = nx;
+                    step <<= 1;
+                }
                 while (lo + 1 < hi) {
                     uint32_t mid = (lo + hi) >> 1;
-                    if ((uint32_t)(sk[mid] >> 32) == tile) lo = mid;
+                    if (K::tile(sk[mid]) == lt) lo = mid;
                     else hi = mid;
                 }
+                uint32_t rq = K::run(sk[lo]);
+                uint32_t end = L.run_first[rq + 1];  // one past the tile's last fill
+                uint32_t tile = tile_base + lt;
                 link = a.tiles[tile].y;
                 uint2 prev = a.tile_runs[tile];
-                a.tiles[tile].y = (uint32_t)sk[lo];
+                a.tiles[tile].y = base + end - 1;
                 // An unclipped path gives a tile an alpha tile exactly when it has
                 // fills (propagate.cs.glsl:130-131), so the column's alpha count is
                 // its number of tiles with fills.
-                if (link == 0xffffffffu && unclipped) {
-                    uint32_t local = tile - (uint32_t)pm[4];
-                    atomicAdd(&a.col_alpha_offsets[(uint32_t)pm[8] + local % width], 1u);
-                }
-                // bit 31 of the count: an earlier run of this tile exists
-                uint32_t count = (lo - i + 1) | (link != 0xffffffffu ? 0x80000000u : 0u);
-                a.tile_runs[tile] = make_uint2(base + i, count);
-                // the tile's first run has no predecessor (tile_runs starts undefined)
-                a.run_prev[base + i] = link == 0xffffffffu ? make_uint2(0, 0) : prev;
+                if (link == 0xffffffffu && unclipped)
+                    atomicAdd(&a.col_alpha_offsets[col_base + lt % width], 1u);
+                uint32_t slot = L.out[p];
+                uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
+                // bit 31 of the count: an earlier range of this tile exists
+                a.tile_runs[tile] =
+                    make_uint2(base + slot, count | (link != 0xffffffffu ? 0x80000000u : 0u));
+                // the tile's first range has no predecessor (tile_runs starts undefined)
+                a.run_prev[base + slot] = link == 0xffffffffu ? make_uint2(0, 0) : prev;
             } else {
-                link = (uint32_t)sk[i - 1];
+                link = base + L.run_first[K::run(sk[p - 1]) + 1] - 1;
             }
-            a.fills[(size_t)f * 3 + 2] = link;
-            a.sorted_fills[base + i] = words[f - base];
+            a.fills[(size_t)(base + first) * 3 + 2] = link;
+        }
+        // Every fill to its slot; a fill inside a run links to the one before.
+        for (uint32_t i = tid; i < n; i += kBlock) {
+            uint32_t rr = L.run_of[i];
+            uint32_t first = L.run_first[rr];
+            uint32_t slot = (uint32_t)L.out[L.rank[rr]] + (i - first);
+            const uint32_t *fw = a.fills + (size_t)(base + i) * 3;
+            a.sorted_fills[base + slot] = make_uint2(fw[0], fw[1]);
+            if (i != first) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
+    __shared__ LinkLds lds;
+    uint32_t path = blockIdx.x;
+    if (path >= a.path_count) return;
+    uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
+    if (f0 == f1) return;
+    const int32_t *pm = a.prop_md + (size_t)path * 12;
+    bool unclipped = pm[7] < 0;
+    uint32_t width = (uint32_t)(pm[2] - pm[0]), height = (uint32_t)(pm[3] - pm[1]);
+    if (width == 0) width = 1;
+    uint64_t rect = (uint64_t)width * height;
+    if (rect <= kLinkRect32)
+        link_path<uint32_t>(a, lds, f0, f1, (uint32_t)pm[4], width, (uint32_t)pm[8], unclipped);
+    else
+        link_path<unsigned long long>(a, lds, f0, f1, (uint32_t)pm[4], width, (uint32_t)pm[8],
+                                      unclipped);
 }
 
 // ------------------------------------------------------------------ propagate
