@@ -706,32 +706,35 @@ __device__ __forceinline__ unsigned long long shfl_xor_key(unsigned long long v,
 }
 
 // Sort keys: (tile - path's first tile, run index). 32 bits when the path's
-// tile rect has at most 2^21 tiles (chunks of 2048 fills, 11 run bits),
-// 64 bits otherwise (chunks of 1024 fills).
+// tile rect has at most 2^(32 - kLinkRunBits) tiles (chunks of 2^kLinkRunBits
+// fills), 64 bits otherwise (chunks of half that, so the LDS is the same).
+constexpr uint32_t kLinkRunBits = 10;
+constexpr uint32_t kLinkFills = 1u << kLinkRunBits;         // fills per 32-bit chunk
+constexpr uint32_t kLinkRect32 = 1u << (32 - kLinkRunBits);  // largest rect for 32-bit keys
 template <typename Key> struct LinkKey;
 template <> struct LinkKey<uint32_t> {
-    static constexpr uint32_t kChunk = 2048;
-    __device__ static uint32_t make(uint32_t tile, uint32_t run) { return tile << 11 | run; }
-    __device__ static uint32_t tile(uint32_t k) { return k >> 11; }
-    __device__ static uint32_t run(uint32_t k) { return k & 2047u; }
+    static constexpr uint32_t kChunk = kLinkFills;
+    __device__ static uint32_t make(uint32_t tile, uint32_t run) {
+        return tile << kLinkRunBits | run;
+    }
+    __device__ static uint32_t tile(uint32_t k) { return k >> kLinkRunBits; }
+    __device__ static uint32_t run(uint32_t k) { return k & (kLinkFills - 1u); }
 };
 template <> struct LinkKey<unsigned long long> {
-    static constexpr uint32_t kChunk = 1024;
+    static constexpr uint32_t kChunk = kLinkFills / 2;
     __device__ static unsigned long long make(uint32_t tile, uint32_t run) {
         return (unsigned long long)tile << 32 | run;
     }
     __device__ static uint32_t tile(unsigned long long k) { return (uint32_t)(k >> 32); }
     __device__ static uint32_t run(unsigned long long k) { return (uint32_t)k; }
 };
-constexpr uint32_t kLinkRect32 = 1u << 21;  // largest tile rect with 32-bit keys
-constexpr uint32_t kLinkLds = 2048;          // u16 entries per LDS table
 
 struct LinkLds {
-    unsigned long long keys[2][1024];     // sort buffers (2 x 2048 u32 or 2 x 1024 u64)
-    uint16_t run_first[kLinkLds + 2];     // first fill of each run; [runs] = n
-    uint16_t run_of[kLinkLds];            // run of each fill
-    uint16_t rank[kLinkLds];              // sorted position of each run
-    uint16_t out[kLinkLds];               // first output slot of each sorted run
+    unsigned long long keys[2][kLinkFills / 2];  // sort buffers (u32 or u64 keys)
+    uint16_t run_first[kLinkFills + 2];          // first fill of each run; [runs] = n
+    uint16_t rank[kLinkFills];                   // sorted position of each run
+    uint16_t out[kLinkFills];                    // first output slot of each sorted run
+    uint32_t starts[kLinkFills / 64];            // run starts per (element row, wave)
     uint32_t scan[4];
 };
 
@@ -746,35 +749,50 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
     Key *keys1 = reinterpret_cast<Key *>(L.keys[1]);
     const Key kPad = ~(Key)0;
     uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    unsigned long long below = (1ull << lane) - 1ull;
     for (uint32_t base = f0; base < f1; base += kChunk) {
         uint32_t n = min(kChunk, f1 - base);
-        // Runs: thread tid holds fills tid * kPer .. + kPer - 1. Loads are
-        // unconditional at clamped indices so they are all in flight at once.
-        uint32_t i0 = tid * kPer;
-        uint32_t tl[kPer];
-        uint32_t before = a.fill_tiles[base + (i0 > 0 ? min(i0, n) - 1 : 0)];
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; k++) tl[k] = a.fill_tiles[base + min(i0 + k, n - 1)];
-        uint32_t starts = 0;
+        // Fill i = k * kBlock + tid is element k of this thread. Its tile, the
+        // tile of fill i - 1 and its words are loaded up front, unconditionally
+        // at clamped indices, so all loads are in flight at once.
+        uint32_t tl[kPer], tp[kPer];
+        uint2 fw[kPer];
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
-            uint32_t prev = k == 0 ? before : tl[k - 1];
-            bool s = i0 + k < n && (i0 + k == 0 || tl[k] != prev);
-            starts |= (uint32_t)s << k;
+            uint32_t i = min(k * kBlock + tid, n - 1);
+            tl[k] = a.fill_tiles[base + i];
+            tp[k] = a.fill_tiles[base + (i > 0 ? i - 1 : 0)];
+            const uint32_t *w = a.fills + (size_t)(base + i) * 3;
+            fw[k] = make_uint2(w[0], w[1]);
         }
-        uint32_t runs;
-        uint32_t r = block_exclusive_scan(__popc(starts), L.scan, &runs);
-        // run ids: the run of fill i is the number of starts up to i, minus 1
+        // Runs: a run starts where the tile changes. Its index is the number
+        // of starts before it, counted by ballots per (element row, wave).
+        uint32_t start_bits = 0, le[kPer];
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
-            uint32_t i = i0 + k;
-            if (i < n) {
-                if (starts >> k & 1u) {
-                    L.run_first[r] = (uint16_t)i;
-                    keys1[r] = K::make(tl[k] - tile_base, r);
-                    r++;
-                }
-                L.run_of[i] = (uint16_t)(r - 1);
+            uint32_t i = k * kBlock + tid;
+            bool st = i < n && (i == 0 || tl[k] != tp[k]);
+            unsigned long long m = __ballot(st);
+            le[k] = (uint32_t)__popcll(m & below) + (st ? 1u : 0u);
+            start_bits |= (uint32_t)st << k;
+            if (lane == 0) L.starts[k * (kBlock / 64) + wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        uint32_t runs = 0, rid[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+#pragma unroll
+            for (uint32_t w = 0; w < kBlock / 64; w++) {
+                uint32_t c = L.starts[k * (kBlock / 64) + w];
+                if (w == wave) rid[k] = runs + le[k] - 1;  // the run of this fill
+                runs += c;
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            if (start_bits >> k & 1u) {
+                L.run_first[rid[k]] = (uint16_t)(k * kBlock + tid);
+                keys1[rid[k]] = K::make(tl[k] - tile_base, rid[k]);
             }
         }
         if (tid == 0) L.run_first[runs] = (uint16_t)n;
@@ -861,11 +879,12 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
 
         // Output slots: exclusive scan of the run lengths in sorted order
         // (thread tid holds sorted runs tid * kPer .. + kPer - 1).
+        uint32_t p0 = tid * kPer;
         uint32_t len[kPer];
         uint32_t sum = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
-            uint32_t p = i0 + k;
+            uint32_t p = p0 + k;
             len[k] = 0;
             if (p < runs) {
                 uint32_t rr = K::run(sk[p]);
@@ -878,12 +897,15 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
         uint32_t o = block_exclusive_scan(sum, L.scan, &total);
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
-            if (i0 + k < runs) L.out[i0 + k] = (uint16_t)o;
+            if (p0 + k < runs) L.out[p0 + k] = (uint16_t)o;
             o += len[k];
         }
         __syncthreads();
 
         // Tile ranges and the links of run starts, one sorted run per thread.
+        // k_bound leaves every fill list empty, so in the path's first chunk a
+        // tile has no earlier fill and no earlier range.
+        bool first_chunk = base == f0;
         for (uint32_t p = tid; p < runs; p += kBlock) {
             Key cur = sk[p];
             uint32_t lt = K::tile(cur), rr = K::run(cur);
@@ -910,8 +932,12 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
                 uint32_t rq = K::run(sk[lo]);
                 uint32_t end = L.run_first[rq + 1];  // one past the tile's last fill
                 uint32_t tile = tile_base + lt;
-                link = a.tiles[tile].y;
-                uint2 prev = a.tile_runs[tile];
+                link = 0xffffffffu;
+                uint2 prev = make_uint2(0, 0);
+                if (!first_chunk) {
+                    link = a.tiles[tile].y;
+                    prev = a.tile_runs[tile];
+                }
                 a.tiles[tile].y = base + end - 1;
                 // An unclipped path gives a tile an alpha tile exactly when it has
                 // fills (propagate.cs.glsl:130-131), so the column's alpha count is
@@ -920,24 +946,25 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
                     atomicAdd(&a.col_alpha_offsets[col_base + lt % width], 1u);
                 uint32_t slot = L.out[p];
                 uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
-                // bit 31 of the count: an earlier range of this tile exists
+                // bit 31 of the count: an earlier range of this tile exists, and
+                // only then is run_prev of this range read
                 a.tile_runs[tile] =
                     make_uint2(base + slot, count | (link != 0xffffffffu ? 0x80000000u : 0u));
-                // the tile's first range has no predecessor (tile_runs starts undefined)
-                a.run_prev[base + slot] = link == 0xffffffffu ? make_uint2(0, 0) : prev;
+                if (link != 0xffffffffu) a.run_prev[base + slot] = prev;
             } else {
                 link = base + L.run_first[K::run(sk[p - 1]) + 1] - 1;
             }
             a.fills[(size_t)(base + first) * 3 + 2] = link;
         }
         // Every fill to its slot; a fill inside a run links to the one before.
-        for (uint32_t i = tid; i < n; i += kBlock) {
-            uint32_t rr = L.run_of[i];
-            uint32_t first = L.run_first[rr];
-            uint32_t slot = (uint32_t)L.out[L.rank[rr]] + (i - first);
-            const uint32_t *fw = a.fills + (size_t)(base + i) * 3;
-            a.sorted_fills[base + slot] = make_uint2(fw[0], fw[1]);
-            if (i != first) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            uint32_t i = k * kBlock + tid;
+            if (i < n) {
+                uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
+                a.sorted_fills[base + slot] = fw[k];
+                if (!(start_bits >> k & 1u)) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
+            }
         }
         __syncthreads();
     }
