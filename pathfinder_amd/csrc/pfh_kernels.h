@@ -117,7 +117,8 @@ class Profiler {
 void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p);
-void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p);  // count, scan
+void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p);   // emit, fill lists
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_row_paths(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p);

@@ -1904,19 +1904,21 @@ void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p) {
                        0, s, a);
 }
 
-void launch_bin(const BatchArgs &a, hipStream_t s, Profiler *p) {
+void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint32_t g = grid_for(a.microline_capacity, kBlock);
     {
         Scope t(p, s, K_BIN_COUNT);
         hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
     }
-    {
-        Scope t(p, s, K_BIN_SCAN);
-        scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
-                       a.scan_partials, &a.counters->fills, s);
-        hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
-                           a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
-    }
+    Scope t(p, s, K_BIN_SCAN);
+    scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
+                   a.scan_partials, &a.counters->fills, s);
+    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
+                       a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
+}
+
+void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {
+    uint32_t g = grid_for(a.microline_capacity, kBlock);
     {
         Scope t(p, s, K_BIN_EMIT);
         hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);

@@ -173,6 +173,7 @@ struct Batch {
     hipEvent_t front_done = nullptr;  // the batch's front stages finished
     hipEvent_t prop_done = nullptr;   // its propagate stage finished
     hipEvent_t sort_done = nullptr;   // its sort stage finished
+    hipEvent_t bound_done = nullptr;  // its tiles were initialised (k_bound on the context stream)
     int32_t load = 1;
     float clear[4] = {0, 0, 0, 0};
     // device inputs
@@ -281,7 +282,7 @@ void dfree(pfh_context *ctx, T *&p) {
 }
 
 void free_batch(pfh_context *ctx, Batch &b) {
-    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done}) {
+    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
@@ -415,19 +416,31 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 // alpha tile count, the framebuffer) follow in batch order on the context
 // stream. Sort reads what propagate writes but not the masks, so it runs on
 // `front` beside fill; tile and the next batch wait for it.
-pfh_status run_batch(pfh_context *ctx, Batch &b, hipStream_t front) {
+// The front stages of a batch on its side stream. k_bound (tile
+// initialisation) runs here too unless the caller has put it on the context
+// stream (bound_done), where it overlaps dice; only bin's emit needs it.
+pfh_status run_front(pfh_context *ctx, Batch &b, hipStream_t front, bool bound_launched) {
     pfh::BatchArgs a = batch_args(ctx, b);
     pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
     pfh::launch_batch_init(a, front, p);
     pfh::launch_dice(a, front, p);
-    pfh::launch_bound(a, front, p);
-    pfh::launch_row_paths(a, front, p);
-    pfh::launch_bin(a, front, p);
+    if (!bound_launched) pfh::launch_bound(a, front, p);
+    pfh::launch_bin_count(a, front, p);
+    if (bound_launched) HIP_TRY(ctx, hipStreamWaitEvent(front, b.bound_done, 0));
+    pfh::launch_bin_emit(a, front, p);
     HIP_TRY(ctx, hipEventRecord(b.front_done, front));
+    return PFH_OK;
+}
+
+pfh_status run_back(pfh_context *ctx, Batch &b, hipStream_t front) {
+    pfh::BatchArgs a = batch_args(ctx, b);
+    pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipStreamWaitEvent(s, b.front_done, 0));
     pfh::launch_propagate(a, s, p);
     HIP_TRY(ctx, hipEventRecord(b.prop_done, s));
+    // the per-row path lists are only read by sort: they build while propagate runs
+    pfh::launch_row_paths(a, front, p);
     HIP_TRY(ctx, hipStreamWaitEvent(front, b.prop_done, 0));
     pfh::launch_sort(a, front, p);
     HIP_TRY(ctx, hipEventRecord(b.sort_done, front));
@@ -453,6 +466,11 @@ pfh_status run_batch(pfh_context *ctx, Batch &b, hipStream_t front) {
     return PFH_OK;
 }
 
+pfh_status run_batch(pfh_context *ctx, Batch &b, hipStream_t front) {
+    pfh_status st = run_front(ctx, b, front, false);
+    return st != PFH_OK ? st : run_back(ctx, b, front);
+}
+
 // Orders the side streams after everything enqueued on the context stream so
 // far (the previous frame reads the batches' buffers until it ends).
 pfh_status start_frame_on_side_streams(pfh_context *ctx) {
@@ -468,11 +486,27 @@ pfh_status replay_frame(pfh_context *ctx) {
                                     hipMemcpyDeviceToDevice, ctx->stream));
     pfh_status s = start_frame_on_side_streams(ctx);
     if (s != PFH_OK) return s;
-    for (size_t i = 0; i < ctx->batches.size(); i++) {
+    // Every batch's tiles are initialised on the context stream first (it is
+    // idle until the first front ends), then every batch's front stages, the
+    // one with the most segments first (its chain is the longest), then the
+    // back stages in batch order.
+    size_t n = ctx->batches.size();
+    for (size_t i = 0; i < n; i++) {
         Batch &b = *ctx->batches[i];
         if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
-        if ((s = run_batch(ctx, b, ctx->side[i & 1])) != PFH_OK) return s;
+        pfh::BatchArgs a = batch_args(ctx, b);
+        pfh::launch_bound(a, ctx->stream, ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof);
+        HIP_TRY(ctx, hipEventRecord(b.bound_done, ctx->stream));
     }
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return ctx->batches[x]->host.segment_count > ctx->batches[y]->host.segment_count;
+    });
+    for (size_t i : order)
+        if ((s = run_front(ctx, *ctx->batches[i], ctx->side[i & 1], true)) != PFH_OK) return s;
+    for (size_t i = 0; i < n; i++)
+        if ((s = run_back(ctx, *ctx->batches[i], ctx->side[i & 1])) != PFH_OK) return s;
     return PFH_OK;
 }
 
@@ -517,7 +551,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     size_t fbn = (size_t)ctx->fbw * ctx->fbh;
     // the batch's inputs go up on the side stream its front stages run on
     hipStream_t front = ctx->side[ctx->batches.size() & 1];
-    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done}) {
+    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             *e = nullptr;
             return fail(ctx, PFH_ERROR_HIP, "hipEventCreateWithFlags failed");
