@@ -79,7 +79,7 @@ struct BatchArgs {
     // frame
     uint32_t *mask;                // 64 words per alpha tile
     uint32_t mask_capacity;        // alpha tiles
-    const float4 *lut;             // 256*256 texels, value = byte / 255
+    const float4 *lut;             // padded area LUT (259 rows of 260 texels, byte / 255)
     int32_t fb_width_tiles, fb_height_tiles;
 };
 

@@ -34,6 +34,8 @@ namespace {
 constexpr uint32_t kFlagQuad = 0x80000000u;
 constexpr uint32_t kFlagCubic = 0x40000000u;
 constexpr int kBlock = 256;
+constexpr uint32_t kLutRows = 259;    // padded area LUT: 256 texels + an edge copy on each side
+constexpr uint32_t kLutStride = 260;  // texels per padded LUT row
 constexpr uint32_t kScanTile = 4096;  // elements per scan block
 constexpr uint32_t kMaxGrid = 16384;
 
@@ -745,7 +747,6 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
     using K = LinkKey<Key>;
     constexpr uint32_t kChunk = K::kChunk;
     constexpr uint32_t kPer = kChunk / kBlock;  // fills (and keys) per thread
-    Key *keys0 = reinterpret_cast<Key *>(L.keys[0]);
     Key *keys1 = reinterpret_cast<Key *>(L.keys[1]);
     const Key kPad = ~(Key)0;
     uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
@@ -1342,23 +1343,38 @@ struct FillRecord {
 };
 
 // texture(uAreaLUT, .) at texel coordinates (s, u) = (256 x - 0.5, 256 y - 0.5)
-// with GL_LINEAR, CLAMP_TO_EDGE and fp32 weights; lut is 256 x 256 float4
-// texels, row-major in y.
-__device__ __forceinline__ float4 sample_lut_texel(const char *__restrict__ lut, float s,
-                                                   float u) {
+// with GL_LINEAR, CLAMP_TO_EDGE and fp32 weights. lut is the 256 x 256 float4
+// texels padded with a copy of the edge on every side: padded texel (j, i),
+// 0 <= i, j <= 258, holds texel (clamp(j - 1), clamp(i - 1)) with clamp to
+// [0, 255], rows kLutStride texels apart. With k = clamp(index + 1, 0, 257)
+// the texels k and k + 1 of the padded table are the clamped texels index and
+// index + 1 for every index, so one clamp per axis replaces four.
+// The table is read through a raw buffer descriptor so that the four texels
+// come from one offset: +16 bytes in the instruction, +1 row in an SGPR.
+typedef __amdgpu_buffer_rsrc_t LutRef;
+__device__ __forceinline__ LutRef lut_ref(const float4 *lut) {
+    // word 3 of a raw (untyped) gfx9 buffer descriptor; the record count is
+    // the table size in bytes, so an offset past it would read zeros
+    return __builtin_amdgcn_make_buffer_rsrc((void *)lut, (short)0,
+                                             (int)(kLutRows * kLutStride * 16u), 0x00020000);
+}
+__device__ __forceinline__ float4 lut_texel(LutRef lut, uint32_t offset, uint32_t row) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(lut, offset, row, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+}
+__device__ __forceinline__ float4 sample_lut_texel(LutRef lut, float s, float u) {
     float fs = floorf(s), fu = floorf(u);
     float wa = s - fs, wb = u - fu;
-    // |s|, |u| < 2^21 for every packed fill, so the integer conversion is exact;
-    // CLAMP_TO_EDGE clamps each texel index to [0, 255]
+    // |s|, |u| < 2^21 for every packed fill, so the integer conversion is exact
     int32_t is = (int32_t)fs, iu = (int32_t)fu;
-    uint32_t i0 = (uint32_t)min(max(is, 0), 255) << 4;
-    uint32_t i1 = (uint32_t)min(max(is + 1, 0), 255) << 4;
-    uint32_t j0 = (uint32_t)min(max(iu, 0), 255) << 12;
-    uint32_t j1 = (uint32_t)min(max(iu + 1, 0), 255) << 12;
-    float4 t00 = *(const float4 *)(lut + (j0 | i0));
-    float4 t10 = *(const float4 *)(lut + (j0 | i1));
-    float4 t01 = *(const float4 *)(lut + (j1 | i0));
-    float4 t11 = *(const float4 *)(lut + (j1 | i1));
+    uint32_t ip = (uint32_t)min(max(is + 1, 0), 257);
+    uint32_t jp = (uint32_t)min(max(iu + 1, 0), 257);
+    uint32_t o = (jp * kLutStride + ip) * 16u;
+    float4 t00 = lut_texel(lut, o, 0u);
+    float4 t10 = lut_texel(lut, o + 16u, 0u);
+    float4 t01 = lut_texel(lut, o, kLutStride * 16u);
+    float4 t11 = lut_texel(lut, o + 16u, kLutStride * 16u);
     float ia = 1.0f - wa, ib = 1.0f - wb;
     float4 r;
     r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
@@ -1380,7 +1396,7 @@ __device__ __forceinline__ float4 sample_lut_texel(const char *__restrict__ lut,
 //    which returns the correctly rounded quotient for every w = k / 256,
 //    k in [1, 65535], and offset in [0, 2w] (checked for all of them); a
 //    column the fill touches always has offset in [0, w].
-__device__ __forceinline__ float4 coverage_record(const char *__restrict__ lut,
+__device__ __forceinline__ float4 coverage_record(LutRef lut,
                                                   const FillRecord &f, float fragx, float fragy) {
     float wx = __builtin_amdgcn_fmed3f(f.fx - fragx, -0.5f, 0.5f);
     float wy = __builtin_amdgcn_fmed3f(f.tx - fragx, -0.5f, 0.5f);
@@ -1456,7 +1472,7 @@ __device__ __forceinline__ FillColumns make_record(uint2 fw, FillRecord &f) {
 // right edge) and the sum ends in abs(), so each lane walks only the fills
 // that touch its column, in list order. rec is this wavefront's 64 records
 // in LDS.
-__device__ __forceinline__ void accumulate_chunk(const char *__restrict__ lut, FillRecord *rec,
+__device__ __forceinline__ void accumulate_chunk(LutRef lut, FillRecord *rec,
                                                  uint2 fw, uint32_t n, uint32_t lane,
                                                  uint32_t col, float fragx, float fragy,
                                                  float4 &acc) {
@@ -1498,16 +1514,23 @@ __device__ __forceinline__ float byte_to_unit(uint32_t c) {
     return __builtin_fmaf(__builtin_fmaf(-q0, 255.0f, x), r, q0);
 }
 
+// unorm8 of a value already in [0, 1]: floor(c * 255 + 0.5), the floor being
+// the truncation of a positive value.
+__device__ __forceinline__ uint32_t unorm8_unit(float c) { return (uint32_t)(c * 255.0f + 0.5f); }
+
 // fill.cs.glsl:80-87: backdrop + coverage, abs, clamp, min with the clip
-// mask texel, unorm8 store of the four rows of this lane's mask texel.
+// mask texel, unorm8 store of the four rows of this lane's mask texel. The
+// coverage sums are finite (sums of LUT texels in [0, 1] times window widths
+// in [-1, 1]), so clamp(|x|, 0, 1) is med3(|x|, 0, 1), and the stored values
+// need no second clamp.
 __device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_index,
                                            int32_t backdrop, int32_t clip, uint32_t lane,
                                            float4 acc) {
     float bd = (float)backdrop;
-    float c0 = clamp_glsl(fabsf(bd + acc.x), 0.0f, 1.0f);
-    float c1 = clamp_glsl(fabsf(bd + acc.y), 0.0f, 1.0f);
-    float c2 = clamp_glsl(fabsf(bd + acc.z), 0.0f, 1.0f);
-    float c3 = clamp_glsl(fabsf(bd + acc.w), 0.0f, 1.0f);
+    float c0 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.x), 0.0f, 1.0f);
+    float c1 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.y), 0.0f, 1.0f);
+    float c2 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.z), 0.0f, 1.0f);
+    float c3 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.w), 0.0f, 1.0f);
     if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
         uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
         c0 = min_glsl(c0, byte_to_unit(ct & 0xffu));
@@ -1515,8 +1538,8 @@ __device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_in
         c2 = min_glsl(c2, byte_to_unit((ct >> 16) & 0xffu));
         c3 = min_glsl(c3, byte_to_unit(ct >> 24));
     }
-    a.mask[(size_t)alpha_index * 64 + lane] =
-        unorm8(c0) | (unorm8(c1) << 8) | (unorm8(c2) << 16) | (unorm8(c3) << 24);
+    a.mask[(size_t)alpha_index * 64 + lane] = unorm8_unit(c0) | (unorm8_unit(c1) << 8) |
+                                              (unorm8_unit(c2) << 16) | (unorm8_unit(c3) << 24);
 }
 
 // The tile slot (0..63) of stream position q = pos + lane of a wavefront
@@ -1570,11 +1593,12 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     float4 *res = s_res[wv];
     int32_t *cadj = s_cadj[wv];
     uint8_t *pair_fill = s_pair[wv];
-    const char *lut = (const char *)a.lut;
+    LutRef lut = lut_ref(a.lut);
     uint32_t n_alpha = uniform(a.counters->alpha);
     uint32_t start = uniform(a.counters->alpha_start);
     uint32_t col = lane & 15u, grp = lane >> 4;
     float fragx = (float)col + 0.5f, fragy = (float)(grp * 4u) + 0.5f;
+    float item_fragy = (float)((lane & 3u) * 4u) + 0.5f;  // row group of this lane's items
     uint32_t stride = ((gridDim.x * blockDim.x) >> 6) * kFillGroup;
     for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kFillGroup);
          g0 < n_alpha; g0 += stride) {
@@ -1644,12 +1668,13 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                 uint32_t r1 = cuts != 0ull
                                   ? uniform(__shfl(pe, 63u - (uint32_t)__clzll((long long)cuts), 64))
                                   : min(r0 + kFillPairCap, pairs);
-                // computeCoverage of every (pair, row group) of the round
+                // computeCoverage of every (pair, row group) of the round; lane l
+                // takes row group l & 3 of pairs r0 + (l >> 2) + 16 i
                 for (uint32_t it = lane; it < 4u * (r1 - r0); it += 64u) {
-                    uint32_t p = r0 + (it >> 2), g = it & 3u;
+                    uint32_t p = r0 + (it >> 2);
                     uint32_t k = pair_fill[p];
                     float x = (float)((int32_t)p + cadj[k]) + 0.5f;
-                    res[it] = coverage_record(lut, rec[k], x, (float)(g * 4u) + 0.5f);
+                    res[it] = coverage_record(lut, rec[k], x, item_fragy);
                 }
                 wave_lds_sync();
                 // the fills with a pair in [r0, r1), tile by tile in stream order;
@@ -1821,11 +1846,17 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
 
 // ------------------------------------------------------------------ tables
 
+// The padded LUT of sample_lut_texel: texel (j, i) of the 259 x 259 table (row
+// stride kLutStride) is byte texel (clamp(j - 1), clamp(i - 1)) / 255.
 __global__ void k_lut_expand(const uint8_t *rgba, float4 *lut) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 256u * 256u) return;
-    lut[i] = make_float4((float)rgba[i * 4 + 0] / 255.0f, (float)rgba[i * 4 + 1] / 255.0f,
-                         (float)rgba[i * 4 + 2] / 255.0f, (float)rgba[i * 4 + 3] / 255.0f);
+    if (i >= kLutRows * kLutStride) return;
+    uint32_t jp = i / kLutStride, ip = i % kLutStride;
+    uint32_t j = (uint32_t)min(max((int32_t)jp - 1, 0), 255);
+    uint32_t k = (uint32_t)min(max((int32_t)ip - 1, 0), 255);
+    const uint8_t *t = rgba + (j * 256u + k) * 4u;
+    lut[i] = make_float4((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f,
+                         (float)t[3] / 255.0f);
 }
 
 // ColorU::to_f32 (color/src/lib.rs:70-73) then f16::from_f32 (gpu/renderer.rs:724-727).
@@ -1981,7 +2012,8 @@ void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p) {
 }
 
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s) {
-    hipLaunchKernelGGL(k_lut_expand, dim3(256), dim3(256), 0, s, rgba, lut);
+    hipLaunchKernelGGL(k_lut_expand, dim3((kLutRows * kLutStride + 255) / 256), dim3(256), 0, s,
+                       rgba, lut);
 }
 
 void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s) {

@@ -616,7 +616,8 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     // Renderer::new uploads the area LUT (gpu/renderer.rs:207-222).
     uint8_t *d_bytes = nullptr;
     if (dalloc(ctx.get(), &d_bytes, 256 * 256 * 4) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
-    if (dalloc(ctx.get(), &ctx->d_lut, 256 * 256) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    // the kernels' padded layout: 259 rows of 260 texels
+    if (dalloc(ctx.get(), &ctx->d_lut, 259 * 260) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     if (dalloc(ctx.get(), &ctx->d_frame, 1) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     const uint8_t *lut = opt.area_lut_rgba ? opt.area_lut_rgba : pfh_area_lut_rgba;
     if (hipMemcpy(d_bytes, lut, 256 * 256 * 4, hipMemcpyHostToDevice) != hipSuccess)
