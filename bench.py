@@ -14,6 +14,15 @@ value = microlines per second of frame time (Msegments/s); ms_per_step = frame
 time. roofline: the kernel with the largest time per frame, its algorithmic
 bytes per frame (formulas below) over its time per frame measured with HIP
 events on the launch stream inside the timed region.
+
+diff: the differentiable render (1024 closed cubic paths, 512 x 512, points
+and colours as leaves, L2 loss against the image of a copy moved by
+N(0, 4 px)), measured in a child process started before this one touches the
+GPU: forward = one frame with the coverage checkpoint (re-rendered from
+device memory), backward = pfh_backward (composite and coverage adjoint),
+ratio = (forward + backward) / forward, all device time with the scene
+resident. step_ms is one whole optimisation step through torch.autograd
+(host packing, uploads, forward, loss, backward).
 """
 import argparse
 import json
@@ -106,6 +115,83 @@ def cpu_baseline_d3d9(scene, size, bg, microlines):
                       "microline count: %.2f s" % dt}
 
 
+def diff_leg(steps, warmup):
+    """The differentiable render's timings (runs in its own process: torch's
+    HIP runtime is loaded before the library)."""
+    import torch
+
+    from pathfinder_amd import diff, synth
+    d = synth.diff_paths()
+    w, h = d["size"]
+    shapes = diff.Shapes(d["flags"], d["contour_ends"], d["path_contour_ends"])
+    dr = diff.DiffRenderer(shapes, w, h, (1.0, 1.0, 1.0, 1.0))
+    with torch.no_grad():
+        target = dr.render(torch.tensor(d["target_points"]), torch.tensor(d["colors"]))
+    p = torch.tensor(d["points"], device="cuda", requires_grad=True)
+    c = torch.tensor(d["colors"], device="cuda", requires_grad=True)
+    opt = torch.optim.SGD([p, c], lr=1e-3)
+
+    def step():
+        opt.zero_grad()
+        img = diff.render(dr, p, c)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        return loss.item()
+
+    for _ in range(max(warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [step() for _ in range(steps)]
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    # device times of the last frame: forward (re-rendered) and backward
+    img = diff.render(dr, p, c)
+    g = (2.0 * (img - target)).detach().contiguous()
+    gp = torch.zeros((len(shapes.packed), 2), dtype=torch.float32, device="cuda")
+    gc = torch.zeros((shapes.path_count, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    r = dr.renderer
+    for _ in range(warmup):
+        r.rerender()
+    r.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.rerender()
+    r.synchronize()
+    fwd_ms = (time.perf_counter() - t0) / steps * 1e3
+    for _ in range(warmup):
+        r.backward(g.data_ptr(), gp.data_ptr(), len(shapes.packed), gc.data_ptr(),
+                   shapes.path_count)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.backward(g.data_ptr(), gp.data_ptr(), len(shapes.packed), gc.data_ptr(),
+                   shapes.path_count)
+    bwd_ms = (time.perf_counter() - t0) / steps * 1e3
+    st = r.stats()
+    dr.close()
+    return {"workload": "diff-1024-cubic-paths-512", "width": w, "height": h,
+            "paths": shapes.path_count, "points": shapes.point_count,
+            "microlines": st["microline_count"], "alpha_tiles": st["alpha_tile_count"],
+            "fwd_ms": round(fwd_ms, 4), "bwd_ms": round(bwd_ms, 4),
+            "fwd_bwd_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3),
+            "step_ms": round(step_ms, 3),
+            "loss_first_last": [round(losses[0], 2), round(losses[-1], 2)]}
+
+
+def run_diff_leg(steps, warmup):
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--diff-leg", "--steps", str(steps),
+           "--warmup", str(warmup)]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        print("bench: diff leg failed (%d): %s" % (out.returncode, out.stderr[-2000:]),
+              file=sys.stderr)
+        return None
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,14 +199,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="4k", choices=["4k", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiable-render leg")
+    ap.add_argument("--diff-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the frames, run a plain 256 MiB device-to-device copy so a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.diff_leg:
+        print(json.dumps(diff_leg(args.steps, args.warmup)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each GPU would render its own band of framebuffer tiles).", file=sys.stderr)
         sys.exit(2)
+
+    # the differentiable leg first, in a child process, before this one uses the GPU
+    diff_result = None if args.no_diff else run_diff_leg(args.steps, args.warmup)
 
     from pathfinder_amd import DestFramebuffer, Renderer, RendererOptions, synth
     from pathfinder_amd import _lib as L
@@ -226,6 +321,8 @@ def main():
                      "traffic_source": (os.path.relpath(TRAFFIC_TABLE[args.workload], ROOT)
                                         if traffic else None)},
     }
+    if diff_result is not None:
+        line["diff"] = diff_result
     if not args.no_cpu_baseline:
         from oracle import packing
         cmds = packing.build(scene)

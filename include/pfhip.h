@@ -218,7 +218,10 @@ typedef enum pfh_debug_kind {
     PFH_DEBUG_FIRST_TILE_MAP = 4, /* int32[fb tiles] */
     PFH_DEBUG_ALPHA_TILES = 5,    /* pfh_alpha_tile[batch alpha tile count] */
     PFH_DEBUG_BACKDROPS = 6,      /* pfh_backdrop_info[backdrop count], after bin */
-    PFH_DEBUG_ALPHA_RANGE = 7     /* uint32[2]: first and end alpha tile id */
+    PFH_DEBUG_ALPHA_RANGE = 7,    /* uint32[2]: first and end alpha tile id */
+    PFH_DEBUG_COVERAGE = 8        /* float[256 per alpha tile of the batch]: the coverage
+                                     checkpoint (pfh_set_gradients), pixel (x, y) of a
+                                     tile at [(y & 3) * 64 + (y >> 2) * 16 + x] */
 } pfh_debug_kind;
 pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind kind, void *dst,
                           size_t capacity, size_t *out_len);
@@ -261,6 +264,35 @@ pfh_status pfh_profile_reset(pfh_context *ctx);
 /* Waits for the stream and returns totals since the last reset, one entry per kernel. */
 pfh_status pfh_profile_read(pfh_context *ctx, pfh_kernel_time *out, uint32_t capacity,
                             uint32_t *count);
+
+/* ------------------------------------------------------------ gradients */
+
+/* No counterpart in the reference: the adjoint of fill + composite for
+ * inverse rendering. The gradient is that of the continuous model the
+ * kernels evaluate: per path and pixel the signed integral C of the winding
+ * number over the pixel (exact area; the LUT coverage sum approximates it),
+ * mask clamp(|C|, 0, 1) (min with the clip texel) and the src-over
+ * composite; 8.8 fixed point, unorm8 masks and pixels and f16 paints pass
+ * gradients through unchanged. Clip paths receive no gradient.
+ *
+ * pfh_set_gradients: before a frame, makes the fill kernel checkpoint C for
+ * every alpha-tile pixel (256 floats per alpha tile, sized with the mask). */
+pfh_status pfh_set_gradients(pfh_context *ctx, int32_t enable);
+
+/* The last frame's image as floats (byte / 255), width x height x 4, rows
+ * top down, into device memory `dst`. */
+pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst);
+
+/* Backward pass of the last frame, which must hold one draw batch composited
+ * over a clear colour and have been rendered with gradients enabled.
+ * grad_image (device): dL/d(image), width x height x 4 floats as
+ * pfh_framebuffer_to_float lays them out. Accumulates (+=) into device
+ * buffers: grad_points, 2 floats per point of the uploaded draw segments
+ * (pfh_segments.points order, which repeats each contour's first point at
+ * its end), and grad_colors, 4 floats (RGBA of the path's paint) per draw
+ * path in push order; path_count bounds it. Waits for the result. */
+pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_points,
+                        uint32_t point_count, float *grad_colors, uint32_t path_count);
 
 /* ------------------------------------------------------------ scene front-end */
 

@@ -47,6 +47,7 @@ DEBUG_FIRST_TILE_MAP = 4
 DEBUG_ALPHA_TILES = 5
 DEBUG_BACKDROPS = 6
 DEBUG_ALPHA_RANGE = 7
+DEBUG_COVERAGE = 8
 
 
 class Vector2F(C.Structure):
@@ -152,6 +153,9 @@ SIGNATURES = {
     "pfh_profile_set": (I32, [P, I32, I32]),
     "pfh_profile_reset": (I32, [P]),
     "pfh_profile_read": (I32, [P, C.POINTER(KernelTime), U32, C.POINTER(U32)]),
+    "pfh_set_gradients": (I32, [P, I32]),
+    "pfh_framebuffer_to_float": (I32, [P, P]),
+    "pfh_backward": (I32, [P, P, P, U32, P, U32]),
     "pfh_scene_create": (P, []),
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),

@@ -80,6 +80,9 @@ struct BatchArgs {
     uint32_t *mask;                // 64 words per alpha tile
     uint32_t mask_capacity;        // alpha tiles
     const float4 *lut;             // padded area LUT (259 rows of 260 texels, byte / 255)
+    float *coverage;               // optional: 256 floats per alpha tile, the signed
+                                   // coverage sum before abs/clamp (adjoint checkpoint)
+    uint32_t *tile_paths;          // optional: the batch path of every tile (adjoint)
     int32_t fb_width_tiles, fb_height_tiles;
 };
 
@@ -95,6 +98,43 @@ struct TileArgs {
     int32_t fb_width_tiles, fb_height_tiles;
     int32_t load;                  // 1: LOAD_ACTION_LOAD, 0: clear
     float clear[4];
+};
+
+// The adjoint of composite (k_tile) for one draw batch drawn over a clear
+// colour: per framebuffer tile, the layers of the sorted list.
+struct TileGradArgs {
+    TileArgs f;                    // the forward composite's inputs
+    const float *coverage;         // fill checkpoint, 256 per alpha tile
+    const uint2 *alpha_tiles;      // the batch's alpha tiles (tile, clip alpha id)
+    uint32_t alpha_start;          // frame-global id of the batch's first alpha tile
+    uint32_t alpha_count;
+    const float *grad_image;       // dL/d(image): width x height RGBA floats, rows top down
+    uint32_t width, height;
+    const uint32_t *list_offsets;  // per fb tile: first slot of its list (exclusive scan)
+    uint32_t slot_capacity;
+    uint32_t *list_tiles;          // scratch: the tile of each list slot
+    float *trans;                  // scratch: 256 floats per list slot
+    const uint32_t *tile_paths;    // the batch path of every tile (k_bound)
+    const uint4 *dice_md;          // the batch's paths: draw path id in .x
+    uint32_t path_count;
+    float *grad_cov;               // out: dL/d(signed coverage sum), 256 per alpha tile
+    uint32_t *grad_stamp;          // out: epoch of each alpha tile grad_cov holds
+    uint32_t epoch;
+    float *grad_colors;            // out, accumulated: RGBA per draw path
+    uint32_t color_count;          // draw paths held by grad_colors
+};
+
+// The adjoint of dice + coverage: per microline of the batch, the derivative
+// of the pixel coverage sums it crosses with respect to its end points,
+// carried to the scene's points.
+struct MicrolineGradArgs {
+    BatchArgs b;
+    uint32_t microline_count;
+    const float *grad_cov;         // dL/d(signed coverage sum), 256 per alpha tile
+    const uint32_t *grad_stamp;    // alpha tiles whose stamp is not epoch read as zero
+    uint32_t epoch;
+    uint32_t alpha_capacity;
+    float *grad_points;            // out, accumulated: 2 floats per scene point
 };
 
 // Every kernel launch of a frame, for per-kernel timing.
@@ -125,6 +165,15 @@ void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p);
 void launch_frame_reset(FrameState *frame, hipStream_t s);
+void launch_fill_zero(float *dst, size_t count, hipStream_t s);
+void launch_list_lengths(const uint4 *list_stats, uint32_t *lengths, uint32_t fb_tiles,
+                         hipStream_t s);
+void launch_tile_backward(const TileGradArgs &a, hipStream_t s);
+void launch_microline_backward(const MicrolineGradArgs &a, hipStream_t s);
+void launch_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words, uint32_t width,
+                                 uint32_t height, float *dst, hipStream_t s);
+// Exclusive scan of n u32 values in place; out[n] receives the total.
+void launch_scan(uint32_t *data, uint32_t n, uint32_t *partials, hipStream_t s);
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s);
 void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s);
 

@@ -524,6 +524,7 @@ __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
                 w = a.tile_path_info[p].w;
             }
             a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, w);
+            if (a.tile_paths) a.tile_paths[t] = p;
         }
         __syncthreads();
     }
@@ -1540,6 +1541,15 @@ __device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_in
     }
     a.mask[(size_t)alpha_index * 64 + lane] = unorm8_unit(c0) | (unorm8_unit(c1) << 8) |
                                               (unorm8_unit(c2) << 16) | (unorm8_unit(c3) << 24);
+    // checkpoint for the adjoint: the signed sum before abs and clamp, pixel
+    // (column lane & 15, row 4 (lane >> 4) + k) at [alpha][k][lane]
+    if (a.coverage) {
+        float *cv = a.coverage + (size_t)alpha_index * 256 + lane;
+        cv[0] = bd + acc.x;
+        cv[64] = bd + acc.y;
+        cv[128] = bd + acc.z;
+        cv[192] = bd + acc.w;
+    }
 }
 
 // The tile slot (0..63) of stream position q = pos + lane of a wavefront
@@ -1844,6 +1854,358 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
                           (unorm8(d[sub][2]) << 16) | (unorm8(d[sub][3]) << 24);
 }
 
+// ------------------------------------------------------------------ adjoint
+//
+// The gradient of the rendered image with respect to the scene's points and
+// paint colours, for one draw batch composited over a clear colour. It is the
+// derivative of the continuous model the kernels evaluate: per path and pixel
+// the signed integral C of the winding number over the pixel (exact area, of
+// which the LUT coverage sum is the fp32 approximation), mask m = clamp(|C|,
+// 0, 1) (min with the clip texel), and the src-over composite of k_tile. The
+// quantisations on the way (8.8 fixed point, unorm8 masks and pixels, f16
+// paints) pass gradients through unchanged.
+
+__global__ void k_fill_zero(float *dst, size_t count) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        dst[i] = 0.0f;
+}
+
+__global__ void k_list_lengths(const uint4 *list_stats, uint32_t *lengths, uint32_t fb_tiles) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < fb_tiles) lengths[i] = list_stats[i].y;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// The composite of k_tile, D_k = D_(k-1) (1 - a_k) + (rgb_k a_k, a_k) with
+// a_k = alpha_k m_k over the list's layers k = 0..n-1, differentiated: with
+// g = dL/dD_n and T_k = prod_(j>k) (1 - a_j),
+//   dL/da_k   = T_k (g . ((rgb_k, 1) - D_(k-1))),
+//   dL/drgb_k = g_rgb T_k a_k,   dL/dalpha_k += dL/da_k m_k   (per draw path),
+//   dL/dm_k   = dL/da_k alpha_k,
+// and for an alpha tile dL/dC = dL/dm sign(C) when 0 < |C| < 1 and the clip
+// texel does not limit m (0 otherwise). One wavefront per framebuffer tile,
+// lane (lx, ly) owns rows 4ly..4ly+3 of column lx as in k_tile. The list is
+// followed once, storing each slot's tile and a_k; T_k (top down) and the
+// gradients (bottom up, D recomputed) then iterate the slots, whose loads do
+// not depend on each other. Each alpha tile written gets this pass's epoch in
+// grad_stamp, so alpha tiles no list composited read as zero without
+// clearing grad_cov.
+__global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
+    const TileArgs &a = g.f;
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (wave >= fbn) return;
+    uint32_t lane = lane_id(), lx = lane & 15u, ly = lane >> 4;
+    uint32_t off = g.list_offsets[wave], end = g.list_offsets[wave + 1];
+    if (end > g.slot_capacity) end = g.slot_capacity;
+    uint32_t gx = wave % (uint32_t)a.fb_width_tiles, gy = wave / (uint32_t)a.fb_width_tiles;
+    uint32_t px = gx * 16 + lx, py0 = gy * 16 + ly * 4;
+    float4 gp[4];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+        uint32_t py = py0 + (uint32_t)sub;
+        gp[sub] = (px < g.width && py < g.height)
+                      ? ((const float4 *)g.grad_image)[(size_t)py * g.width + px]
+                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+
+    // the layer's paint, alpha tile and mask value per sub-row, as k_tile reads them
+    auto layer = [&](int32_t t, float4 &base, int32_t &alpha, uint32_t &color, float ma[4]) {
+        uint4 w = a.tiles[t];
+        alpha = sext24(w.z);
+        color = w.w & 0xffffu;
+        int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
+        float backdrop;
+        if (alpha >= 0) {
+            backdrop = 0.0f;
+        } else {
+            backdrop = (float)((int32_t)w.w >> 24);
+            ctrl &= ~0x3;
+        }
+        base = color < a.paint_count ? a.paints[color] : make_float4(0, 0, 0, 0);
+        int32_t mask_ctrl = ctrl & 0x3;
+        uint32_t texel = 0;
+        if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
+            texel = a.mask[(size_t)(uint32_t)alpha * 64 + lane];
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            float m = 1.0f;
+            if (mask_ctrl != 0) {
+                float cov = (float)((texel >> (8 * sub)) & 0xffu) / 255.0f + backdrop;
+                if (mask_ctrl & 0x1) {
+                    cov = fabsf(cov);
+                } else {
+                    float q = cov - 2.0f * floorf(cov / 2.0f);
+                    cov = 1.0f - fabsf(1.0f - q);
+                }
+                m = min_glsl(m, cov);
+            }
+            ma[sub] = m;
+        }
+        return (int32_t)w.x;
+    };
+
+    // walk 1: each slot's tile (stored by every lane, so each lane later reads
+    // back its own store) and a_k
+    uint32_t n = 0;
+    int32_t t = a.first_tile_map[wave];
+    for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count &&
+                             off + n < end;
+         guard++) {
+        float4 base;
+        int32_t alpha;
+        uint32_t color;
+        float ma[4];
+        int32_t next = layer(t, base, alpha, color, ma);
+        g.list_tiles[off + n] = (uint32_t)t;
+        float *slot = g.trans + (size_t)(off + n) * 256 + lane;
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) slot[sub * 64] = base.w * ma[sub];
+        n++;
+        t = next;
+    }
+    // walk 2: T_k, top down (each lane reads back only what it wrote)
+    float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+#pragma unroll 4
+    for (uint32_t k = n; k-- > 0;) {
+        float *slot = g.trans + (size_t)(off + k) * 256 + lane;
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            float al = slot[sub * 64];
+            slot[sub * 64] = T[sub];
+            T[sub] *= 1.0f - al;
+        }
+    }
+    // walk 3: gradients, bottom up
+    float D[4][4];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) D[sub][c] = a.clear[c];
+    for (uint32_t k = 0; k < n; k++) {
+        float4 base;
+        int32_t alpha;
+        uint32_t color;
+        float ma[4];
+        t = (int32_t)g.list_tiles[off + k];
+        (void)layer(t, base, alpha, color, ma);
+        const float *slot = g.trans + (size_t)(off + k) * 256 + lane;
+        bool alpha_tile = alpha >= 0 && (uint32_t)alpha >= g.alpha_start &&
+                          (uint32_t)alpha - g.alpha_start < g.alpha_count;
+        int32_t clip = -1;
+        if (alpha_tile) clip = (int32_t)g.alpha_tiles[(uint32_t)alpha - g.alpha_start].y;
+        uint32_t ctexel = 0;
+        if (clip >= 0 && (uint32_t)clip < a.mask_capacity)
+            ctexel = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+        float gr = 0.0f, gg = 0.0f, gb = 0.0f, ga = 0.0f;
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            float Tk = slot[sub * 64];
+            float al = base.w * ma[sub];
+            float4 q = gp[sub];
+            float dal = Tk * (q.x * (base.x - D[sub][0]) + q.y * (base.y - D[sub][1]) +
+                              q.z * (base.z - D[sub][2]) + q.w * (1.0f - D[sub][3]));
+            gr += q.x * Tk * al;
+            gg += q.y * Tk * al;
+            gb += q.z * Tk * al;
+            ga += dal * ma[sub];
+            if (alpha_tile) {
+                float C = g.coverage[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane];
+                float aC = fabsf(C);
+                bool limited = false;
+                if (clip >= 0) {
+                    float cv = (float)((ctexel >> (8 * sub)) & 0xffu) / 255.0f;
+                    limited = cv < min(aC, 1.0f);
+                }
+                float dC = 0.0f;
+                if (aC > 0.0f && aC < 1.0f && !limited)
+                    dC = dal * base.w * (C < 0.0f ? -1.0f : 1.0f);
+                g.grad_cov[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane] = dC;
+            }
+            float ia = 1.0f - al;
+            D[sub][0] = D[sub][0] * ia + base.x * al;
+            D[sub][1] = D[sub][1] * ia + base.y * al;
+            D[sub][2] = D[sub][2] * ia + base.z * al;
+            D[sub][3] = D[sub][3] * ia + al;
+        }
+        if (alpha_tile) g.grad_stamp[(uint32_t)alpha] = g.epoch;
+        gr = wave_sum(gr);
+        gg = wave_sum(gg);
+        gb = wave_sum(gb);
+        ga = wave_sum(ga);
+        // the tile's path (k_bound recorded it) and that path's draw path id
+        uint32_t bp = g.tile_paths[t];
+        uint32_t draw_path = bp < g.path_count ? g.dice_md[bp].x : 0xffffffffu;
+        if (lane == 0 && draw_path < g.color_count) {
+            atomicAdd(&g.grad_colors[draw_path * 4 + 0], gr);
+            atomicAdd(&g.grad_colors[draw_path * 4 + 1], gg);
+            atomicAdd(&g.grad_colors[draw_path * 4 + 2], gb);
+            atomicAdd(&g.grad_colors[draw_path * 4 + 3], ga);
+        }
+    }
+}
+
+// The derivative of C (the winding integral over a pixel) with respect to the
+// end points of one directed edge e = p1 - p0 of a closed outline: moving the
+// edge sweeps area with winding +-1, so dC/dp0 = (-e.y, e.x) W0 and dC/dp1 =
+// (-e.y, e.x) W1 with W0 = integral of (1 - t) dt and W1 = integral of t dt
+// over the parameters t in [ta, tb] where the edge lies inside the pixel
+// (image coordinates, y down; edges running towards -x add the area below
+// them). Returns W0, W1 summed over the pixels the edge crosses, each
+// weighted by dL/dC of that pixel.
+__device__ float2 microline_pixel_weights(const MicrolineGradArgs &g, uint32_t path, float x0,
+                                          float y0, float x1, float y1) {
+    const int32_t *pm = g.b.prop_md + (size_t)path * 12;
+    int32_t rx0 = pm[0], ry0 = pm[1], rx1 = pm[2], ry1 = pm[3];
+    uint32_t tile_base = (uint32_t)pm[4];
+    int32_t rw = rx1 - rx0;
+    float dx = x1 - x0, dy = y1 - y0;
+    int32_t cx = (int32_t)floorf(x0), cy = (int32_t)floorf(y0);
+    int32_t ex = (int32_t)floorf(x1), ey = (int32_t)floorf(y1);
+    int32_t sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1;
+    const float inf = __builtin_inff();
+    float tdx = dx != 0.0f ? fabsf(1.0f / dx) : inf;
+    float tdy = dy != 0.0f ? fabsf(1.0f / dy) : inf;
+    float tmx = dx > 0.0f ? ((float)(cx + 1) - x0) * tdx : (dx < 0.0f ? (x0 - (float)cx) * tdx : inf);
+    float tmy = dy > 0.0f ? ((float)(cy + 1) - y0) * tdy : (dy < 0.0f ? (y0 - (float)cy) * tdy : inf);
+    float w0 = 0.0f, w1 = 0.0f, ta = 0.0f;
+    int32_t steps = abs(ex - cx) + abs(ey - cy) + 2;
+    for (int32_t it = 0; it < steps && it < 4096; it++) {
+        float tb = min(min(tmx, tmy), 1.0f);
+        if (tb > ta && cx >= 0 && cy >= 0) {
+            int32_t tx = cx >> 4, ty = cy >> 4;
+            if (tx >= rx0 && tx < rx1 && ty >= ry0 && ty < ry1) {
+                uint32_t tile = tile_base + (uint32_t)((ty - ry0) * rw + (tx - rx0));
+                if (tile < g.b.tile_count) {
+                    int32_t alpha = sext24(g.b.tiles[tile].z);
+                    if (alpha >= 0 && (uint32_t)alpha < g.alpha_capacity &&
+                        g.grad_stamp[(uint32_t)alpha] == g.epoch) {
+                        uint32_t px = (uint32_t)(cx & 15), py = (uint32_t)(cy & 15);
+                        float gc = g.grad_cov[(size_t)(uint32_t)alpha * 256 + (py & 3u) * 64 +
+                                              (py >> 2) * 16 + px];
+                        float sq = 0.5f * (tb * tb - ta * ta);
+                        w0 += gc * ((tb - ta) - sq);
+                        w1 += gc * sq;
+                    }
+                }
+            }
+        }
+        if (tb >= 1.0f) break;
+        ta = tb;
+        if (tmx < tmy) {
+            cx += sx;
+            tmx += tdx;
+        } else {
+            cy += sy;
+            tmy += tdy;
+        }
+    }
+    return make_float2(w0, w1);
+}
+
+__device__ __forceinline__ void add_point_grad(const MicrolineGradArgs &g, uint32_t index,
+                                               float2 gq) {
+    // dL/dp = M^T dL/dq for q = M p + t (get_point)
+    const float *m = g.b.xform;
+    if (index >= g.b.scene.point_count) return;
+    atomicAdd(&g.grad_points[2 * index + 0], m[0] * gq.x + m[2] * gq.y);
+    atomicAdd(&g.grad_points[2 * index + 1], m[1] * gq.x + m[3] * gq.y);
+}
+
+// One microline per thread: its segment (the last with seg_offsets <= m), the
+// unquantised end points as k_dice_emit computes them, the pixel weights, and
+// the chain rule through de Casteljau (Bernstein weights at t = i / n and
+// (i + 1) / n), the quadratic-to-cubic elevation and the transform.
+__global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs g) {
+    const BatchArgs &a = g.b;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < g.microline_count; m += stride) {
+        uint32_t lo = 0, hi = a.segment_count + 1;  // seg_offsets[lo] <= m < seg_offsets[hi]
+        while (lo + 1 < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (a.seg_offsets[mid] <= m) lo = mid;
+            else hi = mid;
+        }
+        uint32_t seg = lo;
+        if (seg >= a.segment_count) continue;
+        uint32_t path = a.microlines[m].w;
+        if (path >= a.path_count) continue;
+        DiceSeg d;
+        if (!dice_setup(a, seg, path, d)) continue;
+        int32_t i = (int32_t)(m - a.seg_offsets[seg]);
+        if (i >= d.n) continue;
+        auto point_at = [&](int32_t k) {
+            if (k == 0) return d.p0;
+            float t = (float)k / (float)d.n;
+            if (d.curve) {
+                float2 p01 = make_float2(mixf(d.p0.x, d.c0.x, t), mixf(d.p0.y, d.c0.y, t));
+                float2 p12 = make_float2(mixf(d.c0.x, d.c1.x, t), mixf(d.c0.y, d.c1.y, t));
+                float2 p23 = make_float2(mixf(d.c1.x, d.p3.x, t), mixf(d.c1.y, d.p3.y, t));
+                float2 p012 = make_float2(mixf(p01.x, p12.x, t), mixf(p01.y, p12.y, t));
+                float2 p123 = make_float2(mixf(p12.x, p23.x, t), mixf(p12.y, p23.y, t));
+                return make_float2(mixf(p012.x, p123.x, t), mixf(p012.y, p123.y, t));
+            }
+            return make_float2(mixf(d.p0.x, d.p3.x, t), mixf(d.p0.y, d.p3.y, t));
+        };
+        float2 q0 = point_at(i), q1 = point_at(i + 1);
+        float2 w = microline_pixel_weights(g, path, q0.x, q0.y, q1.x, q1.y);
+        if (w.x == 0.0f && w.y == 0.0f) continue;
+        float2 nrm = make_float2(-(q1.y - q0.y), q1.x - q0.x);
+        float2 g0 = make_float2(nrm.x * w.x, nrm.y * w.x);  // dL/dq0
+        float2 g1 = make_float2(nrm.x * w.y, nrm.y * w.y);  // dL/dq1
+        // Bernstein weights of the control points at both ends
+        float ta = (float)i / (float)d.n, tb = (float)(i + 1) / (float)d.n;
+        uint4 md = a.dice_md[path];
+        uint2 ix = a.scene.indices[seg - md.z + md.y];
+        uint32_t from = ix.x, flags = ix.y;
+        if (d.curve) {
+            float ua = 1.0f - ta, ub = 1.0f - tb;
+            float ba[4] = {ua * ua * ua, 3.0f * ta * ua * ua, 3.0f * ta * ta * ua, ta * ta * ta};
+            float bb[4] = {ub * ub * ub, 3.0f * tb * ub * ub, 3.0f * tb * tb * ub, tb * tb * tb};
+            float2 gc[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                gc[j] = make_float2(ba[j] * g0.x + bb[j] * g1.x, ba[j] * g0.y + bb[j] * g1.y);
+            if (flags & kFlagQuad) {
+                // c0 = (p0 + 2k) / 3, c1 = (p3 + 2k) / 3
+                const float third = 1.0f / 3.0f, two_thirds = 2.0f / 3.0f;
+                add_point_grad(g, from, make_float2(gc[0].x + gc[1].x * third,
+                                                    gc[0].y + gc[1].y * third));
+                add_point_grad(g, from + 1, make_float2((gc[1].x + gc[2].x) * two_thirds,
+                                                        (gc[1].y + gc[2].y) * two_thirds));
+                add_point_grad(g, from + 2, make_float2(gc[3].x + gc[2].x * third,
+                                                        gc[3].y + gc[2].y * third));
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) add_point_grad(g, from + (uint32_t)j, gc[j]);
+            }
+        } else {
+            add_point_grad(g, from, make_float2((1.0f - ta) * g0.x + (1.0f - tb) * g1.x,
+                                                (1.0f - ta) * g0.y + (1.0f - tb) * g1.y));
+            add_point_grad(g, from + 1,
+                           make_float2(ta * g0.x + tb * g1.x, ta * g0.y + tb * g1.y));
+        }
+    }
+}
+
+// RGBA8 framebuffer rows to floats (byte / 255), width x height x 4.
+__global__ void k_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words, uint32_t width,
+                                       uint32_t height, float *dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= width * height) return;
+    uint32_t x = i % width, y = i / width;
+    uint32_t v = fb[(size_t)y * pitch_words + x];
+    float4 o = make_float4((float)(v & 0xffu) / 255.0f, (float)((v >> 8) & 0xffu) / 255.0f,
+                           (float)((v >> 16) & 0xffu) / 255.0f, (float)(v >> 24) / 255.0f);
+    ((float4 *)dst)[i] = o;
+}
+
 // ------------------------------------------------------------------ tables
 
 // The padded LUT of sample_lut_texel: texel (j, i) of the 259 x 259 table (row
@@ -2009,6 +2371,43 @@ void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p) {
     if (fbn == 0) return;
     Scope t(p, s, K_TILE);
     hipLaunchKernelGGL(k_tile, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+}
+
+void launch_fill_zero(float *dst, size_t count, hipStream_t s) {
+    if (count == 0) return;
+    uint32_t g = grid_for(count, kBlock);
+    hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(kBlock), 0, s, dst, count);
+}
+
+void launch_list_lengths(const uint4 *list_stats, uint32_t *lengths, uint32_t fb_tiles,
+                         hipStream_t s) {
+    if (fb_tiles == 0) return;
+    hipLaunchKernelGGL(k_list_lengths, dim3((fb_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       s, list_stats, lengths, fb_tiles);
+}
+
+void launch_scan(uint32_t *data, uint32_t n, uint32_t *partials, hipStream_t s) {
+    scan_exclusive(data, nullptr, n, n, partials, nullptr, s);
+}
+
+void launch_tile_backward(const TileGradArgs &a, hipStream_t s) {
+    uint32_t fbn = (uint32_t)(a.f.fb_width_tiles * a.f.fb_height_tiles);
+    if (fbn == 0) return;
+    hipLaunchKernelGGL(k_tile_backward, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+}
+
+void launch_microline_backward(const MicrolineGradArgs &a, hipStream_t s) {
+    if (a.microline_count == 0 || a.b.path_count == 0) return;
+    hipLaunchKernelGGL(k_microline_backward, dim3(grid_for(a.microline_count, kBlock)),
+                       dim3(kBlock), 0, s, a);
+}
+
+void launch_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words, uint32_t width,
+                                 uint32_t height, float *dst, hipStream_t s) {
+    uint32_t n = width * height;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_framebuffer_to_float, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       s, fb, pitch_words, width, height, dst);
 }
 
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s) {

@@ -193,6 +193,7 @@ struct Batch {
     uint2 *d_run_prev = nullptr;
     uint2 *d_tile_runs = nullptr;
     uint4 *d_tiles = nullptr;
+    uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
     uint32_t *d_col_alpha = nullptr;
     uint2 *d_alpha = nullptr;
     int32_t *d_zbuffer = nullptr;
@@ -239,6 +240,19 @@ struct pfh_context {
     size_t fb_words = 0;
     uint32_t *d_mask = nullptr;
     uint32_t cap_a = 0;
+    // adjoint: the fill's coverage checkpoint (256 floats per alpha tile,
+    // sized with the mask) and the backward pass's scratch
+    bool gradients = false;
+    float *d_coverage = nullptr;
+    float *d_grad_cov = nullptr;
+    size_t grad_cov_cap = 0;
+    float *d_trans = nullptr;
+    size_t trans_cap = 0;
+    uint32_t *d_list_tiles = nullptr;
+    uint32_t *d_grad_stamp = nullptr;  // epoch of the alpha tiles grad_cov holds
+    uint32_t grad_epoch = 0;
+    uint32_t *d_list_off = nullptr, *d_list_scan = nullptr;
+    size_t list_off_cap = 0;
     pfh::FrameState *d_frame = nullptr;
     uint32_t cap_m = 16384, cap_f = 16384;
     std::vector<std::unique_ptr<Batch>> batches;
@@ -292,7 +306,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
     dfree(ctx, b.d_path_fills);
     dfree(ctx, b.d_tile_runs);
-    dfree(ctx, b.d_tiles); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
     dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
@@ -334,12 +348,20 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     return PFH_OK;
 }
 
+pfh_status ensure_coverage(pfh_context *ctx) {
+    if (!ctx->gradients || ctx->d_coverage) return PFH_OK;
+    return dalloc(ctx, &ctx->d_coverage, (size_t)ctx->cap_a * 256);
+}
+
 pfh_status ensure_mask(pfh_context *ctx, uint32_t tiles) {
-    if (ctx->d_mask && tiles <= ctx->cap_a) return PFH_OK;
+    if (ctx->d_mask && tiles <= ctx->cap_a) return ensure_coverage(ctx);
     uint32_t cap = std::max<uint32_t>(ctx->cap_a ? ctx->cap_a : 65536, next_pow2(tiles));
     dfree(ctx, ctx->d_mask);
+    dfree(ctx, ctx->d_coverage);
+    ctx->d_coverage = nullptr;
     ctx->cap_a = cap;
-    return dalloc(ctx, &ctx->d_mask, (size_t)cap * 64);
+    pfh_status s = dalloc(ctx, &ctx->d_mask, (size_t)cap * 64);
+    return s != PFH_OK ? s : ensure_coverage(ctx);
 }
 
 Batch *find_batch(pfh_context *ctx, uint32_t id) {
@@ -404,6 +426,8 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.mask = ctx->d_mask;
     a.mask_capacity = ctx->cap_a;
     a.lut = ctx->d_lut;
+    a.coverage = ctx->gradients ? ctx->d_coverage : nullptr;
+    a.tile_paths = ctx->gradients ? b.d_tile_paths : nullptr;
     a.fb_width_tiles = ctx->fbw;
     a.fb_height_tiles = ctx->fbh;
     return a;
@@ -419,6 +443,23 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 // The front stages of a batch on its side stream. k_bound (tile
 // initialisation) runs here too unless the caller has put it on the context
 // stream (bound_done), where it overlaps dice; only bin's emit needs it.
+pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
+    pfh::TileArgs t;
+    t.tiles = b.d_tiles;
+    t.tile_count = b.host.tile_count;
+    t.mask_capacity = ctx->cap_a;
+    t.first_tile_map = b.d_ftm;
+    t.mask = ctx->d_mask;
+    t.paints = ctx->d_paints;
+    t.paint_count = ctx->paint_count;
+    t.framebuffer = ctx->d_fb;
+    t.fb_width_tiles = ctx->fbw;
+    t.fb_height_tiles = ctx->fbh;
+    t.load = b.load;
+    for (int i = 0; i < 4; i++) t.clear[i] = b.clear[i];
+    return t;
+}
+
 pfh_status run_front(pfh_context *ctx, Batch &b, hipStream_t front, bool bound_launched) {
     pfh::BatchArgs a = batch_args(ctx, b);
     pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
@@ -446,22 +487,7 @@ pfh_status run_back(pfh_context *ctx, Batch &b, hipStream_t front) {
     HIP_TRY(ctx, hipEventRecord(b.sort_done, front));
     pfh::launch_fill(a, s, p);
     HIP_TRY(ctx, hipStreamWaitEvent(s, b.sort_done, 0));
-    if (b.draw) {
-        pfh::TileArgs t;
-        t.tiles = b.d_tiles;
-        t.tile_count = b.host.tile_count;
-        t.mask_capacity = ctx->cap_a;
-        t.first_tile_map = b.d_ftm;
-        t.mask = ctx->d_mask;
-        t.paints = ctx->d_paints;
-        t.paint_count = ctx->paint_count;
-        t.framebuffer = ctx->d_fb;
-        t.fb_width_tiles = ctx->fbw;
-        t.fb_height_tiles = ctx->fbh;
-        t.load = b.load;
-        for (int i = 0; i < 4; i++) t.clear[i] = b.clear[i];
-        pfh::launch_tile(t, s, p);
-    }
+    if (b.draw) pfh::launch_tile(tile_args(ctx, b), s, p);
     HIP_TRY(ctx, hipGetLastError());
     return PFH_OK;
 }
@@ -566,6 +592,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
+    if (ctx->gradients && (s = dalloc(ctx, &b.d_tile_paths, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_alpha, h.tile_count)) != PFH_OK) return s;
@@ -859,6 +886,12 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
     case PFH_DEBUG_ALPHA_TILES: src = b->d_alpha; bytes = (size_t)b->counters.alpha * 8; break;
     case PFH_DEBUG_BACKDROPS: src = b->d_backdrops; bytes = b->host.backdrops.size() * 12; break;
     case PFH_DEBUG_ALPHA_RANGE: bytes = 8; break;
+    case PFH_DEBUG_COVERAGE:
+        if (!ctx->gradients || !ctx->d_coverage)
+            return fail(ctx, PFH_ERROR_STATE, "no coverage checkpoint (pfh_set_gradients)");
+        src = ctx->d_coverage + (size_t)b->counters.alpha_start * 256;
+        bytes = (size_t)std::min(b->counters.alpha, ctx->cap_a - std::min(b->counters.alpha_start, ctx->cap_a)) * 1024;
+        break;
     default: return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad debug kind %d", (int)kind);
     }
     *out_len = bytes;
@@ -888,6 +921,135 @@ pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
     if (capacity_words < *out_words)
         return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "buffer too small");
     if (alpha) HIP_TRY(ctx, hipMemcpy(dst, ctx->d_mask, *out_words * 4, hipMemcpyDeviceToHost));
+    return PFH_OK;
+}
+
+pfh_status pfh_set_gradients(pfh_context *ctx, int32_t enable) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "set_gradients inside a scene");
+    ctx->gradients = enable != 0;
+    if (!ctx->gradients) return PFH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return ensure_mask(ctx, 0);
+}
+
+pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst) {
+    if (!ctx || !dst) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->d_fb) return fail(ctx, PFH_ERROR_STATE, "no framebuffer yet");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    pfh::launch_framebuffer_to_float(ctx->d_fb, (uint32_t)ctx->fbw * 16, (uint32_t)ctx->width,
+                                     (uint32_t)ctx->height, dst, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PFH_OK;
+}
+
+pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_points,
+                        uint32_t point_count, float *grad_colors, uint32_t path_count) {
+    if (!ctx || !grad_image || !grad_points || !grad_colors)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "backward inside a scene");
+    if (!ctx->gradients || !ctx->d_coverage)
+        return fail(ctx, PFH_ERROR_STATE, "the frame was rendered without pfh_set_gradients");
+    Batch *draw = nullptr;
+    for (auto &b : ctx->batches) {
+        if (!b->draw) continue;
+        if (draw) return fail(ctx, PFH_ERROR_STATE, "backward supports one draw batch per frame");
+        draw = b.get();
+    }
+    if (!draw) return fail(ctx, PFH_ERROR_STATE, "no draw batch in the frame");
+    if (draw->load) return fail(ctx, PFH_ERROR_STATE, "backward needs a frame drawn over a clear colour");
+    if (point_count < ctx->draw_points_host.size())
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "grad_points holds %u points, the scene %zu",
+                    point_count, ctx->draw_points_host.size());
+
+    if (!draw->d_tile_paths)
+        return fail(ctx, PFH_ERROR_STATE, "the draw batch was prepared before pfh_set_gradients");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Batch &b = *draw;
+    hipStream_t s = ctx->stream;
+    pfh_status st;
+    // list slots per framebuffer tile: exclusive scan of the sorted lists' lengths
+    uint32_t fbn = (uint32_t)(ctx->fbw * ctx->fbh);
+    if (ctx->list_off_cap < (size_t)fbn + 1) {
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        dfree(ctx, ctx->d_list_off);
+        dfree(ctx, ctx->d_list_scan);
+        ctx->d_list_off = ctx->d_list_scan = nullptr;
+        if ((st = dalloc(ctx, &ctx->d_list_off, (size_t)fbn + 1)) != PFH_OK) return st;
+        if ((st = dalloc(ctx, &ctx->d_list_scan, pfh::scan_partials_words(fbn))) != PFH_OK)
+            return st;
+        ctx->list_off_cap = (size_t)fbn + 1;
+    }
+    pfh::launch_list_lengths(b.d_list_stats, ctx->d_list_off, fbn, s);
+    pfh::launch_scan(ctx->d_list_off, fbn, ctx->d_list_scan, s);
+    uint32_t slots = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&slots, ctx->d_list_off + fbn, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (ctx->trans_cap < (size_t)slots * 256 || !ctx->d_trans) {
+        dfree(ctx, ctx->d_trans);
+        dfree(ctx, ctx->d_list_tiles);
+        ctx->d_trans = nullptr;
+        ctx->d_list_tiles = nullptr;
+        ctx->trans_cap = 0;
+        size_t n = std::max<size_t>(slots, 1);
+        if ((st = dalloc(ctx, &ctx->d_trans, n * 256)) != PFH_OK) return st;
+        if ((st = dalloc(ctx, &ctx->d_list_tiles, n)) != PFH_OK) return st;
+        ctx->trans_cap = n * 256;
+    }
+    uint32_t alpha_end = 0;
+    for (auto &x : ctx->batches) alpha_end = std::max(alpha_end, x->counters.frame_alpha_end);
+    alpha_end = std::min(alpha_end, ctx->cap_a);
+    if (ctx->grad_cov_cap < (size_t)alpha_end * 256 || !ctx->d_grad_cov) {
+        dfree(ctx, ctx->d_grad_cov);
+        dfree(ctx, ctx->d_grad_stamp);
+        ctx->d_grad_cov = nullptr;
+        ctx->d_grad_stamp = nullptr;
+        ctx->grad_cov_cap = 0;
+        size_t n = std::max<size_t>(alpha_end, 1);
+        if ((st = dalloc(ctx, &ctx->d_grad_cov, n * 256)) != PFH_OK) return st;
+        if ((st = dalloc(ctx, &ctx->d_grad_stamp, n)) != PFH_OK) return st;
+        ctx->grad_cov_cap = n * 256;
+        ctx->grad_epoch = 0;
+    }
+    // grad_cov entries count only where their stamp is this pass's epoch
+    if (++ctx->grad_epoch <= 1) {  // fresh stamps, or the counter wrapped
+        pfh::launch_fill_zero((float *)ctx->d_grad_stamp, ctx->grad_cov_cap / 256, s);
+        ctx->grad_epoch = 1;
+    }
+    pfh::TileGradArgs tg;
+    tg.f = tile_args(ctx, b);
+    tg.coverage = ctx->d_coverage;
+    tg.alpha_tiles = b.d_alpha;
+    tg.alpha_start = b.counters.alpha_start;
+    tg.alpha_count = b.counters.alpha;
+    tg.grad_image = grad_image;
+    tg.width = (uint32_t)ctx->width;
+    tg.height = (uint32_t)ctx->height;
+    tg.list_offsets = ctx->d_list_off;
+    tg.slot_capacity = slots;
+    tg.trans = ctx->d_trans;
+    tg.list_tiles = ctx->d_list_tiles;
+    tg.grad_stamp = ctx->d_grad_stamp;
+    tg.epoch = ctx->grad_epoch;
+    tg.tile_paths = b.d_tile_paths;
+    tg.dice_md = (const uint4 *)b.d_dice;
+    tg.path_count = b.host.path_count;
+    tg.grad_cov = ctx->d_grad_cov;
+    tg.grad_colors = grad_colors;
+    tg.color_count = path_count;
+    pfh::launch_tile_backward(tg, s);
+    pfh::MicrolineGradArgs mg;
+    mg.b = batch_args(ctx, b);
+    mg.microline_count = b.counters.microline_count;
+    mg.grad_cov = ctx->d_grad_cov;
+    mg.grad_stamp = ctx->d_grad_stamp;
+    mg.epoch = ctx->grad_epoch;
+    mg.alpha_capacity = alpha_end;
+    mg.grad_points = grad_points;
+    pfh::launch_microline_backward(mg, s);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(s));
     return PFH_OK;
 }
 

@@ -553,6 +553,22 @@ class Renderer:
     def synchronize(self):
         self._check(self._lib.pfh_synchronize(self._h))
 
+    def set_gradients(self, enable=True):
+        """Checkpoint the fill's coverage sums so pfh_backward can run on the
+        frames that follow."""
+        self._check(self._lib.pfh_set_gradients(self._h, 1 if enable else 0))
+
+    def framebuffer_to_float(self, dst_ptr):
+        """The last frame as width x height x 4 floats into device memory."""
+        self._check(self._lib.pfh_framebuffer_to_float(self._h, C.c_void_p(dst_ptr)))
+
+    def backward(self, grad_image_ptr, grad_points_ptr, point_count, grad_colors_ptr,
+                 path_count):
+        """pfh_backward on device pointers (accumulates into the outputs)."""
+        self._check(self._lib.pfh_backward(self._h, C.c_void_p(grad_image_ptr),
+                                           C.c_void_p(grad_points_ptr), point_count,
+                                           C.c_void_p(grad_colors_ptr), path_count))
+
     def read_pixels(self):
         d = self.options.dest
         out = np.zeros((d.height, d.width, 4), np.uint8)

@@ -62,6 +62,46 @@ def small_scene(seed=SEED_BASE + 0, paths=256, size=1024):
     return scene, (size, size), ColorF(1.0, 1.0, 1.0, 1.0)
 
 
+def diff_paths(seed=SEED_BASE + 4, paths=1024, size=512, sigma=4.0):
+    """The differentiable-render workload: `paths` closed paths of 8-32 cubic
+    Beziers with control points uniform on the size x size canvas (the small
+    scene's distribution on this canvas), sRGB colours (half opaque, the rest
+    alpha U[0.2, 0.9]) as floats in [0, 1], and target points: a copy with
+    every point moved by N(0, sigma) px. Returns a dict of arrays: flags,
+    contour_ends, path_contour_ends, points (N, 2), colors (P, 4),
+    target_points."""
+    rng = np.random.default_rng(seed)
+    pts, flags, ends, pends = [], [], [], []
+    npts = 0
+    for p in range(paths):
+        k = int(rng.integers(8, 33))
+        start = rng.uniform(0, size, 2)
+        ctrl = rng.uniform(0, size, (k, 3, 2))
+        ctrl[-1, 2] = start
+        path_pts = [start]
+        path_fl = [0]
+        for j in range(k):
+            path_pts += [ctrl[j, 0], ctrl[j, 1], ctrl[j, 2]]
+            path_fl += [L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0]
+        pts.append(np.asarray(path_pts, np.float32))
+        flags.append(np.asarray(path_fl, np.uint8))
+        npts += len(path_pts)
+        ends.append(npts)
+        pends.append(len(ends))
+    rgb = rng.integers(0, 256, (paths, 3)) / 255.0
+    alpha = np.where(rng.random(paths) < 0.5, 1.0, rng.uniform(0.2, 0.9, paths))
+    points = np.concatenate(pts)
+    return {
+        "flags": np.concatenate(flags),
+        "contour_ends": np.asarray(ends, np.uint32),
+        "path_contour_ends": np.asarray(pends, np.uint32),
+        "points": points,
+        "colors": np.concatenate([rgb, alpha[:, None]], axis=1).astype(np.float32),
+        "target_points": (points + rng.normal(0.0, sigma, points.shape)).astype(np.float32),
+        "size": (size, size),
+    }
+
+
 def _blobs(rng, cx, cy, radius, k, quad_frac, jitter, ctrl_jitter):
     """Closed star-shaped contours, one per path, path i with k[i] mixed
     quadratic/cubic segments. Returns (points, flags, contour_ends)."""

@@ -1,0 +1,166 @@
+"""The HIP adjoint (pfh_backward through pathfinder_amd.diff) against the FP64
+CPU backward of oracle/diff.py.
+
+The CPU backward is fed the GPU forward's own state: its masks (the unorm8
+texels the composite read), its coverage checkpoint C and the paints as the
+kernels hold them (u8, then f16), with dL/dimage from the GPU's image. The
+two then evaluate the same formulas on the same inputs (fp32 against FP64, and
+microline end points computed in fp32 against FP64), so the gradients must
+agree closely. The FP64 formulas themselves are pinned against finite
+differences in tests/test_diff_oracle.py.
+
+At the 1024-path 512 x 512 workload, where the CPU model is too slow, a
+gradient step must lower the loss (a size-independent property).
+"""
+import numpy as np
+import pytest
+
+from oracle import diff as odiff
+from oracle import exact, packing
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _blob_arrays(rng, paths, size, k=5):
+    pts, flags, ends, pends = [], [], [], []
+    for p in range(paths):
+        c = rng.uniform(size * 0.25, size * 0.75, 2)
+        r = rng.uniform(size * 0.1, size * 0.3)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+        on = [c + r * np.array([np.cos(a), np.sin(a)]) for a in ang]
+        path, fl = [on[0]], [0]
+        for j in range(k):
+            a, b = on[j], on[(j + 1) % k]
+            path += [a + (b - a) / 3 + rng.normal(0, 3, 2), a + 2 * (b - a) / 3 + rng.normal(0, 3, 2), b]
+            fl += [1, 2, 0]
+        path, fl = path[:-1], fl[:-1]
+        pts += path
+        flags += fl
+        ends.append(len(pts))
+        pends.append(len(ends))
+    return np.asarray(pts, np.float32), np.asarray(flags, np.uint8), ends, pends
+
+
+def _f16_colors(colors):
+    from pathfinder_amd.diff import colors_to_u8
+    c = colors_to_u8(colors).astype(np.float32) * np.float32(1.0 / 255.0)
+    return c.astype(np.float16).astype(np.float64)
+
+
+def _gpu_layers(dr, scene, W, H, path_count):
+    """Per path (C, m) as the GPU forward left them."""
+    from pathfinder_amd import _lib as L
+    r = dr.renderer
+    cmds = packing.build(scene)
+    (kind, b), = [kb for kb in cmds["batches"] if kb[0] == 4]
+    bid = b["batch_id"]
+    tiles = r.read_debug(bid, L.DEBUG_TILES).view(np.uint32).reshape(-1, 4)
+    a0 = int(r.read_debug(bid, L.DEBUG_ALPHA_RANGE).view(np.uint32)[0])
+    cov = r.read_debug(bid, L.DEBUG_COVERAGE).view(np.float32).reshape(-1, 256)
+    mask = r.read_mask()
+    md = np.asarray(b["propagate_metadata"]).astype(np.int64)
+    dice = np.asarray(b["dice_metadata"]).astype(np.int64)
+    out = [(np.zeros((H, W)), np.zeros((H, W))) for _ in range(path_count)]
+    yy, xx = np.mgrid[0:16, 0:16]
+    order = (yy & 3) * 64 + (yy >> 2) * 16 + xx
+    for bp in range(len(md)):
+        C, m = out[int(dice[bp, 0])]
+        x0, y0, x1, y1, off = (int(v) for v in md[bp, :5])
+        for ty in range(y0, y1):
+            for tx in range(x0, x1):
+                t = off + (ty - y0) * (x1 - x0) + (tx - x0)
+                w2, w3 = int(tiles[t, 2]), int(tiles[t, 3])
+                alpha = ((w2 << 8) & 0xffffffff) >> 8
+                alpha = alpha - (1 << 24) if alpha >= 1 << 23 else alpha
+                bd = ((w3 >> 24) & 0xff) - (256 if (w3 >> 24) & 0x80 else 0)
+                if alpha >= 0:
+                    cv = cov[alpha - a0][order].astype(np.float64)
+                    mk = exact.mask_tile(mask, alpha) / 255.0
+                else:
+                    cv = np.full((16, 16), float(bd))
+                    mk = np.full((16, 16), 1.0 if bd != 0 else 0.0)
+                ys, xs = slice(ty * 16, min(ty * 16 + 16, H)), slice(tx * 16, min(tx * 16 + 16, W))
+                h, w = ys.stop - ys.start, xs.stop - xs.start
+                if h > 0 and w > 0:
+                    C[ys, xs] = cv[:h, :w]
+                    m[ys, xs] = mk[:h, :w]
+    return out
+
+
+@pytest.mark.parametrize("seed,paths,size", [(11, 3, 64), (12, 8, 96)])
+def test_gpu_gradient_matches_cpu_backward(seed, paths, size):
+    import torch
+
+    from pathfinder_amd import diff
+    _say("case diff gpu vs cpu", seed, paths, size)
+    rng = np.random.default_rng(seed)
+    pts, flags, ends, pends = _blob_arrays(rng, paths, size)
+    colors = np.concatenate([rng.uniform(0, 1, (paths, 3)),
+                             rng.uniform(0.3, 1.0, (paths, 1))], axis=1).astype(np.float32)
+    shapes = diff.Shapes(flags, ends, pends)
+    bg = (1.0, 1.0, 1.0, 1.0)
+    dr = diff.DiffRenderer(shapes, size, size, bg)
+    try:
+        p_t = torch.tensor(pts, device="cuda", requires_grad=True)
+        c_t = torch.tensor(colors, device="cuda", requires_grad=True)
+        img = diff.render(dr, p_t, c_t)
+        target = torch.full_like(img, 0.5)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        gp = p_t.grad.cpu().numpy().astype(np.float64)
+        gc = c_t.grad.cpu().numpy().astype(np.float64)
+        img_np = img.detach().cpu().numpy().astype(np.float64)
+        scene = dr.scene(pts, colors)
+        layers = _gpu_layers(dr, scene, size, size, paths)
+    finally:
+        dr.close()
+    osh = odiff.Shapes(flags, ends, pends)
+    flat = odiff.flatten(pts.astype(np.float64), osh)
+    cols16 = _f16_colors(colors)
+    # the CPU composite of the GPU masks reproduces the GPU image
+    cpu_img, _ = odiff.composite([m for C, m in layers], cols16, size, size, bg)
+    assert np.abs(cpu_img - img_np).max() <= 1.0 / 255.0 + 1e-6
+    g = 2.0 * (img_np - 0.5)
+    cp, cc = odiff.backward(pts.astype(np.float64), cols16, osh, size, size, bg, g,
+                            (flat, layers))
+    ep = np.linalg.norm(gp - cp) / max(np.linalg.norm(cp), 1e-12)
+    ec = np.linalg.norm(gc - cc) / max(np.linalg.norm(cc), 1e-12)
+    _say("points: |cpu| %.4g rel err %.3g; colours: |cpu| %.4g rel err %.3g" %
+         (np.linalg.norm(cp), ep, np.linalg.norm(cc), ec))
+    assert np.linalg.norm(cp) > 0 and np.linalg.norm(cc) > 0
+    assert ep <= 2e-3, ep
+    assert ec <= 1e-4, ec
+
+
+def test_gradient_step_lowers_loss_at_512():
+    import torch
+
+    from pathfinder_amd import diff, synth
+    d = synth.diff_paths()
+    size = d["size"]
+    _say("case diff 1024 paths", size)
+    shapes = diff.Shapes(d["flags"], d["contour_ends"], d["path_contour_ends"])
+    dr = diff.DiffRenderer(shapes, size[0], size[1], (1.0, 1.0, 1.0, 1.0))
+    try:
+        with torch.no_grad():
+            target = dr.render(torch.tensor(d["target_points"]), torch.tensor(d["colors"]))
+        p_t = torch.tensor(d["points"], device="cuda", requires_grad=True)
+        c_t = torch.tensor(d["colors"], device="cuda", requires_grad=True)
+        img = diff.render(dr, p_t, c_t)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        g = p_t.grad
+        assert torch.isfinite(g).all() and torch.isfinite(c_t.grad).all()
+        assert g.abs().sum() > 0
+        with torch.no_grad():
+            step = 0.5 / g.abs().max()
+            img2 = dr.render(p_t - step * g, c_t)
+            loss2 = ((img2 - target) ** 2).sum()
+        _say("loss %.3f -> %.3f" % (loss.item(), loss2.item()))
+        assert loss2.item() < loss.item()
+    finally:
+        dr.close()
