@@ -23,6 +23,15 @@ device memory), backward = pfh_backward (composite and coverage adjoint),
 ratio = (forward + backward) / forward, all device time with the scene
 resident. step_ms is one whole optimisation step through torch.autograd
 (host packing, uploads, forward, loss, backward).
+
+batch_glyphs / batch_canvases: many items of uneven sizes packed into frames
+(pathfinder_amd.batch): 50,000 glyphs at 8-72 px on 4096 x 4096 atlas pages,
+and 512 canvases of 64^2-1024^2 with 10-2000 paths each on 8192-wide frames
+(a tile budget per frame keeps the 24-bit alpha tile ids). One frame renders
+all of its items with the pipeline's fixed launch sequence; aggregate
+Msegments/s = all microlines / the frames' summed device time (re-rendered,
+resident). per_item_ms: device time of 32 sampled items rendered alone
+(p50 / p99).
 """
 import argparse
 import json
@@ -180,6 +189,67 @@ def diff_leg(steps, warmup):
             "loss_first_last": [round(losses[0], 2), round(losses[-1], 2)]}
 
 
+def batch_leg(kind, steps, warmup, sample=32):
+    import numpy as np
+
+    from pathfinder_amd import (ColorF, DestFramebuffer, RectF, Renderer, RendererOptions,
+                                Scene, batch, synth)
+    from pathfinder_amd.scene import ColorU, PathArrays
+    if kind == "glyphs":
+        items = synth.glyphs()
+        fw = fh = 4096
+        budget, align, bg = 6_000_000, 1, ColorF(0, 0, 0, 0)
+    else:
+        items = synth.canvases()
+        fw, fh = 8192, 8192
+        budget, align, bg = 14_000_000, 16, ColorF(1, 1, 1, 1)
+    t0 = time.perf_counter()
+    frames = batch.build_frames(items, fw, fh, budget, align=align)
+    pack_s = time.perf_counter() - t0
+
+    def timed(scene, w, h):
+        r = Renderer(RendererOptions(DestFramebuffer.full_window(w, h), bg))
+        try:
+            cmds = scene.build()
+            r.begin_scene()
+            r.render_commands(cmds)
+            r.end_scene()
+            for _ in range(max(warmup, 1)):
+                r.rerender()
+            r.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                r.rerender()
+            r.synchronize()
+            return (time.perf_counter() - t) / steps * 1e3, r.stats()["microline_count"]
+        finally:
+            r.close()
+
+    frame_ms, ml = [], 0
+    for f in frames:
+        ms, m = timed(f.scene, f.width, f.height)
+        frame_ms.append(round(ms, 3))
+        ml += m
+    rng = np.random.default_rng(0)
+    per = []
+    for i in rng.choice(len(items), min(sample, len(items)), replace=False):
+        it = items[int(i)]
+        s = Scene()
+        s.set_view_box(RectF(0, 0, it.width, it.height))
+        a = it.arrays
+        paints = np.asarray([s.push_paint(ColorU(*c)) for c in it.colors], np.uint32)
+        s.push_draw_paths(PathArrays(a.points, a.flags, a.contour_ends, a.path_contour_ends,
+                                     paints, a.fill_rules))
+        per.append(timed(s, it.width, it.height)[0])
+    total = sum(frame_ms)
+    return {"workload": "%s-%d" % (kind, len(items)), "items": len(items),
+            "frames": len(frames), "frame_size": [fw, fh], "microlines": ml,
+            "frames_ms": round(total, 3), "aggregate_msegments_s": round(ml / total / 1e3, 1),
+            "frame_ms": frame_ms, "pack_s": round(pack_s, 2),
+            "per_item_ms_p50": round(float(np.percentile(per, 50)), 3),
+            "per_item_ms_p99": round(float(np.percentile(per, 99)), 3), "sampled": len(per)}
+
+
 def run_diff_leg(steps, warmup):
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--diff-leg", "--steps", str(steps),
@@ -202,6 +272,8 @@ def main():
     ap.add_argument("--no-diff", action="store_true",
                     help="skip the differentiable-render leg")
     ap.add_argument("--diff-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the glyph-atlas and multi-canvas legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the frames, run a plain 256 MiB device-to-device copy so a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE")
@@ -323,6 +395,10 @@ def main():
     }
     if diff_result is not None:
         line["diff"] = diff_result
+    if not args.no_batch:
+        for kind in ("glyphs", "canvases"):
+            print("bench: batch leg %s" % kind, file=sys.stderr, flush=True)
+            line["batch_" + kind] = batch_leg(kind, min(args.steps, 10), args.warmup)
     if not args.no_cpu_baseline:
         from oracle import packing
         cmds = packing.build(scene)

@@ -10,6 +10,12 @@ run (and the oracle) sees the same inputs.
   4k      3840x2160, 20,000 star-shaped paths (log-uniform 10-600 mixed
           quadratic/cubic segments, log-uniform sizes 4-2000 px), 60% opaque,
           20% even-odd, 500 rectangle/ellipse clip paths; about 5M microlines.
+  glyphs  50,000 glyphs (1-4 quadratic contours with holes) at 8-72 px, per
+          glyph 2x3 affine, for atlas pages (pathfinder_amd.batch).
+  canvases 512 canvases of log-uniform 64^2-1024^2, 10-2000 small-scene paths
+          each (pathfinder_amd.batch).
+  diff    1024 closed cubic paths on 512x512 with colours and a perturbed
+          target, the differentiable-render workload.
 """
 import math
 
@@ -100,6 +106,106 @@ def diff_paths(seed=SEED_BASE + 4, paths=1024, size=512, sigma=4.0):
         "target_points": (points + rng.normal(0.0, sigma, points.shape)).astype(np.float32),
         "size": (size, size),
     }
+
+
+def _cubic_paths(rng, n, size):
+    """n closed paths of 8-32 cubics with control points uniform in
+    [0, size)^2, the last cubic returning to the start (as small_scene),
+    in flat PathArrays form without paints."""
+    k = rng.integers(8, 33, n)
+    sizes = 1 + 3 * k
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(sizes.sum())
+    pts = rng.uniform(0, size, (total, 2)).astype(np.float32)
+    pts[starts + sizes - 1] = pts[starts]
+    local = np.arange(total) - np.repeat(starts, sizes)
+    pattern = np.asarray([L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0], np.uint8)
+    flags = np.where(local == 0, 0, pattern[(local - 1) % 3]).astype(np.uint8)
+    ends = np.cumsum(sizes).astype(np.uint32)
+    return PathArrays(pts, flags, ends, np.arange(1, n + 1, dtype=np.uint32))
+
+
+def _rgba(rng, n):
+    rgb = rng.integers(0, 256, size=(n, 3))
+    alpha = np.where(rng.random(n) < 0.5, 255, np.round(rng.uniform(0.2, 0.9, n) * 255))
+    return [(int(r), int(g), int(b), int(a)) for (r, g, b), a in zip(rgb, alpha.astype(int))]
+
+
+def canvases(seed=SEED_BASE + 3, count=512, size_lo=64, size_hi=1024, paths_lo=10,
+             paths_hi=2000):
+    """The multi-scene workload: `count` square canvases, side log-uniform in
+    [size_lo, size_hi], each with log-uniform [paths_lo, paths_hi] closed paths
+    of the small scene's distribution on that canvas. Returns batch.Items."""
+    from .batch import Item
+    rng = np.random.default_rng(seed)
+    items = []
+    for c in range(count):
+        side = int(round(math.exp(rng.uniform(math.log(size_lo), math.log(size_hi)))))
+        n = int(round(math.exp(rng.uniform(math.log(paths_lo), math.log(paths_hi)))))
+        items.append(Item(_cubic_paths(rng, n, side), _rgba(rng, n), side, side))
+    return items
+
+
+def _glyph_contour(rng, cx, cy, radius, m, reverse):
+    """A closed star-shaped contour of m quadratics around (cx, cy)."""
+    ang = np.sort(rng.uniform(0, 2 * np.pi, m))
+    if reverse:
+        ang = ang[::-1]
+    r = radius * rng.uniform(0.8, 1.0, m)
+    on = np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], axis=1)
+    nxt = np.roll(on, -1, axis=0)
+    mid = 0.5 * (on + nxt)
+    ctrl = np.asarray([cx, cy]) + (mid - np.asarray([cx, cy])) * rng.uniform(0.95, 1.15, (m, 1))
+    pts = np.empty((2 * m, 2))
+    pts[0::2] = on
+    pts[1::2] = ctrl
+    flags = np.zeros(2 * m, np.uint8)
+    flags[1::2] = L.POINT_CONTROL_0
+    return pts, flags
+
+
+def glyphs(seed=SEED_BASE + 2, count=50000, em=2048.0, size_lo=8, size_hi=72):
+    """The glyph workload: `count` synthetic glyphs, each one path of 1-4
+    closed contours of 10-40 quadratics in a 2048-unit em box (an outer
+    contour and reversed holes inside it), drawn at a size uniform in
+    [size_lo, size_hi] px through a per-glyph 2x3 affine (scale, rotation and
+    shear within +-0.2) into a cell around its transformed bounds. White,
+    opaque. Returns batch.Items."""
+    from .batch import Item
+    rng = np.random.default_rng(seed)
+    items = []
+    for g in range(count):
+        size = int(rng.integers(size_lo, size_hi + 1))
+        nc = int(rng.integers(1, 5))
+        c = em / 2
+        outer = em * 0.42
+        pts, flags, ends = [], [], []
+        p, f = _glyph_contour(rng, c, c, outer, int(rng.integers(10, 41)), False)
+        pts.append(p)
+        flags.append(f)
+        ends.append(len(p))
+        for h in range(nc - 1):
+            a = rng.uniform(0, 2 * np.pi)
+            d = rng.uniform(0, 0.35) * outer
+            p, f = _glyph_contour(rng, c + d * np.cos(a), c + d * np.sin(a),
+                                  rng.uniform(0.12, 0.22) * outer, int(rng.integers(10, 41)),
+                                  True)
+            pts.append(p)
+            flags.append(f)
+            ends.append(ends[-1] + len(p))
+        pts = np.concatenate(pts)
+        sc = size / em
+        rot, shear = rng.uniform(-0.2, 0.2, 2)
+        m = np.asarray([[math.cos(rot), -math.sin(rot)], [math.sin(rot), math.cos(rot)]]) @ \
+            np.asarray([[sc, shear * sc], [0.0, sc]])
+        q = pts @ m.T
+        lo = q.min(axis=0)
+        q = q - lo + 1.0
+        w, h = (int(math.ceil(v)) + 1 for v in q.max(axis=0))
+        arr = PathArrays(q.astype(np.float32), np.concatenate(flags), np.asarray(ends, np.uint32),
+                         np.asarray([len(ends)], np.uint32))
+        items.append(Item(arr, [(255, 255, 255, 255)], w, h))
+    return items
 
 
 def _blobs(rng, cx, cy, radius, k, quad_frac, jitter, ctrl_jitter):
