@@ -202,6 +202,32 @@ pfh_status pfh_end_scene(pfh_context *ctx);
 /* Re-runs every batch of the last scene from device-resident inputs (no host
  * uploads, no host synchronisation). Used to time frames. */
 pfh_status pfh_rerender_scene(pfh_context *ctx);
+
+/* No counterpart in the reference: with graphs enabled, pfh_rerender_scene
+ * captures the frame's launches on the context's three streams into a HIP
+ * graph on its first call and replays the graph afterwards (per-kernel
+ * timing keeps the launch-by-launch path). The graph is dropped at
+ * pfh_begin_scene, when buffers grow, and when gradients are switched. */
+pfh_status pfh_set_graphs(pfh_context *ctx, int32_t enable);
+
+/* No counterpart in the reference (its dispatch sizes are fixed by the GL
+ * driver): the launch shape of this context's frames. grid_cap bounds the
+ * workgroups of the grid-stride kernels (batch init, dice, bound, bin,
+ * propagate), fill_grid those of the fill kernel (one wavefront per 64
+ * alpha tiles at a time). Defaults 16384 and 8192. */
+typedef struct pfh_launch_shape {
+    uint32_t grid_cap;
+    uint32_t fill_grid;
+} pfh_launch_shape;
+pfh_status pfh_set_launch_shape(pfh_context *ctx, const pfh_launch_shape *shape);
+pfh_status pfh_get_launch_shape(pfh_context *ctx, pfh_launch_shape *shape);
+
+/* The autotuner: re-renders the last scene `frames` times per candidate
+ * (1024, 2048, 4096, 8192, 16384 workgroups), first for fill_grid, then for
+ * grid_cap, timing whole frames with events on the context stream; keeps
+ * the fastest shape in the context and returns it in `chosen` (may be
+ * NULL). Results do not depend on the shape. */
+pfh_status pfh_autotune(pfh_context *ctx, uint32_t frames, pfh_launch_shape *chosen);
 pfh_status pfh_synchronize(pfh_context *ctx);
 
 /* Reads the destination framebuffer, RGBA8, top row = scene row 0

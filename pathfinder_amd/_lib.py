@@ -126,6 +126,10 @@ class Outline(C.Structure):
                 ("contour_ends", C.c_void_p), ("contour_count", C.c_uint32)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("grid_cap", C.c_uint32), ("fill_grid", C.c_uint32)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int32
@@ -154,6 +158,10 @@ SIGNATURES = {
     "pfh_profile_reset": (I32, [P]),
     "pfh_profile_read": (I32, [P, C.POINTER(KernelTime), U32, C.POINTER(U32)]),
     "pfh_set_gradients": (I32, [P, I32]),
+    "pfh_set_graphs": (I32, [P, I32]),
+    "pfh_set_launch_shape": (I32, [P, C.POINTER(LaunchShape)]),
+    "pfh_get_launch_shape": (I32, [P, C.POINTER(LaunchShape)]),
+    "pfh_autotune": (I32, [P, U32, C.POINTER(LaunchShape)]),
     "pfh_framebuffer_to_float": (I32, [P, P]),
     "pfh_backward": (I32, [P, P, P, U32, P, U32]),
     "pfh_scene_create": (P, []),

@@ -84,6 +84,9 @@ struct BatchArgs {
                                    // coverage sum before abs/clamp (adjoint checkpoint)
     uint32_t *tile_paths;          // optional: the batch path of every tile (adjoint)
     int32_t fb_width_tiles, fb_height_tiles;
+    // launch shape (pfh_launch_shape)
+    uint32_t grid_cap;             // workgroups of the grid-stride kernels, at most
+    uint32_t fill_grid;            // workgroups of k_fill, at most
 };
 
 struct TileArgs {

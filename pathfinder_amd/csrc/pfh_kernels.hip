@@ -64,10 +64,11 @@ __device__ __forceinline__ int32_t sext24(uint32_t w) { return (int32_t)(w << 8)
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
-inline uint32_t grid_for(uint64_t items, uint32_t per_block) {
+inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap = kMaxGrid) {
     uint64_t g = (items + per_block - 1) / per_block;
     if (g < 1) g = 1;
-    if (g > kMaxGrid) g = kMaxGrid;
+    if (cap < 1) cap = 1;
+    if (g > cap) g = cap;
     return (uint32_t)g;
 }
 
@@ -2269,11 +2270,11 @@ void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint64_t n = (uint64_t)a.fb_width_tiles * a.fb_height_tiles + 8;
     if ((uint64_t)a.column_count * 3 + 1 > n) n = (uint64_t)a.column_count * 3 + 1;
     Scope t(p, s, K_BATCH_INIT);
-    hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock, a.grid_cap)), dim3(kBlock), 0, s, a);
 }
 
 void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.segment_count, kBlock);
+    uint32_t g = grid_for(a.segment_count, kBlock, a.grid_cap);
     {
         Scope t(p, s, K_DICE_COUNT);
         hipLaunchKernelGGL(k_dice_count, dim3(g), dim3(kBlock), 0, s, a);
@@ -2293,12 +2294,13 @@ void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
 void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (a.tile_count == 0) return;
     Scope t(p, s, K_BOUND);
-    hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock * kBoundItems)), dim3(kBlock),
+    hipLaunchKernelGGL(k_bound, dim3(grid_for(a.tile_count, kBlock * kBoundItems, a.grid_cap)),
+                       dim3(kBlock),
                        0, s, a);
 }
 
 void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.microline_capacity, kBlock);
+    uint32_t g = grid_for(a.microline_capacity, kBlock, a.grid_cap);
     {
         Scope t(p, s, K_BIN_COUNT);
         hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
@@ -2311,7 +2313,7 @@ void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p) {
 }
 
 void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.microline_capacity, kBlock);
+    uint32_t g = grid_for(a.microline_capacity, kBlock, a.grid_cap);
     {
         Scope t(p, s, K_BIN_EMIT);
         hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);
@@ -2325,7 +2327,7 @@ void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {
 }
 
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.column_count, kBlock);
+    uint32_t g = grid_for(a.column_count, kBlock, a.grid_cap);
     if (a.column_count) {
         Scope t(p, s, K_PROPAGATE_COUNT);
         hipLaunchKernelGGL(k_propagate_count, dim3(g), dim3(kBlock), 0, s, a);
@@ -2360,8 +2362,7 @@ void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p) {
 
 void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (a.tile_count == 0) return;
-    uint32_t g = grid_for((uint64_t)a.tile_count * 64, kBlock);
-    if (g > 8192) g = 8192;
+    uint32_t g = grid_for((uint64_t)a.tile_count * 64, kBlock, a.fill_grid);
     Scope t(p, s, K_FILL);
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, s, a);
 }

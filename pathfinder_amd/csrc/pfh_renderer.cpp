@@ -243,6 +243,12 @@ struct pfh_context {
     // adjoint: the fill's coverage checkpoint (256 floats per alpha tile,
     // sized with the mask) and the backward pass's scratch
     bool gradients = false;
+    // launch shape of the grid-stride kernels and of k_fill (pfh_autotune)
+    pfh_launch_shape shape = {16384, 8192};
+    // pfh_rerender_scene through a captured graph of the frame
+    bool use_graphs = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     float *d_coverage = nullptr;
     float *d_grad_cov = nullptr;
     size_t grad_cov_cap = 0;
@@ -430,6 +436,8 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.tile_paths = ctx->gradients ? b.d_tile_paths : nullptr;
     a.fb_width_tiles = ctx->fbw;
     a.fb_height_tiles = ctx->fbh;
+    a.grid_cap = ctx->shape.grid_cap;
+    a.fill_grid = ctx->shape.fill_grid;
     return a;
 }
 
@@ -505,7 +513,25 @@ pfh_status start_frame_on_side_streams(pfh_context *ctx) {
     return PFH_OK;
 }
 
-pfh_status replay_frame(pfh_context *ctx) {
+void drop_graph(pfh_context *ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+}
+
+// Sizes every batch's capacity buffers (allocation happens here, never while
+// a frame is being captured).
+pfh_status size_all(pfh_context *ctx) {
+    for (auto &b : ctx->batches) {
+        pfh_status s = size_capacity_buffers(ctx, *b);
+        if (s != PFH_OK) return s;
+    }
+    return PFH_OK;
+}
+
+// The frame's launches for batches whose buffers are sized (capturable).
+pfh_status enqueue_frame(pfh_context *ctx) {
     pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
     if (ctx->d_fb_saved)
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_fb, ctx->d_fb_saved, ctx->fb_words * 4,
@@ -519,7 +545,6 @@ pfh_status replay_frame(pfh_context *ctx) {
     size_t n = ctx->batches.size();
     for (size_t i = 0; i < n; i++) {
         Batch &b = *ctx->batches[i];
-        if ((s = size_capacity_buffers(ctx, b)) != PFH_OK) return s;
         pfh::BatchArgs a = batch_args(ctx, b);
         pfh::launch_bound(a, ctx->stream, ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof);
         HIP_TRY(ctx, hipEventRecord(b.bound_done, ctx->stream));
@@ -533,6 +558,33 @@ pfh_status replay_frame(pfh_context *ctx) {
         if ((s = run_front(ctx, *ctx->batches[i], ctx->side[i & 1], true)) != PFH_OK) return s;
     for (size_t i = 0; i < n; i++)
         if ((s = run_back(ctx, *ctx->batches[i], ctx->side[i & 1])) != PFH_OK) return s;
+    return PFH_OK;
+}
+
+pfh_status replay_frame(pfh_context *ctx) {
+    pfh_status s = size_all(ctx);
+    return s != PFH_OK ? s : enqueue_frame(ctx);
+}
+
+// Captures the frame's launches on the three streams into a graph (the side
+// streams join through the frame-start event and leave through the events
+// the context stream waits on) and instantiates it.
+pfh_status capture_frame(pfh_context *ctx) {
+    pfh_status s = size_all(ctx);
+    if (s != PFH_OK) return s;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (hipStream_t side : ctx->side) HIP_TRY(ctx, hipStreamSynchronize(side));
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    s = enqueue_frame(ctx);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (s != PFH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    if (e != hipSuccess) return fail(ctx, PFH_ERROR_HIP, "frame capture: %s", hipGetErrorString(e));
+    ctx->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
     return PFH_OK;
 }
 
@@ -663,6 +715,7 @@ void pfh_context_destroy(pfh_context *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (hipStream_t side : ctx->side)
         if (side) (void)hipStreamSynchronize(side);
+    drop_graph(ctx);
     free_batches(ctx);
     ctx->pool.release_all();
     for (hipStream_t side : ctx->side)
@@ -682,6 +735,7 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     for (hipStream_t side : ctx->side) HIP_TRY(ctx, hipStreamSynchronize(side));
+    drop_graph(ctx);
     free_batches(ctx);
     int32_t fbw = (frame->width + 15) / 16, fbh = (frame->height + 15) / 16;
     size_t words = (size_t)fbw * fbh * 256;
@@ -820,6 +874,7 @@ pfh_status pfh_end_scene(pfh_context *ctx) {
     bool overflow = false;
     pfh_status s = collect(ctx, &overflow);
     for (int round = 0; s == PFH_OK && overflow && round < 4; round++) {
+        drop_graph(ctx);  // capacities grew: a captured frame is stale
         ctx->replays++;
         if ((s = replay_frame(ctx)) != PFH_OK) break;
         s = collect(ctx, &overflow);
@@ -846,7 +901,104 @@ pfh_status pfh_end_scene(pfh_context *ctx) {
 pfh_status pfh_rerender_scene(pfh_context *ctx) {
     if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
     if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "rerender inside a scene");
-    return replay_frame(ctx);
+    // per-kernel timing records events, which a graph would not replay as timings
+    if (!ctx->use_graphs || ctx->prof.mode != PFH_PROFILE_OFF) return replay_frame(ctx);
+    if (!ctx->graph_exec) {
+        pfh_status s = capture_frame(ctx);
+        if (s != PFH_OK) {
+            drop_graph(ctx);
+            return s;
+        }
+    }
+    HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    return PFH_OK;
+}
+
+pfh_status pfh_set_launch_shape(pfh_context *ctx, const pfh_launch_shape *shape) {
+    if (!ctx || !shape) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (shape->grid_cap < 1 || shape->grid_cap > 65536 || shape->fill_grid < 1 ||
+        shape->fill_grid > 65536)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "launch shape out of range (1..65536)");
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "set_launch_shape inside a scene");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->shape = *shape;
+    return PFH_OK;
+}
+
+pfh_status pfh_get_launch_shape(pfh_context *ctx, pfh_launch_shape *shape) {
+    if (!ctx || !shape) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    *shape = ctx->shape;
+    return PFH_OK;
+}
+
+// Device time of `frames` re-rendered frames, between events on the context
+// stream (every side stream's work joins it before the frame ends).
+static pfh_status time_frames(pfh_context *ctx, uint32_t frames, float *ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, PFH_ERROR_HIP, "hipEventCreate: %s", hipGetErrorString(err));
+    }
+    pfh_status s = replay_frame(ctx);  // warm-up, and any buffer sizing
+    if (s == PFH_OK) err = hipEventRecord(e0, ctx->stream);
+    for (uint32_t i = 0; s == PFH_OK && err == hipSuccess && i < frames; i++) s = replay_frame(ctx);
+    if (s == PFH_OK && err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+    if (s == PFH_OK && err == hipSuccess) err = hipEventSynchronize(e1);
+    if (s == PFH_OK && err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s != PFH_OK) return s;
+    if (err != hipSuccess) return fail(ctx, PFH_ERROR_HIP, "timing: %s", hipGetErrorString(err));
+    return PFH_OK;
+}
+
+pfh_status pfh_autotune(pfh_context *ctx, uint32_t frames, pfh_launch_shape *chosen) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "autotune inside a scene");
+    if (ctx->batches.empty()) return fail(ctx, PFH_ERROR_STATE, "autotune needs a rendered scene");
+    if (frames < 1) frames = 1;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    drop_graph(ctx);
+    int saved_prof = ctx->prof.mode;
+    ctx->prof.mode = PFH_PROFILE_OFF;
+    static const uint32_t kCandidates[] = {1024, 2048, 4096, 8192, 16384};
+    pfh_launch_shape best = ctx->shape;
+    pfh_status s = PFH_OK;
+    // one parameter at a time: the fill kernel's grid, then the other kernels' cap
+    for (int param = 0; param < 2 && s == PFH_OK; param++) {
+        float best_ms = 0.0f;
+        uint32_t best_v = param == 0 ? best.fill_grid : best.grid_cap;
+        bool have = false;
+        for (uint32_t v : kCandidates) {
+            ctx->shape = best;
+            (param == 0 ? ctx->shape.fill_grid : ctx->shape.grid_cap) = v;
+            float ms = 0.0f;
+            if ((s = time_frames(ctx, frames, &ms)) != PFH_OK) break;
+            if (!have || ms < best_ms) {
+                best_ms = ms;
+                best_v = v;
+                have = true;
+            }
+        }
+        (param == 0 ? best.fill_grid : best.grid_cap) = best_v;
+    }
+    ctx->shape = best;
+    ctx->prof.mode = saved_prof;
+    if (s != PFH_OK) return s;
+    if (chosen) *chosen = best;
+    return PFH_OK;
+}
+
+pfh_status pfh_set_graphs(pfh_context *ctx, int32_t enable) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "set_graphs inside a scene");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->use_graphs = enable != 0;
+    return PFH_OK;
 }
 
 pfh_status pfh_synchronize(pfh_context *ctx) {
@@ -927,6 +1079,7 @@ pfh_status pfh_read_mask(pfh_context *ctx, uint32_t *dst, size_t capacity_words,
 pfh_status pfh_set_gradients(pfh_context *ctx, int32_t enable) {
     if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
     if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "set_gradients inside a scene");
+    drop_graph(ctx);
     ctx->gradients = enable != 0;
     if (!ctx->gradients) return PFH_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));

@@ -553,6 +553,25 @@ class Renderer:
     def synchronize(self):
         self._check(self._lib.pfh_synchronize(self._h))
 
+    def launch_shape(self):
+        s = L.LaunchShape()
+        self._check(self._lib.pfh_get_launch_shape(self._h, C.byref(s)))
+        return {"grid_cap": s.grid_cap, "fill_grid": s.fill_grid}
+
+    def set_launch_shape(self, grid_cap, fill_grid):
+        s = L.LaunchShape(grid_cap, fill_grid)
+        self._check(self._lib.pfh_set_launch_shape(self._h, C.byref(s)))
+
+    def autotune(self, frames=5):
+        """Picks the fastest launch shape for the last scene (pfh_autotune)."""
+        s = L.LaunchShape()
+        self._check(self._lib.pfh_autotune(self._h, frames, C.byref(s)))
+        return {"grid_cap": s.grid_cap, "fill_grid": s.fill_grid}
+
+    def set_graphs(self, enable=True):
+        """Re-render frames through a captured HIP graph."""
+        self._check(self._lib.pfh_set_graphs(self._h, 1 if enable else 0))
+
     def set_gradients(self, enable=True):
         """Checkpoint the fill's coverage sums so pfh_backward can run on the
         frames that follow."""
