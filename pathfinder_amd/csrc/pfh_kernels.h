@@ -66,6 +66,7 @@ struct BatchArgs {
     uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)
     uint2 *run_prev;               // at a run's first sorted index: the previous run or (0, 0)
     uint4 *tiles;                  // tile_count
+    uint8_t *tile_flags;           // tile_count: 1 when bin gave the tile a fill
     uint32_t *col_alpha_offsets;   // column_count + 1
     uint2 *alpha_tiles;            // tile_count
     int32_t *zbuffer;              // 8 + fb tiles
@@ -161,7 +162,8 @@ void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_bound(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p);  // count, scan
-void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p);   // emit, fill lists
+void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p);
+void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p);  // per-tile fill lists
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_row_paths(const BatchArgs &a, hipStream_t s, Profiler *p);
 void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p);

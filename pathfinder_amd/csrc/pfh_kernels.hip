@@ -525,6 +525,7 @@ __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
                 w = a.tile_path_info[p].w;
             }
             a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, w);
+            a.tile_flags[t] = 0;
             if (a.tile_paths) a.tile_paths[t] = p;
         }
         __syncthreads();
@@ -597,6 +598,7 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
                     a.fills[(size_t)f * 3 + 0] = fw.x;
                     a.fills[(size_t)f * 3 + 1] = fw.y;
                     a.fill_tiles[f] = tile;
+                    a.tile_flags[tile] = 1;
                 }
             }
             count++;
@@ -617,6 +619,7 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
                     a.fills[(size_t)f * 3 + 0] = fw.x;
                     a.fills[(size_t)f * 3 + 1] = fw.y;
                     a.fill_tiles[f] = tile;
+                    a.tile_flags[tile] = 1;
                 }
             }
             count++;
@@ -744,8 +747,7 @@ struct LinkLds {
 
 template <typename Key>
 __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t f1,
-                          uint32_t tile_base, uint32_t width, uint32_t col_base,
-                          bool unclipped) {
+                          uint32_t tile_base) {
     using K = LinkKey<Key>;
     constexpr uint32_t kChunk = K::kChunk;
     constexpr uint32_t kPer = kChunk / kBlock;  // fills (and keys) per thread
@@ -942,11 +944,6 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
                     prev = a.tile_runs[tile];
                 }
                 a.tiles[tile].y = base + end - 1;
-                // An unclipped path gives a tile an alpha tile exactly when it has
-                // fills (propagate.cs.glsl:130-131), so the column's alpha count is
-                // its number of tiles with fills.
-                if (link == 0xffffffffu && unclipped)
-                    atomicAdd(&a.col_alpha_offsets[col_base + lt % width], 1u);
                 uint32_t slot = L.out[p];
                 uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
                 // bit 31 of the count: an earlier range of this tile exists, and
@@ -980,15 +977,12 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
     if (f0 == f1) return;
     const int32_t *pm = a.prop_md + (size_t)path * 12;
-    bool unclipped = pm[7] < 0;
     uint32_t width = (uint32_t)(pm[2] - pm[0]), height = (uint32_t)(pm[3] - pm[1]);
-    if (width == 0) width = 1;
     uint64_t rect = (uint64_t)width * height;
     if (rect <= kLinkRect32)
-        link_path<uint32_t>(a, lds, f0, f1, (uint32_t)pm[4], width, (uint32_t)pm[8], unclipped);
+        link_path<uint32_t>(a, lds, f0, f1, (uint32_t)pm[4]);
     else
-        link_path<unsigned long long>(a, lds, f0, f1, (uint32_t)pm[4], width, (uint32_t)pm[8],
-                                      unclipped);
+        link_path<unsigned long long>(a, lds, f0, f1, (uint32_t)pm[4]);
 }
 
 // ------------------------------------------------------------------ propagate
@@ -1031,6 +1025,7 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
     for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
         uint4 tl[kPropRows];
         uint2 cl[kPropRows];
+        uint8_t fl[kPropRows];
         // unconditional loads at clamped indices (a load under a branch is
         // waited for before the branch joins); rows past the column or the
         // buffers are not used below
@@ -1039,6 +1034,7 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
             uint32_t y = y0 + u;
             uint32_t t = dofs + y * width + (uint32_t)tile_x;
             tl[u] = a.tiles[min(t, a.tile_count - 1u)];
+            fl[u] = a.tile_flags[min(t, a.tile_count - 1u)];
             if (cload) {
                 uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;
                 uint32_t ct = cofs + (uy - cry0) * (crx1 - crx0) + (ux - crx0);
@@ -1058,11 +1054,11 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
             }
             uint4 tile = tl[u];
             int32_t draw_alpha = -1, clip_alpha = -1;
-            int32_t first_fill = (int32_t)tile.y;
             int32_t delta = (int32_t)tile.z >> 24;
             uint32_t word = tile.w & 0x00ffffffu;
             int32_t tile_backdrop = backdrop;
-            bool have_mask = first_fill >= 0;
+            // first_fill >= 0 in the shader: bin gave the tile a fill
+            bool have_mask = fl[u] != 0;
             bool need_alpha = have_mask;
             if (clip) {
                 uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;  // unsigned, as the shader
@@ -1113,15 +1109,11 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
     return count;
 }
 
-// Alpha tile counts of the columns of clipped paths (k_fill_link has counted
-// the others).
+// Alpha tile counts per column.
 __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
-         c += gridDim.x * blockDim.x) {
-        uint32_t dpath = (uint32_t)a.backdrops[(size_t)c * 3 + 2];
-        if (dpath < a.path_count && a.prop_md[(size_t)dpath * 12 + 7] < 0) continue;
+         c += gridDim.x * blockDim.x)
         a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
-    }
 }
 
 // Allocates the batch's alpha tile range in the frame (uFirstAlphaTileIndex,
@@ -2318,12 +2310,14 @@ void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {
         Scope t(p, s, K_BIN_EMIT);
         hipLaunchKernelGGL(k_bin_emit, dim3(g), dim3(kBlock), 0, s, a);
     }
-    if (a.path_count) {
-        Scope t(p, s, K_FILL_LINK);
-        hipLaunchKernelGGL(k_path_fills, dim3((a.path_count + kBlock) / kBlock), dim3(kBlock), 0,
-                           s, a);
-        hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
-    }
+}
+
+void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p) {
+    if (!a.path_count) return;
+    Scope t(p, s, K_FILL_LINK);
+    hipLaunchKernelGGL(k_path_fills, dim3((a.path_count + kBlock) / kBlock), dim3(kBlock), 0, s,
+                       a);
+    hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
 }
 
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {

@@ -174,6 +174,8 @@ struct Batch {
     hipEvent_t prop_done = nullptr;   // its propagate stage finished
     hipEvent_t sort_done = nullptr;   // its sort stage finished
     hipEvent_t bound_done = nullptr;  // its tiles were initialised (k_bound on the context stream)
+    hipEvent_t link_done = nullptr;   // its per-tile fill lists are built
+    hipEvent_t fill_done = nullptr;   // its masks are written (fill on its side stream)
     int32_t load = 1;
     float clear[4] = {0, 0, 0, 0};
     // device inputs
@@ -194,6 +196,7 @@ struct Batch {
     uint2 *d_tile_runs = nullptr;
     uint4 *d_tiles = nullptr;
     uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
+    uint8_t *d_tile_flags = nullptr;   // 1 per tile bin gave a fill
     uint32_t *d_col_alpha = nullptr;
     uint2 *d_alpha = nullptr;
     int32_t *d_zbuffer = nullptr;
@@ -302,7 +305,8 @@ void dfree(pfh_context *ctx, T *&p) {
 }
 
 void free_batch(pfh_context *ctx, Batch &b) {
-    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done}) {
+    for (hipEvent_t *e :
+         {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done, &b.link_done, &b.fill_done}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
@@ -313,6 +317,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_path_fills);
     dfree(ctx, b.d_tile_runs);
     dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_tile_flags);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
     dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
@@ -419,6 +424,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.run_prev = b.d_run_prev;
     a.path_fills = b.d_path_fills;
     a.tiles = b.d_tiles;
+    a.tile_flags = b.d_tile_flags;
     a.col_alpha_offsets = b.d_col_alpha;
     a.alpha_tiles = b.d_alpha;
     a.zbuffer = b.d_zbuffer;
@@ -478,6 +484,10 @@ pfh_status run_front(pfh_context *ctx, Batch &b, hipStream_t front, bool bound_l
     if (bound_launched) HIP_TRY(ctx, hipStreamWaitEvent(front, b.bound_done, 0));
     pfh::launch_bin_emit(a, front, p);
     HIP_TRY(ctx, hipEventRecord(b.front_done, front));
+    // the per-tile fill lists build beside propagate (which needs only bin's
+    // tile flags); fill waits for them
+    pfh::launch_fill_link(a, front, p);
+    HIP_TRY(ctx, hipEventRecord(b.link_done, front));
     return PFH_OK;
 }
 
@@ -493,10 +503,22 @@ pfh_status run_back(pfh_context *ctx, Batch &b, hipStream_t front) {
     HIP_TRY(ctx, hipStreamWaitEvent(front, b.prop_done, 0));
     pfh::launch_sort(a, front, p);
     HIP_TRY(ctx, hipEventRecord(b.sort_done, front));
+    HIP_TRY(ctx, hipStreamWaitEvent(s, b.link_done, 0));
     pfh::launch_fill(a, s, p);
-    HIP_TRY(ctx, hipStreamWaitEvent(s, b.sort_done, 0));
-    if (b.draw) pfh::launch_tile(tile_args(ctx, b), s, p);
+    // only the composite reads the sorted lists; a clip batch's sort joins
+    // the context stream at the end of the frame (join_side_streams)
+    if (b.draw) {
+        HIP_TRY(ctx, hipStreamWaitEvent(s, b.sort_done, 0));
+        pfh::launch_tile(tile_args(ctx, b), s, p);
+    }
     HIP_TRY(ctx, hipGetLastError());
+    return PFH_OK;
+}
+
+// Makes the context stream wait for every batch's sort (the last work of
+// each side stream in a frame), so waiting for it waits for the frame.
+pfh_status join_side_streams(pfh_context *ctx) {
+    for (auto &b : ctx->batches) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, b->sort_done, 0));
     return PFH_OK;
 }
 
@@ -556,9 +578,50 @@ pfh_status enqueue_frame(pfh_context *ctx) {
     });
     for (size_t i : order)
         if ((s = run_front(ctx, *ctx->batches[i], ctx->side[i & 1], true)) != PFH_OK) return s;
-    for (size_t i = 0; i < n; i++)
-        if ((s = run_back(ctx, *ctx->batches[i], ctx->side[i & 1])) != PFH_OK) return s;
-    return PFH_OK;
+    // Back stages. Propagate runs in batch order on the context stream (it
+    // allocates each batch's frame-global alpha tile range, clip batches
+    // first). Every batch but the last fills its masks on its side stream
+    // once its propagate is done, so the next batch's propagate does not wait
+    // for it; the last batch fills on the context stream after them (its
+    // clip masks), beside its sort. Composites follow in batch order.
+    pfh::Profiler *p = ctx->prof.mode == PFH_PROFILE_OFF ? nullptr : &ctx->prof;
+    for (size_t i = 0; i < n; i++) {
+        Batch &b = *ctx->batches[i];
+        pfh::BatchArgs a = batch_args(ctx, b);
+        HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, b.front_done, 0));
+        pfh::launch_propagate(a, ctx->stream, p);
+        HIP_TRY(ctx, hipEventRecord(b.prop_done, ctx->stream));
+    }
+    for (size_t i = 0; i < n; i++) {
+        Batch &b = *ctx->batches[i];
+        pfh::BatchArgs a = batch_args(ctx, b);
+        hipStream_t side = ctx->side[i & 1];
+        pfh::launch_row_paths(a, side, p);
+        HIP_TRY(ctx, hipStreamWaitEvent(side, b.prop_done, 0));
+        if (i + 1 < n) {
+            for (size_t j = 0; j < i; j++)  // earlier batches' masks (its clip masks)
+                HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[j]->fill_done, 0));
+            pfh::launch_fill(a, side, p);
+            HIP_TRY(ctx, hipEventRecord(b.fill_done, side));
+        }
+        pfh::launch_sort(a, side, p);
+        HIP_TRY(ctx, hipEventRecord(b.sort_done, side));
+    }
+    if (n) {
+        Batch &last = *ctx->batches[n - 1];
+        for (size_t j = 0; j + 1 < n; j++)
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->batches[j]->fill_done, 0));
+        HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, last.link_done, 0));
+        pfh::launch_fill(batch_args(ctx, last), ctx->stream, p);
+    }
+    for (size_t i = 0; i < n; i++) {
+        Batch &b = *ctx->batches[i];
+        if (!b.draw) continue;
+        HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, b.sort_done, 0));
+        pfh::launch_tile(tile_args(ctx, b), ctx->stream, p);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return join_side_streams(ctx);
 }
 
 pfh_status replay_frame(pfh_context *ctx) {
@@ -629,7 +692,8 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     size_t fbn = (size_t)ctx->fbw * ctx->fbh;
     // the batch's inputs go up on the side stream its front stages run on
     hipStream_t front = ctx->side[ctx->batches.size() & 1];
-    for (hipEvent_t *e : {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done}) {
+    for (hipEvent_t *e :
+         {&b.front_done, &b.prop_done, &b.sort_done, &b.bound_done, &b.link_done, &b.fill_done}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             *e = nullptr;
             return fail(ctx, PFH_ERROR_HIP, "hipEventCreateWithFlags failed");
@@ -644,6 +708,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tile_flags, h.tile_count)) != PFH_OK) return s;
     if (ctx->gradients && (s = dalloc(ctx, &b.d_tile_paths, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
@@ -841,6 +906,8 @@ pfh_status pfh_prepare_and_draw_tiles(pfh_context *ctx, const pfh_tile_batch *ba
 
 static pfh_status collect(pfh_context *ctx, bool *overflow) {
     *overflow = false;
+    pfh_status js = join_side_streams(ctx);
+    if (js != PFH_OK) return js;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     uint64_t max_m = 0, max_f = 0, alpha_end = 0;
     for (auto &b : ctx->batches) {
