@@ -745,6 +745,105 @@ struct LinkLds {
     uint32_t scan[4];
 };
 
+// The rest of a chunk once its sorted run keys are in sk: output slots, tile
+// ranges and run-start links, every fill to its slot, and the barrier before
+// the next chunk.
+template <typename Key, uint32_t kPer>
+__device__ __forceinline__ void link_finish(const BatchArgs &a, LinkLds &L, const Key *sk,
+                                            uint32_t base, uint32_t n, uint32_t runs,
+                                            uint32_t tile_base, uint32_t f0, uint32_t start_bits,
+                                            const uint32_t (&rid)[kPer],
+                                            const uint2 (&fw)[kPer]) {
+    using K = LinkKey<Key>;
+    uint32_t tid = threadIdx.x;
+
+    // Output slots: exclusive scan of the run lengths in sorted order
+    // (thread tid holds sorted runs tid * kPer .. + kPer - 1).
+    uint32_t p0 = tid * kPer;
+    uint32_t len[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        uint32_t p = p0 + k;
+        len[k] = 0;
+        if (p < runs) {
+            uint32_t rr = K::run(sk[p]);
+            len[k] = (uint32_t)L.run_first[rr + 1] - L.run_first[rr];
+            L.rank[rr] = (uint16_t)p;
+        }
+        sum += len[k];
+    }
+    uint32_t total;
+    uint32_t o = block_exclusive_scan(sum, L.scan, &total);
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        if (p0 + k < runs) L.out[p0 + k] = (uint16_t)o;
+        o += len[k];
+    }
+    __syncthreads();
+
+    // Tile ranges and the links of run starts, one sorted run per thread.
+    // k_bound leaves every fill list empty, so in the path's first chunk a
+    // tile has no earlier fill and no earlier range.
+    bool first_chunk = base == f0;
+    for (uint32_t p = tid; p < runs; p += kBlock) {
+        Key cur = sk[p];
+        uint32_t lt = K::tile(cur), rr = K::run(cur);
+        uint32_t first = L.run_first[rr];
+        uint32_t link;
+        if (p == 0 || K::tile(sk[p - 1]) != lt) {
+            // last sorted run of this tile: galloping, then binary search
+            uint32_t lo = p, hi = runs, step = 1;
+            for (;;) {
+                uint32_t nx = lo + step;
+                if (nx >= runs) break;
+                if (K::tile(sk[nx]) != lt) {
+                    hi = nx;
+                    break;
+                }
+                lo = nx;
+                step <<= 1;
+            }
+            while (lo + 1 < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (K::tile(sk[mid]) == lt) lo = mid;
+                else hi = mid;
+            }
+            uint32_t rq = K::run(sk[lo]);
+            uint32_t end = L.run_first[rq + 1];  // one past the tile's last fill
+            uint32_t tile = tile_base + lt;
+            link = 0xffffffffu;
+            uint2 prev = make_uint2(0, 0);
+            if (!first_chunk) {
+                link = a.tiles[tile].y;
+                prev = a.tile_runs[tile];
+            }
+            a.tiles[tile].y = base + end - 1;
+            uint32_t slot = L.out[p];
+            uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
+            // bit 31 of the count: an earlier range of this tile exists, and
+            // only then is run_prev of this range read
+            a.tile_runs[tile] =
+                make_uint2(base + slot, count | (link != 0xffffffffu ? 0x80000000u : 0u));
+            if (link != 0xffffffffu) a.run_prev[base + slot] = prev;
+        } else {
+            link = base + L.run_first[K::run(sk[p - 1]) + 1] - 1;
+        }
+        a.fills[(size_t)(base + first) * 3 + 2] = link;
+    }
+    // Every fill to its slot; a fill inside a run links to the one before.
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        uint32_t i = k * kBlock + tid;
+        if (i < n) {
+            uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
+            a.sorted_fills[base + slot] = fw[k];
+            if (!(start_bits >> k & 1u)) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
+        }
+    }
+    __syncthreads();
+}
+
 template <typename Key>
 __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t f1,
                           uint32_t tile_base) {
@@ -803,7 +902,10 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
         if (tid == 0) L.run_first[runs] = (uint16_t)n;
         __syncthreads();
 
-        // Bitonic sort of the runs' keys, padded to size. Key e lives in
+        // Bitonic sort of the runs' keys, padded to size. Up to 256 runs,
+        // wavefront 0 sorts them alone in registers (element q * 64 + lane of
+        // register q: lane shuffles under 64 apart, register swaps above) and
+        // the other waves wait at one barrier. Otherwise key e lives in
         // register q of lane l of wave w, e = q * 64W + w * 64 + l with
         // W = min(size / 64, 4) waves: partners at distance < 64 are exchanged
         // across lanes, at distance >= 64W within the thread, and the
@@ -811,6 +913,52 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
         // step). The input is in keys1 and the first LDS step writes keys0.
         uint32_t size = 64;
         while (size < runs) size <<= 1;
+        if (size <= 256) {
+            if (wave == 0) {
+                uint32_t nq = size >> 6;
+                Key w4[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; q++) {
+                    uint32_t e = q * 64u + lane;
+                    w4[q] = (q < nq && e < runs) ? keys1[e] : kPad;
+                }
+                for (uint32_t k = 2; k <= size; k <<= 1) {
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                        if (j < 64u) {
+#pragma unroll
+                            for (uint32_t q = 0; q < 4; q++) {
+                                if (q < nq) {
+                                    Key o = shfl_xor_key(w4[q], j);
+                                    uint32_t e = q * 64u + lane;
+                                    bool keep_min = ((lane & j) == 0) == ((e & k) == 0);
+                                    w4[q] = keep_min ? min(w4[q], o) : max(w4[q], o);
+                                }
+                            }
+                        } else {
+                            uint32_t rj = j >> 6;
+#pragma unroll
+                            for (uint32_t q = 0; q < 4; q++) {
+                                if ((q & rj) == 0 && (q | rj) < nq) {
+                                    uint32_t e = q * 64u + lane;
+                                    Key x = w4[q], y = w4[q | rj];
+                                    bool up = (e & k) == 0;
+                                    w4[q] = up ? min(x, y) : max(x, y);
+                                    w4[q | rj] = up ? max(x, y) : min(x, y);
+                                }
+                            }
+                        }
+                    }
+                }
+                Key *out0 = reinterpret_cast<Key *>(L.keys[0]);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; q++)
+                    if (q < nq) out0[q * 64u + lane] = w4[q];
+            }
+            __syncthreads();
+            link_finish<Key>(a, L, reinterpret_cast<Key *>(L.keys[0]), base, n, runs, tile_base,
+                             f0, start_bits, rid, fw);
+            continue;
+        }
         uint32_t lw = size >= 256 ? 2u : (size == 128 ? 1u : 0u);
         uint32_t nw = 1u << lw, span = 64u << lw, nr = size >> (6 + lw);
         bool active = wave < nw;
@@ -881,93 +1029,16 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
                 if (q < nr) sk[q * span + wave * 64u + lane] = v[q];
         }
         __syncthreads();
-
-        // Output slots: exclusive scan of the run lengths in sorted order
-        // (thread tid holds sorted runs tid * kPer .. + kPer - 1).
-        uint32_t p0 = tid * kPer;
-        uint32_t len[kPer];
-        uint32_t sum = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; k++) {
-            uint32_t p = p0 + k;
-            len[k] = 0;
-            if (p < runs) {
-                uint32_t rr = K::run(sk[p]);
-                len[k] = (uint32_t)L.run_first[rr + 1] - L.run_first[rr];
-                L.rank[rr] = (uint16_t)p;
-            }
-            sum += len[k];
-        }
-        uint32_t total;
-        uint32_t o = block_exclusive_scan(sum, L.scan, &total);
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; k++) {
-            if (p0 + k < runs) L.out[p0 + k] = (uint16_t)o;
-            o += len[k];
-        }
-        __syncthreads();
-
-        // Tile ranges and the links of run starts, one sorted run per thread.
-        // k_bound leaves every fill list empty, so in the path's first chunk a
-        // tile has no earlier fill and no earlier range.
-        bool first_chunk = base == f0;
-        for (uint32_t p = tid; p < runs; p += kBlock) {
-            Key cur = sk[p];
-            uint32_t lt = K::tile(cur), rr = K::run(cur);
-            uint32_t first = L.run_first[rr];
-            uint32_t link;
-            if (p == 0 || K::tile(sk[p - 1]) != lt) {
-                // last sorted run of this tile: galloping, then binary search
-                uint32_t lo = p, hi = runs, step = 1;
-                for (;;) {
-                    uint32_t nx = lo + step;
-                    if (nx >= runs) break;
-                    if (K::tile(sk[nx]) != lt) {
-                        hi = nx;
-                        break;
-                    }
-                    lo = nx;
-                    step <<= 1;
-                }
-                while (lo + 1 < hi) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if (K::tile(sk[mid]) == lt) lo = mid;
-                    else hi = mid;
-                }
-                uint32_t rq = K::run(sk[lo]);
-                uint32_t end = L.run_first[rq + 1];  // one past the tile's last fill
-                uint32_t tile = tile_base + lt;
-                link = 0xffffffffu;
-                uint2 prev = make_uint2(0, 0);
-                if (!first_chunk) {
-                    link = a.tiles[tile].y;
-                    prev = a.tile_runs[tile];
-                }
-                a.tiles[tile].y = base + end - 1;
-                uint32_t slot = L.out[p];
-                uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
-                // bit 31 of the count: an earlier range of this tile exists, and
-                // only then is run_prev of this range read
-                a.tile_runs[tile] =
-                    make_uint2(base + slot, count | (link != 0xffffffffu ? 0x80000000u : 0u));
-                if (link != 0xffffffffu) a.run_prev[base + slot] = prev;
-            } else {
-                link = base + L.run_first[K::run(sk[p - 1]) + 1] - 1;
-            }
-            a.fills[(size_t)(base + first) * 3 + 2] = link;
-        }
-        // Every fill to its slot; a fill inside a run links to the one before.
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; k++) {
-            uint32_t i = k * kBlock + tid;
-            if (i < n) {
-                uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
-                a.sorted_fills[base + slot] = fw[k];
-                if (!(start_bits >> k & 1u)) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
-            }
-        }
-        __syncthreads();
+        link_finish<Key>(a, L, sk, base, n, runs, tile_base, f0, start_bits, rid, fw);
     }
+}
+
+// Paths of at most kLinkWaveFills fills whose tile rect has at most 2^24
+// tiles are linked by k_fill_link_small, a wavefront each; the others by
+// k_fill_link, a workgroup each.
+constexpr uint32_t kLinkWaveFills = 256;
+__device__ __forceinline__ bool link_by_wave(uint32_t n, uint64_t rect) {
+    return n <= kLinkWaveFills && rect <= (1ull << 24);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
@@ -979,10 +1050,181 @@ __global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
     const int32_t *pm = a.prop_md + (size_t)path * 12;
     uint32_t width = (uint32_t)(pm[2] - pm[0]), height = (uint32_t)(pm[3] - pm[1]);
     uint64_t rect = (uint64_t)width * height;
+    if (link_by_wave(f1 - f0, rect)) return;
     if (rect <= kLinkRect32)
         link_path<uint32_t>(a, lds, f0, f1, (uint32_t)pm[4]);
     else
         link_path<unsigned long long>(a, lds, f0, f1, (uint32_t)pm[4]);
+}
+
+// The fill lists of one path of at most 256 fills, a wavefront per path (four
+// per workgroup, no workgroup barriers): the same runs, (tile, run) sort,
+// output slots and links as link_path for a single chunk, with the run keys
+// (local tile << 8 | run) sorted in registers, four per lane (element
+// q * 64 + lane): lane shuffles for partners under 64 apart, register swaps
+// above. k_bound leaves every fill list empty, so no tile has an earlier
+// fill or range.
+struct LinkWaveLds {
+    uint32_t keys[kLinkWaveFills];
+    uint16_t run_first[kLinkWaveFills + 2];
+    uint16_t rank[kLinkWaveFills];
+    uint16_t out[kLinkWaveFills];
+};
+
+__device__ __forceinline__ void link_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
+    constexpr uint32_t kQ = kLinkWaveFills / 64;
+    __shared__ LinkWaveLds lds[kBlock / 64];
+    uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t path = blockIdx.x * (kBlock / 64) + wave;
+    if (path >= a.path_count) return;
+    uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
+    uint32_t n = f1 - f0;
+    const int32_t *pm = a.prop_md + (size_t)path * 12;
+    uint32_t width = (uint32_t)(pm[2] - pm[0]), height = (uint32_t)(pm[3] - pm[1]);
+    if (n == 0 || !link_by_wave(n, (uint64_t)width * height)) return;
+    uint32_t tile_base = (uint32_t)pm[4];
+    LinkWaveLds &L = lds[wave];
+    unsigned long long below = (1ull << lane) - 1ull;
+    // fill i = k * 64 + lane: its tile, the tile of fill i - 1, its words
+    uint32_t tl[kQ], tp[kQ];
+    uint2 fw[kQ];
+#pragma unroll
+    for (uint32_t k = 0; k < kQ; k++) {
+        uint32_t i = min(k * 64u + lane, n - 1);
+        tl[k] = a.fill_tiles[f0 + i];
+        tp[k] = a.fill_tiles[f0 + (i > 0 ? i - 1 : 0)];
+        const uint32_t *w = a.fills + (size_t)(f0 + i) * 3;
+        fw[k] = make_uint2(w[0], w[1]);
+    }
+    uint32_t runs = 0, rid[kQ], start_bits = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kQ; k++) {
+        uint32_t i = k * 64u + lane;
+        bool st = i < n && (i == 0 || tl[k] != tp[k]);
+        unsigned long long m = __ballot(st);
+        rid[k] = runs + (uint32_t)__popcll(m & below) + (st ? 1u : 0u) - 1u;
+        runs += (uint32_t)__popcll(m);
+        start_bits |= (uint32_t)st << k;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kQ; k++) {
+        if (start_bits >> k & 1u) {
+            L.run_first[rid[k]] = (uint16_t)(k * 64u + lane);
+            L.keys[rid[k]] = (tl[k] - tile_base) << 8 | rid[k];
+        }
+    }
+    if (lane == 0) L.run_first[runs] = (uint16_t)n;
+    link_wave_sync();
+    uint32_t size = 64;
+    while (size < runs) size <<= 1;
+    uint32_t nq = size >> 6;
+    uint32_t v[kQ];
+#pragma unroll
+    for (uint32_t q = 0; q < kQ; q++) {
+        uint32_t e = q * 64u + lane;
+        v[q] = (q < nq && e < runs) ? L.keys[e] : 0xffffffffu;
+    }
+    for (uint32_t k = 2; k <= size; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j < 64u) {
+#pragma unroll
+                for (uint32_t q = 0; q < kQ; q++) {
+                    if (q < nq) {
+                        uint32_t o = __shfl_xor(v[q], (int)j, 64);
+                        uint32_t e = q * 64u + lane;
+                        bool keep_min = ((lane & j) == 0) == ((e & k) == 0);
+                        v[q] = keep_min ? min(v[q], o) : max(v[q], o);
+                    }
+                }
+            } else {
+                uint32_t rj = j >> 6;
+#pragma unroll
+                for (uint32_t q = 0; q < kQ; q++) {
+                    if ((q & rj) == 0 && (q | rj) < nq) {
+                        uint32_t e = q * 64u + lane;
+                        uint32_t x = v[q], y = v[q | rj];
+                        bool up = (e & k) == 0;
+                        v[q] = up ? min(x, y) : max(x, y);
+                        v[q | rj] = up ? max(x, y) : min(x, y);
+                    }
+                }
+            }
+        }
+    }
+    link_wave_sync();  // every lane has read the unsorted keys
+#pragma unroll
+    for (uint32_t q = 0; q < kQ; q++)
+        if (q < nq) L.keys[q * 64u + lane] = v[q];
+    link_wave_sync();
+    // output slots: exclusive scan of the run lengths in sorted order
+    uint32_t carry = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kQ; q++) {
+        if (q < nq) {
+            uint32_t p = q * 64u + lane;
+            uint32_t len = 0;
+            if (p < runs) {
+                uint32_t r = v[q] & 0xffu;
+                len = (uint32_t)L.run_first[r + 1] - L.run_first[r];
+                L.rank[r] = (uint16_t)p;
+            }
+            uint32_t x = len;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t y = __shfl_up(x, d, 64);
+                if (lane >= (uint32_t)d) x += y;
+            }
+            if (p < runs) L.out[p] = (uint16_t)(carry + x - len);
+            carry += __shfl(x, 63, 64);
+        }
+    }
+    link_wave_sync();
+    const uint32_t *sk = L.keys;
+    // tile ranges and the links of run starts, one sorted run per lane
+#pragma unroll
+    for (uint32_t q = 0; q < kQ; q++) {
+        uint32_t p = q * 64u + lane;
+        if (q >= nq || p >= runs) continue;
+        uint32_t cur = sk[p];
+        uint32_t lt = cur >> 8, rr = cur & 0xffu;
+        uint32_t first = L.run_first[rr];
+        uint32_t link;
+        if (p == 0 || (sk[p - 1] >> 8) != lt) {
+            uint32_t lo = p, hi = runs;  // last sorted run of this tile
+            while (lo + 1 < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if ((sk[mid] >> 8) == lt) lo = mid;
+                else hi = mid;
+            }
+            uint32_t rq = sk[lo] & 0xffu;
+            uint32_t end = L.run_first[rq + 1];
+            uint32_t tile = tile_base + lt;
+            link = 0xffffffffu;
+            a.tiles[tile].y = f0 + end - 1;
+            uint32_t slot = L.out[p];
+            uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
+            a.tile_runs[tile] = make_uint2(f0 + slot, count);
+        } else {
+            link = f0 + L.run_first[(sk[p - 1] & 0xffu) + 1] - 1;
+        }
+        a.fills[(size_t)(f0 + first) * 3 + 2] = link;
+    }
+    // every fill to its slot; a fill inside a run links to the one before
+#pragma unroll
+    for (uint32_t k = 0; k < kQ; k++) {
+        uint32_t i = k * 64u + lane;
+        if (i < n) {
+            uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
+            a.sorted_fills[f0 + slot] = fw[k];
+            if (!(start_bits >> k & 1u)) a.fills[(size_t)(f0 + i) * 3 + 2] = f0 + i - 1;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ propagate
@@ -1109,11 +1351,33 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
     return count;
 }
 
-// Alpha tile counts per column.
+// Alpha tile counts per column. An unclipped path gives a tile an alpha
+// tile exactly when bin gave it a fill (propagate.cs.glsl:130-131), so its
+// columns count flags; clipped paths take the full column walk.
 __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
-         c += gridDim.x * blockDim.x)
-        a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
+         c += gridDim.x * blockDim.x) {
+        uint32_t dpath = (uint32_t)a.backdrops[(size_t)c * 3 + 2];
+        const int32_t *md = a.prop_md + (size_t)dpath * 12;
+        if (dpath >= a.path_count || md[7] >= 0) {
+            a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
+            continue;
+        }
+        uint32_t width = (uint32_t)(md[2] - md[0]), height = (uint32_t)(md[3] - md[1]);
+        uint32_t t0 = (uint32_t)md[4] + (uint32_t)a.backdrops[(size_t)c * 3 + 1];
+        uint32_t count = 0;
+        for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
+            uint8_t fl[kPropRows];
+#pragma unroll
+            for (uint32_t u = 0; u < kPropRows; u++)
+                fl[u] = a.tile_flags[min(t0 + (y0 + u) * width, a.tile_count - 1u)];
+#pragma unroll
+            for (uint32_t u = 0; u < kPropRows; u++)
+                count += (y0 + u < height && t0 + (y0 + u) * width < a.tile_count && fl[u]) ? 1u
+                                                                                             : 0u;
+        }
+        a.col_alpha_offsets[c] = count;
+    }
 }
 
 // Allocates the batch's alpha tile range in the frame (uFirstAlphaTileIndex,
@@ -2317,6 +2581,7 @@ void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p) {
     Scope t(p, s, K_FILL_LINK);
     hipLaunchKernelGGL(k_path_fills, dim3((a.path_count + kBlock) / kBlock), dim3(kBlock), 0, s,
                        a);
+    hipLaunchKernelGGL(k_fill_link_small, dim3((a.path_count + 3) / 4), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
 }
 
