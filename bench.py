@@ -81,7 +81,8 @@ def copy_calibration(nbytes):
     """Three device-to-device copies of nbytes with the HIP runtime's own copy
     kernel: a copy of known size for scaling memory counters."""
     import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
+    from pathfinder_amd import _lib
+    hip = _lib.hip_runtime()
     src, dst = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0

@@ -186,6 +186,34 @@ SIGNATURES = {
 }
 
 _lib = None
+_hip = None
+
+
+def _share_torch_hip_runtime():
+    """Loads the HIP runtime that torch ships, if torch is installed.
+
+    libpfhip.so needs libamdhip64.so.7. torch ships its own copy with that
+    soname, and its libraries ask for it as `libamdhip64.so` from their own
+    directory. Loaded first under RTLD_GLOBAL, torch's copy satisfies both
+    requests, so the process holds one HIP runtime whichever of torch and
+    this library loads first. Two runtimes in one process do not share
+    devices: the second to initialise finds no GPU, and torch's device
+    pointers and streams would be foreign to this library.
+    """
+    global _hip
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    for d in (spec.submodule_search_locations or []) if spec else []:
+        path = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            _hip = C.CDLL(path, mode=C.RTLD_GLOBAL)
+            return
+
+
+def hip_runtime():
+    """The process's HIP runtime (the one libpfhip.so is bound to)."""
+    load()
+    return _hip if _hip is not None else C.CDLL("libamdhip64.so.7")
 
 
 def load():
@@ -196,6 +224,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("pathfinder_amd: %s is not built; run "
                            "`python -m pathfinder_amd.build`" % LIB_PATH)
+    _share_torch_hip_runtime()
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

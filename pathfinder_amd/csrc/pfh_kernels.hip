@@ -72,15 +72,37 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap = kMax
     return (uint32_t)g;
 }
 
+// x of the lane dpp_ctrl names, or 0 where it names none or the lane's row
+// is not in row_mask.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, false);
+}
+
+// Inclusive scan over the 64 lanes of a wavefront; every lane must be
+// active. Within each row of 16 lanes: x += x of the lane 1, 2, 4, 8 below
+// (row_shr); then lane 15's total is added to rows 1 and 3 (row_bcast:15)
+// and lane 31's total, now that of rows 0-1, to rows 2 and 3 (row_bcast:31).
+// Register moves between lanes, no LDS round trips.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    x += dpp_or_zero<0x111, 0xf>(x);
+    x += dpp_or_zero<0x112, 0xf>(x);
+    x += dpp_or_zero<0x114, 0xf>(x);
+    x += dpp_or_zero<0x118, 0xf>(x);
+    x += dpp_or_zero<0x142, 0xa>(x);
+    x += dpp_or_zero<0x143, 0xc>(x);
+    return x;
+}
+
+// The value of lane 63 (the total of an inclusive scan), in an SGPR.
+__device__ __forceinline__ uint32_t last_lane(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Exclusive scan of one value per thread over a 256-thread block (4 waves).
 __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds4, uint32_t *total) {
     uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
+    uint32_t x = wave_inclusive_scan(v);
     if (lane == 63) lds4[wave] = x;
     __syncthreads();
     uint32_t before = 0, tot = 0;
@@ -96,8 +118,7 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds4, uint32_t *t
 }
 
 __device__ uint32_t block_reduce_sum(uint32_t v, uint32_t *lds4) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    v = last_lane(wave_inclusive_scan(v));
     if (lane_id() == 0) lds4[threadIdx.x >> 6] = v;
     __syncthreads();
     uint32_t t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
@@ -1174,14 +1195,9 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
                 len = (uint32_t)L.run_first[r + 1] - L.run_first[r];
                 L.rank[r] = (uint16_t)p;
             }
-            uint32_t x = len;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                uint32_t y = __shfl_up(x, d, 64);
-                if (lane >= (uint32_t)d) x += y;
-            }
+            uint32_t x = wave_inclusive_scan(len);
             if (p < runs) L.out[p] = (uint16_t)(carry + x - len);
-            carry += __shfl(x, 63, 64);
+            carry += last_lane(x);
         }
     }
     link_wave_sync();
@@ -1681,16 +1697,6 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-    uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    return x;
-}
-
 // The pixel columns c0..c1 of the tile (0..15) that a fill's window sees; an
 // empty range (c0 > c1) when it sees none.
 struct FillColumns {
@@ -1891,7 +1897,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
         uint32_t n = (valid && !chained) ? min(rn, 1024u) : 0u;
         uint32_t incl = wave_inclusive_scan(n);
         uint32_t S = incl - n;
-        uint32_t total = uniform(__shfl(incl, 63, 64));
+        uint32_t total = last_lane(incl);
 
         int32_t cur = -1;
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1909,7 +1915,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             uint32_t span = have ? (fc.c0 <= fc.c1 ? fc.c1 - fc.c0 + 1u : 1u) : 0u;
             uint32_t pe = wave_inclusive_scan(span);
             uint32_t pb = pe - span;
-            uint32_t pairs = uniform(__shfl(pe, 63, 64));
+            uint32_t pairs = last_lane(pe);
             if (have) {
                 rec[lane] = f;
                 cadj[lane] = (int32_t)fc.c0 - (int32_t)pb;
@@ -1933,7 +1939,8 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             for (uint32_t r0 = 0; r0 < pairs;) {
                 unsigned long long cuts = __ballot(cut && pe > r0 && pe <= r0 + kFillPairCap);
                 uint32_t r1 = cuts != 0ull
-                                  ? uniform(__shfl(pe, 63u - (uint32_t)__clzll((long long)cuts), 64))
+                                  ? (uint32_t)__builtin_amdgcn_readlane(
+                                        (int)pe, 63 - __clzll((long long)cuts))
                                   : min(r0 + kFillPairCap, pairs);
                 // computeCoverage of every (pair, row group) of the round; lane l
                 // takes row group l & 3 of pairs r0 + (l >> 2) + 16 i

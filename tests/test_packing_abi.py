@@ -112,6 +112,25 @@ def test_struct_layouts_match_reference_sizes():
         assert list(sizes) == [16, 12, 16, 8, 16, 48, 16, 12, 8]
 
 
+def test_library_and_torch_share_one_hip_runtime():
+    # In a fresh process, load the library before torch: both must bind the
+    # same libamdhip64, or the second runtime to initialise finds no GPU.
+    import subprocess
+    import sys
+    code = (
+        "import re, sys\n"
+        "sys.path.insert(0, %r)\n"
+        "from pathfinder_amd import _lib\n"
+        "_lib.load()\n"
+        "import torch\n"
+        "maps = open('/proc/self/maps').read()\n"
+        "print(sorted(set(re.findall(r'\\S*/libamdhip64\\S*', maps))))\n" % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], check=True, capture_output=True,
+                         text=True).stdout.strip()
+    paths = eval(out)
+    assert len(paths) == 1, paths
+
+
 def test_no_device_is_reported_without_gpu():
     import torch
     if torch.cuda.is_available():
