@@ -1836,8 +1836,49 @@ __device__ __forceinline__ void fill_lookup(const BatchArgs &a, uint32_t pos, ui
     if (q < total) fw = a.sorted_fills[trx + tn - 1u - (q - tS)];
 }
 
-constexpr uint32_t kFillGroup = 64;     // alpha tiles per wavefront group
-constexpr uint32_t kFillPairCap = 48;   // (fill, pixel column) pairs per round
+constexpr uint32_t kFillGroup = 64;  // alpha tiles per wavefront group
+constexpr uint32_t kZeroSlot = 256;  // res entry that holds float4(0)
+
+// computeCoverage (fill_area.inc.glsl:11-27) of one fill for the pixel
+// column centred at x and all four row groups of the tile, written to
+// out[64 g] for row group g (rows 4g..4g+3). Every value is bit-identical to
+// coverage_record(lut, f, x, 4g + 0.5): the x window, the quotient, the LUT
+// row (u) and its weight depend on the column only and are computed once;
+// per row group only yy, the LUT column (s) and the bilinear blend remain.
+__device__ __forceinline__ void coverage_pair(LutRef lut, const FillRecord &f, float x,
+                                              float4 *out) {
+    float wx = __builtin_amdgcn_fmed3f(f.fx - x, -0.5f, 0.5f);
+    float wy = __builtin_amdgcn_fmed3f(f.tx - x, -0.5f, 0.5f);
+    float offset = __builtin_fmaf(0.5f, wy - wx, wx) - (f.lx - x);
+    float q0 = offset * f.rw;
+    float t = __builtin_fmaf(__builtin_fmaf(-q0, f.w, offset), f.rw, q0);
+    float rise = t * f.dy;
+    float dX = wx - wy;
+    float u = __builtin_fmaf(fabsf(f.d * dX), 16.0f, -0.5f);
+    float fu = floorf(u);
+    float wb = u - fu, ib = 1.0f - wb;
+    uint32_t jp = (uint32_t)min(max((int32_t)fu + 1, 0), 257);
+    uint32_t row0 = jp * kLutStride * 16u;
+#pragma unroll
+    for (uint32_t g = 0; g < 4; g++) {
+        float yy = (f.ly - ((float)(g * 4u) + 0.5f)) + rise;
+        float sc = __builtin_fmaf(yy + 8.0f, 16.0f, -0.5f);
+        float fs = floorf(sc);
+        float wa = sc - fs, ia = 1.0f - wa;
+        uint32_t ip = (uint32_t)min(max((int32_t)fs + 1, 0), 257);
+        uint32_t o = row0 + ip * 16u;
+        float4 t00 = lut_texel(lut, o, 0u);
+        float4 t10 = lut_texel(lut, o + 16u, 0u);
+        float4 t01 = lut_texel(lut, o, kLutStride * 16u);
+        float4 t11 = lut_texel(lut, o + 16u, kLutStride * 16u);
+        float4 r;
+        r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
+        r.y = (t00.y * ia + t10.y * wa) * ib + (t01.y * ia + t11.y * wa) * wb;
+        r.z = (t00.z * ia + t10.z * wa) * ib + (t01.z * ia + t11.z * wa) * wb;
+        r.w = (t00.w * ia + t10.w * wa) * ib + (t01.w * ia + t11.w * wa) * wb;
+        out[64u * g] = make_float4(r.x * dX, r.y * dX, r.z * dX, r.w * dX);
+    }
+}
 
 // fill.cs.glsl:61-88 with fill_compute.inc.glsl:11-25. Lane (lx, ly) of a
 // wavefront owns the four pixels of column lx, rows 4ly..4ly+3 of an alpha
@@ -1845,20 +1886,20 @@ constexpr uint32_t kFillPairCap = 48;   // (fill, pixel column) pairs per round
 //
 // A wavefront takes 64 consecutive alpha tiles and streams their fills: each
 // tile's fills from the highest index down (its run in sorted_fills), tile
-// after tile, 64 at a time. Lane k turns fill k of a chunk into a record and
+// after tile, 64 at a time. Lane k turns fill k of a chunk into a record, and
 // the chunk's (fill, pixel column) pairs are numbered in stream order. Round
-// by round (kFillPairCap pairs), the wavefront evaluates computeCoverage for
-// every (pair, row group) into LDS, 64 at a time; then each lane adds, in
-// stream order, the values of the round's fills that touch its column (a
-// fill adds exactly +-0 to a column it does not touch, and the sum ends in
-// abs()), storing a tile's mask when the stream moves on to the next tile.
-// A pixel meets each fill in one pair only, so its sum runs over the same
-// fills in the same order as the list walk.
+// by round (64 pairs), lane i evaluates computeCoverage of pair i for the
+// four row groups into LDS; then the wavefront walks the round's fills in
+// stream order and every lane adds the value of its column, or exactly +0
+// where the fill's window does not see the column (dX = 0 there, and the
+// sum ends in abs(), so +-0 changes no bit), storing a tile's mask when the
+// stream moves on to the next tile. A pixel thus sums the same fills in the
+// same order as the reference's list walk.
 // Tiles whose fills are split over several runs (rare) take the list walk
 // one chunk at a time instead (accumulate_chunk).
 __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     __shared__ FillRecord s_rec[kBlock / 64][64];
-    __shared__ float4 s_res[kBlock / 64][4 * kFillPairCap];
+    __shared__ float4 s_res[kBlock / 64][4 * 64 + 1];
     __shared__ int32_t s_cadj[kBlock / 64][64];
     __shared__ uint8_t s_pair[kBlock / 64][64 * 16];
     uint32_t wv = threadIdx.x >> 6, lane = lane_id();
@@ -1871,7 +1912,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     uint32_t start = uniform(a.counters->alpha_start);
     uint32_t col = lane & 15u, grp = lane >> 4;
     float fragx = (float)col + 0.5f, fragy = (float)(grp * 4u) + 0.5f;
-    float item_fragy = (float)((lane & 3u) * 4u) + 0.5f;  // row group of this lane's items
+    if (lane == 0) res[kZeroSlot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t stride = ((gridDim.x * blockDim.x) >> 6) * kFillGroup;
     for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kFillGroup);
          g0 < n_alpha; g0 += stride) {
@@ -1907,58 +1948,47 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
         fill_lookup(a, 0u, total, S, rx, n, lane, ts, fw, tlast);
         for (uint32_t pos = 0; pos < total; pos += 64u) {
             bool have = pos + lane < total;
-            uint32_t nf = min(total - pos, 64u);
             FillRecord f;
             FillColumns fc = make_record(fw, f);
             // every fill takes at least one pair (a fill that sees no column
-            // gets a pair no lane reads), so each fill lies in some round
-            uint32_t span = have ? (fc.c0 <= fc.c1 ? fc.c1 - fc.c0 + 1u : 1u) : 0u;
+            // gets a pair whose value no lane reads), so each fill lies in
+            // some round
+            uint32_t nvis = (have && fc.c0 <= fc.c1) ? fc.c1 - fc.c0 + 1u : 0u;
+            uint32_t span = have ? max(nvis, 1u) : 0u;
             uint32_t pe = wave_inclusive_scan(span);
             uint32_t pb = pe - span;
             uint32_t pairs = last_lane(pe);
+            // what the accumulation reads of fill k: first column, visible
+            // columns, first pair, tile slot
+            uint32_t info = (fc.c0 & 15u) | (nvis << 4) | (pb << 9) | (ts << 20);
             if (have) {
                 rec[lane] = f;
                 cadj[lane] = (int32_t)fc.c0 - (int32_t)pb;
                 for (uint32_t j = 0; j < span; j++) pair_fill[pb + j] = (uint8_t)lane;
             }
-            // the fills (bit k = fill k) whose window sees this lane's column
-            unsigned long long mine = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < 16; q++) {
-                unsigned long long m = __ballot(have && fc.c0 <= q && q <= fc.c1);
-                mine = q == col ? m : mine;
-            }
-            // a round may end after a fill that ends its tile or the chunk
-            bool cut = have && (tlast || lane == nf - 1u);
             // the next chunk's fills load meanwhile
             uint32_t nts;
             uint2 nfw;
             bool ntlast;
             fill_lookup(a, pos + 64u, total, S, rx, n, lane, nts, nfw, ntlast);
+            (void)tlast;
             wave_lds_sync();
-            for (uint32_t r0 = 0; r0 < pairs;) {
-                unsigned long long cuts = __ballot(cut && pe > r0 && pe <= r0 + kFillPairCap);
-                uint32_t r1 = cuts != 0ull
-                                  ? (uint32_t)__builtin_amdgcn_readlane(
-                                        (int)pe, 63 - __clzll((long long)cuts))
-                                  : min(r0 + kFillPairCap, pairs);
-                // computeCoverage of every (pair, row group) of the round; lane l
-                // takes row group l & 3 of pairs r0 + (l >> 2) + 16 i
-                for (uint32_t it = lane; it < 4u * (r1 - r0); it += 64u) {
-                    uint32_t p = r0 + (it >> 2);
+            for (uint32_t r0 = 0; r0 < pairs; r0 += 64u) {
+                uint32_t r1 = min(r0 + 64u, pairs);
+                uint32_t p = r0 + lane;
+                if (p < r1) {
                     uint32_t k = pair_fill[p];
                     float x = (float)((int32_t)p + cadj[k]) + 0.5f;
-                    res[it] = coverage_record(lut, rec[k], x, item_fragy);
+                    coverage_pair(lut, rec[k], x, res + lane);
                 }
                 wave_lds_sync();
-                // the fills with a pair in [r0, r1), tile by tile in stream order;
-                // each lane adds the values of the fills that touch its column
+                // the fills with a pair in [r0, r1), in stream order
                 uint32_t ka = (uint32_t)__popcll(__ballot(have && pe <= r0));
                 uint32_t kb = (uint32_t)__popcll(__ballot(have && pb < r1));
-                for (uint32_t k = ka; k < kb;) {
-                    int32_t t = (int32_t)__builtin_amdgcn_readlane(ts, k);
-                    unsigned long long seg = __ballot(have && (int32_t)ts == t);
-                    uint32_t ke = min(64u - (uint32_t)__clzll((long long)seg), kb);
+                uint32_t rw = r1 - r0;
+                for (uint32_t k = ka; k < kb; k++) {
+                    uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)info, (int)k);
+                    int32_t t = (int32_t)(fi >> 20);
                     if (t != cur) {
                         if (cur >= 0)
                             store_mask(a, start + g0 + (uint32_t)cur,
@@ -1968,22 +1998,14 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                         cur = t;
                         acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     }
-                    unsigned long long below = ke == 64u ? ~0ull : (1ull << ke) - 1ull;
-                    unsigned long long m = mine & below;
-                    while (m != 0ull) {
-                        uint32_t j = (uint32_t)__ffsll((long long)m) - 1u;
-                        uint32_t pp = col - (uint32_t)cadj[j];
-                        if (pp >= r1) break;
-                        m &= m - 1ull;
-                        float4 v = res[(pp - r0) * 4u + grp];
-                        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-                    }
-                    mine = (mine & ~below) | m;
-                    k = ke;
+                    uint32_t d = col - (fi & 15u);
+                    uint32_t pp = ((fi >> 9) & 0x7ffu) + d - r0;
+                    bool use = d < ((fi >> 4) & 31u) && pp < rw;
+                    float4 v = res[use ? pp + 64u * grp : kZeroSlot];
+                    acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
                 }
                 // every lane is done with this round's values
                 wave_lds_sync();
-                r0 = r1;
             }
             ts = nts;
             fw = nfw;
