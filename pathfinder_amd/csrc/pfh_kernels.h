@@ -66,7 +66,9 @@ struct BatchArgs {
     uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)
     uint2 *run_prev;               // at a run's first sorted index: the previous run or (0, 0)
     uint4 *tiles;                  // tile_count
-    uint8_t *tile_flags;           // tile_count: 1 when bin gave the tile a fill
+    uint8_t *tile_flags;           // tile_count: nonzero when bin gave the tile a fill (2 once
+                                   // k_fill_link has written its head)
+    uint32_t *tile_delta;          // tile_count: bin's backdrop deltas << 24 (tile word 2's top byte)
     uint32_t *col_alpha_offsets;   // column_count + 1
     uint2 *alpha_tiles;            // tile_count
     int32_t *zbuffer;              // 8 + fb tiles

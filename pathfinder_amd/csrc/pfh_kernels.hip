@@ -491,19 +491,39 @@ __device__ __forceinline__ uint32_t reference_tile_search(const BatchArgs &a, ui
     return lo;
 }
 
-// bound.cs.glsl:49-84. A workgroup initialises 1024 consecutive tiles: it
-// stages the first tile indices of the paths they belong to in LDS and looks
-// each tile up there. The reference search returns the same path, the last
-// one starting at or before the tile, except when it lands on an empty path
-// that starts at the tile itself; that case takes the reference search.
+// bound.cs.glsl:49-84. The reference initialises every TileD3D11 here
+// (next and first fill ~0, alpha -1 with a zero backdrop delta, the path's
+// colour and control word). Here the only state bin and propagate need
+// before propagate writes the tiles is cleared: the per-tile fill flag and
+// backdrop delta word (bin adds delta << 24 to it, as the shader does to
+// tile word 2). Propagate then writes all four words of every tile with the
+// values bound and bin would have left plus its own.
+// With the adjoint on, bound also records each tile's path: a workgroup
+// takes 1024 consecutive tiles, stages the first tile indices of the paths
+// they belong to in LDS and looks each tile up there. The reference search
+// returns the same path, the last one starting at or before the tile,
+// except when it lands on an empty path that starts at the tile itself; that
+// case takes the reference search.
 constexpr uint32_t kBoundItems = 4;
 
 __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
     constexpr uint32_t kPer = kBlock * kBoundItems;
     __shared__ uint32_t zs[kPer + 1];
-    __shared__ uint32_t ws[kPer + 1];
     __shared__ uint32_t qrange[2];
-    if (a.path_count == 0) return;
+    {
+        uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, n = gridDim.x * blockDim.x;
+        uint4 *d4 = reinterpret_cast<uint4 *>(a.tile_delta);
+        for (uint32_t i = gid; i < a.tile_count / 4u; i += n) d4[i] = make_uint4(0, 0, 0, 0);
+        uint4 *f4 = reinterpret_cast<uint4 *>(a.tile_flags);
+        for (uint32_t i = gid; i < a.tile_count / 16u; i += n) f4[i] = make_uint4(0, 0, 0, 0);
+        if (gid < 16u) {
+            uint32_t t = (a.tile_count & ~15u) + gid;
+            if (t < a.tile_count) a.tile_flags[t] = 0;
+            if (gid < 4u && (a.tile_count & ~3u) + gid < a.tile_count)
+                a.tile_delta[(a.tile_count & ~3u) + gid] = 0;
+        }
+    }
+    if (a.path_count == 0 || !a.tile_paths) return;
     uint32_t wave = threadIdx.x >> 6;
     auto first_tile = [&](uint32_t i) { return a.tile_path_info[i].z; };
     for (uint32_t b0 = blockIdx.x * kPer; b0 < a.tile_count; b0 += gridDim.x * kPer) {
@@ -518,15 +538,13 @@ __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
         bool staged = q1 - q0 < kPer;
         if (staged)
             for (uint32_t i = threadIdx.x; i <= q1 - q0; i += blockDim.x) {
-                uint4 tpi = a.tile_path_info[q0 + i];
-                zs[i] = tpi.z;
-                ws[i] = tpi.w;
+                zs[i] = a.tile_path_info[q0 + i].z;
             }
         __syncthreads();
         for (uint32_t k = 0; k < kBoundItems; k++) {
             uint32_t t = b0 + k * kBlock + threadIdx.x;
             if (t > b1) break;
-            uint32_t p, w;
+            uint32_t p;
             if (staged) {
                 uint32_t lo = 0, hi = q1 - q0 + 1;
                 while (lo + 1 < hi) {
@@ -535,19 +553,13 @@ __global__ void __launch_bounds__(kBlock) k_bound(BatchArgs a) {
                     else hi = mid;
                 }
                 p = q0 + lo;
-                w = ws[lo];
                 if (p > 0 && zs[lo] == t &&
-                    (lo > 0 ? zs[lo - 1] : a.tile_path_info[p - 1].z) == t) {
+                    (lo > 0 ? zs[lo - 1] : a.tile_path_info[p - 1].z) == t)
                     p = reference_tile_search(a, t);
-                    w = a.tile_path_info[p].w;
-                }
             } else {
                 p = reference_tile_search(a, t);
-                w = a.tile_path_info[p].w;
             }
-            a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, 0x00ffffffu, w);
-            a.tile_flags[t] = 0;
-            if (a.tile_paths) a.tile_paths[t] = p;
+            a.tile_paths[t] = p;
         }
         __syncthreads();
     }
@@ -660,7 +672,7 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
                 } else {
                     uint32_t t = off + (uint32_t)(tx - r.x0) +
                                  (uint32_t)(ty - r.y0) * (uint32_t)(r.x1 - r.x0);
-                    if (t < a.tile_count) atomicAdd(&a.tiles[t].z, (uint32_t)delta << 24);
+                    if (t < a.tile_count) atomicAdd(&a.tile_delta[t], (uint32_t)delta << 24);
                 }
             }
         }
@@ -835,11 +847,14 @@ __device__ __forceinline__ void link_finish(const BatchArgs &a, LinkLds &L, cons
             uint32_t tile = tile_base + lt;
             link = 0xffffffffu;
             uint2 prev = make_uint2(0, 0);
-            if (!first_chunk) {
+            // flag 2: an earlier chunk of this path gave the tile a head
+            // (bin set the flag to 1; tiles are not initialised otherwise)
+            if (!first_chunk && a.tile_flags[tile] == 2) {
                 link = a.tiles[tile].y;
                 prev = a.tile_runs[tile];
             }
             a.tiles[tile].y = base + end - 1;
+            a.tile_flags[tile] = 2;
             uint32_t slot = L.out[p];
             uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
             // bit 31 of the count: an earlier range of this tile exists, and
@@ -1277,11 +1292,18 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
         cofs = (uint32_t)cm[4];
     }
     bool cload = clip && a.clip_tile_count != 0;
+    // tile word 3 as bound leaves it: the colour and control bits of the
+    // path bound's search finds, the column's own path, except for a path's
+    // first tile when empty paths start at the same index: the search may
+    // stop on one of them (bound.cs.glsl:53-66)
+    uint32_t word = a.tile_path_info[dpath].w & 0x00ffffffu, word0 = word;
+    if (tile_x == 0 && dpath > 0 && a.tile_path_info[dpath - 1].z == dofs)
+        word0 = a.tile_path_info[reference_tile_search(a, dofs)].w & 0x00ffffffu;
     uint32_t count = 0;
     // rows are fetched kPropRows at a time (the loads do not depend on the
     // running backdrop), then processed in order
     for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
-        uint4 tl[kPropRows];
+        uint32_t dl[kPropRows];
         uint2 cl[kPropRows];
         uint8_t fl[kPropRows];
         // unconditional loads at clamped indices (a load under a branch is
@@ -1291,7 +1313,7 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
         for (uint32_t u = 0; u < kPropRows; u++) {
             uint32_t y = y0 + u;
             uint32_t t = dofs + y * width + (uint32_t)tile_x;
-            tl[u] = a.tiles[min(t, a.tile_count - 1u)];
+            dl[u] = a.tile_delta[min(t, a.tile_count - 1u)];
             fl[u] = a.tile_flags[min(t, a.tile_count - 1u)];
             if (cload) {
                 uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;
@@ -1310,10 +1332,8 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
                 stop = true;
                 break;
             }
-            uint4 tile = tl[u];
             int32_t draw_alpha = -1, clip_alpha = -1;
-            int32_t delta = (int32_t)tile.z >> 24;
-            uint32_t word = tile.w & 0x00ffffffu;
+            int32_t delta = (int32_t)dl[u] >> 24;
             int32_t tile_backdrop = backdrop;
             // first_fill >= 0 in the shader: bin gave the tile a fill
             bool have_mask = fl[u] != 0;
@@ -1356,9 +1376,17 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
                 count++;
             }
             if (EMIT) {
-                uint2 zw = make_uint2(((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24),
-                                      word | ((uint32_t)tile_backdrop << 24));
-                *reinterpret_cast<uint2 *>(&a.tiles[t].z) = zw;
+                // next ~0 (k_sort_tiles links it later), first fill ~0 unless
+                // the tile has fills (k_fill_link, running beside this
+                // kernel, writes that word), alpha tile and delta, word 3
+                uint32_t z = ((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24);
+                uint32_t w = (y == 0 ? word0 : word) | ((uint32_t)tile_backdrop << 24);
+                if (have_mask) {
+                    a.tiles[t].x = 0xffffffffu;
+                    *reinterpret_cast<uint2 *>(&a.tiles[t].z) = make_uint2(z, w);
+                } else {
+                    a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, z, w);
+                }
             }
             backdrop += delta;
         }
