@@ -598,7 +598,6 @@ pfh_status enqueue_frame(pfh_context *ctx) {
         Batch &b = *ctx->batches[i];
         pfh::BatchArgs a = batch_args(ctx, b);
         hipStream_t side = ctx->side[i & 1];
-        pfh::launch_row_paths(a, side, p);
         HIP_TRY(ctx, hipStreamWaitEvent(side, b.prop_done, 0));
         if (i + 1 < n) {
             for (size_t j = 0; j < i; j++)  // earlier batches' masks (its clip masks)
@@ -606,6 +605,10 @@ pfh_status enqueue_frame(pfh_context *ctx) {
             pfh::launch_fill(a, side, p);
             HIP_TRY(ctx, hipEventRecord(b.fill_done, side));
         }
+        // the per-row path lists and the sort run beside the last batch's
+        // fill, not beside the propagate and fill lists it waits for
+        HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[n - 1]->prop_done, 0));
+        pfh::launch_row_paths(a, side, p);
         pfh::launch_sort(a, side, p);
         HIP_TRY(ctx, hipEventRecord(b.sort_done, side));
     }
