@@ -274,10 +274,15 @@ pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats);
 
 /* Per-kernel device timing: HIP events recorded around kernel launches on the
  * stream each kernel runs on (the reference times draw calls with GL timer queries per
- * TimeCategory, gpu/perf.rs:78-219). PFH_PROFILE_ONE times only `kernel`. */
+ * TimeCategory, gpu/perf.rs:78-219). PFH_PROFILE_ONE times only `kernel`.
+ * PFH_PROFILE_SERIAL times every kernel alone: frames re-rendered by
+ * pfh_rerender_scene run every stage of every batch in order on the context
+ * stream instead of overlapping batches and stages on the side streams (the
+ * results are the same). */
 #define PFH_PROFILE_OFF 0
 #define PFH_PROFILE_ALL 1
 #define PFH_PROFILE_ONE 2
+#define PFH_PROFILE_SERIAL 3
 typedef struct pfh_kernel_time {
     const char *name;
     uint64_t launches;

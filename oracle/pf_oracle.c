@@ -18,8 +18,10 @@
  *                           round_even)
  *   length(v)             = sqrt(v.x * v.x + v.y * v.y), sqrt correctly rounded
  *   uint(x) for x < 0     = 0 (saturating, like v_cvt_u32_f32)
- *   texture() on the LUT  = bilinear, CLAMP_TO_EDGE, fp32 weights:
- *                           (T00(1-a) + T10 a)(1-b) + (T01(1-a) + T11 a) b
+ *   texture() on the LUT  = bilinear, CLAMP_TO_EDGE, fp32 weights a, b, each
+ *                           lerp fused as llvmpipe's lp_build_lerp emits it:
+ *                           lerp(w, x, y) = fma(w, y - x, x);
+ *                           lerp(b, lerp(a, T00, T10), lerp(a, T01, T11))
  *   unorm8 load           = c / 255
  *   unorm8 store          = floor(clamp(x, 0, 1) * 255 + 0.5)
  * These choices are the contract the HIP kernels are held to bit for bit.
@@ -631,7 +633,9 @@ static void stage_sort(orc_ctx *ctx, orc_batch *b) {
 
 /* ---------------------------------------------------------------- fill */
 
-/* texture(uAreaLUT, uv) with GL_LINEAR + CLAMP_TO_EDGE, fp32 weights. */
+/* texture(uAreaLUT, uv) with GL_LINEAR + CLAMP_TO_EDGE, fp32 weights; each
+ * linear interpolation is llvmpipe's v0 + w * (v1 - v0) as one fused
+ * multiply-add (lp_build_lerp), along s first, then along t. */
 static inline void sample_lut(const orc_ctx *ctx, float u, float v, float out[4]) {
     float s = u * 256.0f - 0.5f, t = v * 256.0f - 0.5f;
     float fs = floorf(s), ft = floorf(t);
@@ -646,11 +650,10 @@ static inline void sample_lut(const orc_ctx *ctx, float u, float v, float out[4]
     const float *t10 = ctx->lut + ((size_t)j0 * 256 + i1) * 4;
     const float *t01 = ctx->lut + ((size_t)j1 * 256 + i0) * 4;
     const float *t11 = ctx->lut + ((size_t)j1 * 256 + i1) * 4;
-    float ia = 1.0f - a, ib = 1.0f - bb;
     for (int c = 0; c < 4; c++) {
-        float r0 = t00[c] * ia + t10[c] * a;
-        float r1 = t01[c] * ia + t11[c] * a;
-        out[c] = r0 * ib + r1 * bb;
+        float r0 = fmaf(a, t10[c] - t00[c], t00[c]);
+        float r1 = fmaf(a, t11[c] - t01[c], t01[c]);
+        out[c] = fmaf(bb, r1 - r0, r0);
     }
 }
 

@@ -115,6 +115,7 @@ class KernelTime(C.Structure):
 PROFILE_OFF = 0
 PROFILE_ALL = 1
 PROFILE_ONE = 2
+PROFILE_SERIAL = 3
 
 
 class BuildOptions(C.Structure):

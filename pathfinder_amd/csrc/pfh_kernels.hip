@@ -1645,25 +1645,53 @@ struct FillRecord {
 };
 
 // texture(uAreaLUT, .) at texel coordinates (s, u) = (256 x - 0.5, 256 y - 0.5)
-// with GL_LINEAR, CLAMP_TO_EDGE and fp32 weights. lut is the 256 x 256 float4
-// texels padded with a copy of the edge on every side: padded texel (j, i),
-// 0 <= i, j <= 258, holds texel (clamp(j - 1), clamp(i - 1)) with clamp to
-// [0, 255], rows kLutStride texels apart. With k = clamp(index + 1, 0, 257)
-// the texels k and k + 1 of the padded table are the clamped texels index and
-// index + 1 for every index, so one clamp per axis replaces four.
-// The table is read through a raw buffer descriptor so that the four texels
-// come from one offset: +16 bytes in the instruction, +1 row in an SGPR.
+// with GL_LINEAR, CLAMP_TO_EDGE and fp32 weights wa = s - floor(s),
+// wb = u - floor(u). Each linear interpolation is the one Mesa's llvmpipe
+// emits for float textures, v0 + w * (v1 - v0) as a fused multiply-add
+// (lp_build_lerp): A = lerp(wa, t00, t10), B = lerp(wa, t01, t11), then
+// lerp(wb, A, B).
+// lut holds, per padded texel, the texel and its difference to the next
+// texel along s: 8 floats. The 256 x 256 texels are padded with a copy of
+// the edge on every side: padded texel (j, i), 0 <= i, j <= 258, is texel
+// (clamp(j - 1), clamp(i - 1)) with clamp to [0, 255], rows kLutStride
+// texels apart. With k = clamp(index + 1, 0, 257) the texels k and k + 1 of
+// the padded table are the clamped texels index and index + 1 for every
+// index, so one clamp per axis replaces four.
+// The table is read through a raw buffer descriptor so that a row's texel
+// and difference come from one offset: +16 bytes in the instruction, +1 row
+// in an SGPR.
+constexpr uint32_t kLutTexelBytes = 32;
 typedef __amdgpu_buffer_rsrc_t LutRef;
 __device__ __forceinline__ LutRef lut_ref(const float4 *lut) {
     // word 3 of a raw (untyped) gfx9 buffer descriptor; the record count is
     // the table size in bytes, so an offset past it would read zeros
     return __builtin_amdgcn_make_buffer_rsrc((void *)lut, (short)0,
-                                             (int)(kLutRows * kLutStride * 16u), 0x00020000);
+                                             (int)(kLutRows * kLutStride * kLutTexelBytes),
+                                             0x00020000);
 }
 __device__ __forceinline__ float4 lut_texel(LutRef lut, uint32_t offset, uint32_t row) {
     auto v = __builtin_amdgcn_raw_buffer_load_b128(lut, offset, row, 0);
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                        __uint_as_float(v[3]));
+}
+// The bilinear blend of padded texels (jp, ip) .. (jp + 1, ip + 1) at byte
+// offset o = (jp * kLutStride + ip) * kLutTexelBytes.
+__device__ __forceinline__ float4 lut_blend(LutRef lut, uint32_t o, float wa, float wb) {
+    constexpr uint32_t kRow = kLutStride * kLutTexelBytes;
+    float4 t00 = lut_texel(lut, o, 0u);
+    float4 d0 = lut_texel(lut, o + 16u, 0u);
+    float4 t01 = lut_texel(lut, o, kRow);
+    float4 d1 = lut_texel(lut, o + 16u, kRow);
+    float4 A, B, r;
+    A.x = __builtin_fmaf(wa, d0.x, t00.x); B.x = __builtin_fmaf(wa, d1.x, t01.x);
+    A.y = __builtin_fmaf(wa, d0.y, t00.y); B.y = __builtin_fmaf(wa, d1.y, t01.y);
+    A.z = __builtin_fmaf(wa, d0.z, t00.z); B.z = __builtin_fmaf(wa, d1.z, t01.z);
+    A.w = __builtin_fmaf(wa, d0.w, t00.w); B.w = __builtin_fmaf(wa, d1.w, t01.w);
+    r.x = __builtin_fmaf(wb, B.x - A.x, A.x);
+    r.y = __builtin_fmaf(wb, B.y - A.y, A.y);
+    r.z = __builtin_fmaf(wb, B.z - A.z, A.z);
+    r.w = __builtin_fmaf(wb, B.w - A.w, A.w);
+    return r;
 }
 __device__ __forceinline__ float4 sample_lut_texel(LutRef lut, float s, float u) {
     float fs = floorf(s), fu = floorf(u);
@@ -1672,18 +1700,7 @@ __device__ __forceinline__ float4 sample_lut_texel(LutRef lut, float s, float u)
     int32_t is = (int32_t)fs, iu = (int32_t)fu;
     uint32_t ip = (uint32_t)min(max(is + 1, 0), 257);
     uint32_t jp = (uint32_t)min(max(iu + 1, 0), 257);
-    uint32_t o = (jp * kLutStride + ip) * 16u;
-    float4 t00 = lut_texel(lut, o, 0u);
-    float4 t10 = lut_texel(lut, o + 16u, 0u);
-    float4 t01 = lut_texel(lut, o, kLutStride * 16u);
-    float4 t11 = lut_texel(lut, o + 16u, kLutStride * 16u);
-    float ia = 1.0f - wa, ib = 1.0f - wb;
-    float4 r;
-    r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
-    r.y = (t00.y * ia + t10.y * wa) * ib + (t01.y * ia + t11.y * wa) * wb;
-    r.z = (t00.z * ia + t10.z * wa) * ib + (t01.z * ia + t11.z * wa) * wb;
-    r.w = (t00.w * ia + t10.w * wa) * ib + (t01.w * ia + t11.w * wa) * wb;
-    return r;
+    return lut_blend(lut, (jp * kLutStride + ip) * kLutTexelBytes, wa, wb);
 }
 
 // computeCoverage (fill_area.inc.glsl:11-27) of one fill for the pixel column
@@ -1884,26 +1901,17 @@ __device__ __forceinline__ void coverage_pair(LutRef lut, const FillRecord &f, f
     float dX = wx - wy;
     float u = __builtin_fmaf(fabsf(f.d * dX), 16.0f, -0.5f);
     float fu = floorf(u);
-    float wb = u - fu, ib = 1.0f - wb;
+    float wb = u - fu;
     uint32_t jp = (uint32_t)min(max((int32_t)fu + 1, 0), 257);
-    uint32_t row0 = jp * kLutStride * 16u;
+    uint32_t row0 = jp * kLutStride * kLutTexelBytes;
 #pragma unroll
     for (uint32_t g = 0; g < 4; g++) {
         float yy = (f.ly - ((float)(g * 4u) + 0.5f)) + rise;
         float sc = __builtin_fmaf(yy + 8.0f, 16.0f, -0.5f);
         float fs = floorf(sc);
-        float wa = sc - fs, ia = 1.0f - wa;
+        float wa = sc - fs;
         uint32_t ip = (uint32_t)min(max((int32_t)fs + 1, 0), 257);
-        uint32_t o = row0 + ip * 16u;
-        float4 t00 = lut_texel(lut, o, 0u);
-        float4 t10 = lut_texel(lut, o + 16u, 0u);
-        float4 t01 = lut_texel(lut, o, kLutStride * 16u);
-        float4 t11 = lut_texel(lut, o + 16u, kLutStride * 16u);
-        float4 r;
-        r.x = (t00.x * ia + t10.x * wa) * ib + (t01.x * ia + t11.x * wa) * wb;
-        r.y = (t00.y * ia + t10.y * wa) * ib + (t01.y * ia + t11.y * wa) * wb;
-        r.z = (t00.z * ia + t10.z * wa) * ib + (t01.z * ia + t11.z * wa) * wb;
-        r.w = (t00.w * ia + t10.w * wa) * ib + (t01.w * ia + t11.w * wa) * wb;
+        float4 r = lut_blend(lut, row0 + ip * kLutTexelBytes, wa, wb);
         out[64u * g] = make_float4(r.x * dX, r.y * dX, r.z * dX, r.w * dX);
     }
 }
@@ -2522,17 +2530,23 @@ __global__ void k_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words,
 
 // ------------------------------------------------------------------ tables
 
-// The padded LUT of sample_lut_texel: texel (j, i) of the 259 x 259 table (row
-// stride kLutStride) is byte texel (clamp(j - 1), clamp(i - 1)) / 255.
+// The padded LUT of lut_blend: padded texel (j, i) of the 259 x 259 table (row
+// stride kLutStride) is byte texel (clamp(j - 1), clamp(i - 1)) / 255, and
+// beside it the difference to padded texel (j, i + 1), both in float.
+__device__ __forceinline__ float4 padded_texel(const uint8_t *rgba, uint32_t jp, uint32_t ip) {
+    uint32_t j = (uint32_t)min(max((int32_t)jp - 1, 0), 255);
+    uint32_t k = (uint32_t)min(max((int32_t)ip - 1, 0), 255);
+    const uint8_t *t = rgba + (j * 256u + k) * 4u;
+    return make_float4((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f,
+                       (float)t[3] / 255.0f);
+}
 __global__ void k_lut_expand(const uint8_t *rgba, float4 *lut) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kLutRows * kLutStride) return;
     uint32_t jp = i / kLutStride, ip = i % kLutStride;
-    uint32_t j = (uint32_t)min(max((int32_t)jp - 1, 0), 255);
-    uint32_t k = (uint32_t)min(max((int32_t)ip - 1, 0), 255);
-    const uint8_t *t = rgba + (j * 256u + k) * 4u;
-    lut[i] = make_float4((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f,
-                         (float)t[3] / 255.0f);
+    float4 t = padded_texel(rgba, jp, ip), n = padded_texel(rgba, jp, ip + 1u);
+    lut[2u * i] = t;
+    lut[2u * i + 1u] = make_float4(n.x - t.x, n.y - t.y, n.z - t.z, n.w - t.w);
 }
 
 // ColorU::to_f32 (color/src/lib.rs:70-73) then f16::from_f32 (gpu/renderer.rs:724-727).

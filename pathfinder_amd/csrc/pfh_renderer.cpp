@@ -111,7 +111,8 @@ class EventProfiler : public pfh::Profiler {
         for (hipEvent_t e : free_) (void)hipEventDestroy(e);
     }
     bool wants(int k) const override {
-        return mode == PFH_PROFILE_ALL || (mode == PFH_PROFILE_ONE && k == only);
+        return mode == PFH_PROFILE_ALL || mode == PFH_PROFILE_SERIAL ||
+               (mode == PFH_PROFILE_ONE && k == only);
     }
     void begin(hipStream_t s, int k) override {
         Pending p;
@@ -562,6 +563,13 @@ pfh_status enqueue_frame(pfh_context *ctx) {
                                     hipMemcpyDeviceToDevice, ctx->stream));
     pfh_status s = start_frame_on_side_streams(ctx);
     if (s != PFH_OK) return s;
+    if (ctx->prof.mode == PFH_PROFILE_SERIAL) {
+        // every stage of every batch in order on the context stream, so that
+        // each kernel's events time it alone
+        for (auto &b : ctx->batches)
+            if ((s = run_batch(ctx, *b, ctx->stream)) != PFH_OK) return s;
+        return join_side_streams(ctx);
+    }
     // Every batch's tiles are initialised on the context stream first (it is
     // idle until the first front ends), then every batch's front stages, the
     // one with the most segments first (its chain is the longest), then the
@@ -767,7 +775,8 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     uint8_t *d_bytes = nullptr;
     if (dalloc(ctx.get(), &d_bytes, 256 * 256 * 4) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     // the kernels' padded layout: 259 rows of 260 texels
-    if (dalloc(ctx.get(), &ctx->d_lut, 259 * 260) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    // 259 rows of 260 padded texels, each a texel and its difference to the next
+    if (dalloc(ctx.get(), &ctx->d_lut, 2 * 259 * 260) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     if (dalloc(ctx.get(), &ctx->d_frame, 1) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     const uint8_t *lut = opt.area_lut_rgba ? opt.area_lut_rgba : pfh_area_lut_rgba;
     if (hipMemcpy(d_bytes, lut, 256 * 256 * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -1315,7 +1324,7 @@ const char *pfh_profile_kernel_name(uint32_t kernel) {
 }
 
 pfh_status pfh_profile_set(pfh_context *ctx, int32_t mode, int32_t kernel) {
-    if (!ctx || mode < PFH_PROFILE_OFF || mode > PFH_PROFILE_ONE ||
+    if (!ctx || mode < PFH_PROFILE_OFF || mode > PFH_PROFILE_SERIAL ||
         (mode == PFH_PROFILE_ONE && (kernel < 0 || kernel >= pfh::K_COUNT)))
         return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad profile mode %d / kernel %d", mode,
                     kernel);

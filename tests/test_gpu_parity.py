@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from helpers import assert_bit_exact, compare_frame, oracle_frame, product_frame
+from pathfinder_amd import _lib as L
 from pathfinder_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -78,6 +79,17 @@ def test_rerender_is_identical(renderer_factory):
         r.rerender()
     r.synchronize()
     assert np.array_equal(first, r.read_pixels())
+    # serialized stages (PFH_PROFILE_SERIAL) give the same frame and time
+    # every kernel
+    first_mask = r.read_mask().copy()
+    r.profile(L.PROFILE_SERIAL)
+    r.profile_reset()
+    r.rerender()
+    prof = r.profile_read()
+    r.profile(L.PROFILE_OFF)
+    assert np.array_equal(first, r.read_pixels())
+    assert np.array_equal(first_mask, r.read_mask())
+    assert prof["fill"][0] >= 1 and prof["sort_tiles"][0] >= 1 and prof["tile"][0] == 1
 
 
 @pytest.mark.slow
