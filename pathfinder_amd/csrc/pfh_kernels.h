@@ -12,6 +12,9 @@ namespace pfh {
 
 // Per-batch device counters. Written by the kernels, read by the host once
 // per frame (pfh_end_scene) and by later kernels of the same batch.
+// Tile rows per column segment of propagate's work list.
+constexpr uint32_t kPropSegmentRows = 16;
+
 struct BatchCounters {
     uint32_t microlines;      // total microlines diced (may exceed capacity)
     uint32_t fills;           // total fills binned (may exceed capacity)
@@ -70,6 +73,11 @@ struct BatchArgs {
                                    // k_fill_link has written its head)
     uint32_t *tile_delta;          // tile_count: bin's backdrop deltas << 24 (tile word 2's top byte)
     uint32_t *col_alpha_offsets;   // column_count + 1
+    const uint2 *prop_segs;        // propagate's column segments {column, segment}, row
+                                   // segment major (16 tile rows per segment)
+    uint32_t prop_seg_count;
+    int32_t *seg_delta;            // [segment][column]: backdrop delta sum of the segment
+    uint32_t *seg_alpha;           // [segment][column]: alpha tiles the segment allocates
     uint2 *alpha_tiles;            // tile_count
     int32_t *zbuffer;              // 8 + fb tiles
     int32_t *first_tile_map;       // fb tiles

@@ -1260,16 +1260,20 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
 
 // ------------------------------------------------------------------ propagate
 
-constexpr uint32_t kPropRows = 8;  // tile rows a column walk fetches at a time
+constexpr uint32_t kPropRows = 8;   // tile rows a segment walk fetches at a time
+constexpr uint32_t kPropSeg = kPropSegmentRows;
 
-// propagate.cs.glsl:103-218 for one column. EMIT = false counts the alpha
-// tiles the column allocates; EMIT = true writes tile words and alpha tiles
-// from `alpha_base` (batch-local) on. Linking and z-buffer writes are done by
-// k_sort_tiles.
+// propagate.cs.glsl:103-218 for rows [y_begin, y_begin + kPropSeg) of one
+// column, entered with the running backdrop of row y_begin. EMIT = false
+// counts the alpha tiles the rows allocate and sums their backdrop deltas
+// into *delta_sum; EMIT = true writes the tile words and alpha tiles from
+// `alpha_base` (batch-local) on. Whether a tile needs an alpha tile does not
+// depend on the running backdrop, so segments of a column are counted
+// independently. Linking and z-buffer writes are done by k_sort_tiles.
 template <bool EMIT>
-__device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t alpha_base,
-                                     uint32_t first_alpha) {
-    int32_t backdrop = a.backdrops[(size_t)col * 3 + 0];
+__device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t y_begin,
+                                      int32_t backdrop, uint32_t alpha_base, uint32_t first_alpha,
+                                      int32_t *delta_sum) {
     int32_t tile_x = a.backdrops[(size_t)col * 3 + 1];
     uint32_t dpath = (uint32_t)a.backdrops[(size_t)col * 3 + 2];
     if (dpath >= a.path_count) {
@@ -1300,9 +1304,11 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
     if (tile_x == 0 && dpath > 0 && a.tile_path_info[dpath - 1].z == dofs)
         word0 = a.tile_path_info[reference_tile_search(a, dofs)].w & 0x00ffffffu;
     uint32_t count = 0;
+    int32_t dsum = 0;
+    uint32_t y_end = min(y_begin + kPropSeg, height);
     // rows are fetched kPropRows at a time (the loads do not depend on the
     // running backdrop), then processed in order
-    for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
+    for (uint32_t y0 = y_begin; y0 < y_end; y0 += kPropRows) {
         uint32_t dl[kPropRows];
         uint2 cl[kPropRows];
         uint8_t fl[kPropRows];
@@ -1325,7 +1331,7 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
 #pragma unroll
         for (uint32_t u = 0; u < kPropRows; u++) {
             uint32_t y = y0 + u;
-            if (y >= height) break;
+            if (y >= y_end) break;
             uint32_t t = dofs + y * width + (uint32_t)tile_x;
             if (t >= a.tile_count) {
                 atomicOr(&a.counters->error, 4u);
@@ -1389,38 +1395,27 @@ __device__ uint32_t propagate_column(const BatchArgs &a, uint32_t col, uint32_t 
                 }
             }
             backdrop += delta;
+            dsum += delta;
         }
         if (stop) break;
     }
+    if (delta_sum) *delta_sum = dsum;
     return count;
 }
 
-// Alpha tile counts per column. An unclipped path gives a tile an alpha
-// tile exactly when bin gave it a fill (propagate.cs.glsl:130-131), so its
-// columns count flags; clipped paths take the full column walk.
+// Per column segment (the batch's segment list, row segment major): its
+// alpha tile count and backdrop delta sum, and the column's alpha count
+// (the scan's input) as the sum over its segments.
 __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
-         c += gridDim.x * blockDim.x) {
-        uint32_t dpath = (uint32_t)a.backdrops[(size_t)c * 3 + 2];
-        const int32_t *md = a.prop_md + (size_t)dpath * 12;
-        if (dpath >= a.path_count || md[7] >= 0) {
-            a.col_alpha_offsets[c] = propagate_column<false>(a, c, 0, 0);
-            continue;
-        }
-        uint32_t width = (uint32_t)(md[2] - md[0]), height = (uint32_t)(md[3] - md[1]);
-        uint32_t t0 = (uint32_t)md[4] + (uint32_t)a.backdrops[(size_t)c * 3 + 1];
-        uint32_t count = 0;
-        for (uint32_t y0 = 0; y0 < height; y0 += kPropRows) {
-            uint8_t fl[kPropRows];
-#pragma unroll
-            for (uint32_t u = 0; u < kPropRows; u++)
-                fl[u] = a.tile_flags[min(t0 + (y0 + u) * width, a.tile_count - 1u)];
-#pragma unroll
-            for (uint32_t u = 0; u < kPropRows; u++)
-                count += (y0 + u < height && t0 + (y0 + u) * width < a.tile_count && fl[u]) ? 1u
-                                                                                             : 0u;
-        }
-        a.col_alpha_offsets[c] = count;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.prop_seg_count;
+         j += gridDim.x * blockDim.x) {
+        uint2 cs = a.prop_segs[j];
+        int32_t dsum = 0;
+        uint32_t n = propagate_segment<false>(a, cs.x, cs.y * kPropSeg, 0, 0, 0, &dsum);
+        size_t k = (size_t)cs.y * a.column_count + cs.x;
+        a.seg_delta[k] = dsum;
+        a.seg_alpha[k] = n;
+        if (n) atomicAdd(&a.col_alpha_offsets[cs.x], n);
     }
 }
 
@@ -1445,11 +1440,22 @@ __global__ void k_propagate_finalize(BatchArgs a) {
     a.zbuffer[4] += (int32_t)count;
 }
 
+// Per column segment: the running backdrop and alpha tile id at its first
+// row are the column's (after bin) plus the sums of the segments above it.
 __global__ void __launch_bounds__(kBlock) k_propagate_emit(BatchArgs a) {
     uint32_t first_alpha = a.counters->alpha_start;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < a.column_count;
-         c += gridDim.x * blockDim.x)
-        propagate_column<true>(a, c, a.col_alpha_offsets[c], first_alpha);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.prop_seg_count;
+         j += gridDim.x * blockDim.x) {
+        uint2 cs = a.prop_segs[j];
+        int32_t backdrop = a.backdrops[(size_t)cs.x * 3 + 0];
+        uint32_t alpha = a.col_alpha_offsets[cs.x];
+        for (uint32_t q = 0; q < cs.y; q++) {
+            size_t k = (size_t)q * a.column_count + cs.x;
+            backdrop += a.seg_delta[k];
+            alpha += a.seg_alpha[k];
+        }
+        propagate_segment<true>(a, cs.x, cs.y * kPropSeg, backdrop, alpha, first_alpha, nullptr);
+    }
 }
 
 // ------------------------------------------------------------------ sort
@@ -2657,8 +2663,8 @@ void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p) {
 }
 
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.column_count, kBlock, a.grid_cap);
-    if (a.column_count) {
+    uint32_t g = grid_for(a.prop_seg_count, kBlock, a.grid_cap);
+    if (a.prop_seg_count) {
         Scope t(p, s, K_PROPAGATE_COUNT);
         hipLaunchKernelGGL(k_propagate_count, dim3(g), dim3(kBlock), 0, s, a);
     }
@@ -2668,7 +2674,7 @@ void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
                        a.scan_partials, nullptr, s);
         hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
     }
-    if (a.column_count) {
+    if (a.prop_seg_count) {
         Scope t(p, s, K_PROPAGATE_EMIT);
         hipLaunchKernelGGL(k_propagate_emit, dim3(g), dim3(kBlock), 0, s, a);
     }
