@@ -1944,11 +1944,15 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     __shared__ float4 s_res[kBlock / 64][4 * 64 + 1];
     __shared__ int32_t s_cadj[kBlock / 64][64];
     __shared__ uint8_t s_pair[kBlock / 64][64 * 16];
+    __shared__ unsigned long long s_cmask[kBlock / 64][16];
+    __shared__ uint32_t s_cstart[kBlock / 64][17];
     uint32_t wv = threadIdx.x >> 6, lane = lane_id();
     FillRecord *rec = s_rec[wv];
     float4 *res = s_res[wv];
     int32_t *cadj = s_cadj[wv];
     uint8_t *pair_fill = s_pair[wv];
+    unsigned long long *cmask = s_cmask[wv];
+    uint32_t *cstart = s_cstart[wv];
     LutRef lut = lut_ref(a.lut);
     uint32_t n_alpha = uniform(a.counters->alpha);
     uint32_t start = uniform(a.counters->alpha_start);
@@ -2005,7 +2009,10 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             uint32_t info = (fc.c0 & 15u) | (nvis << 4) | (pb << 9) | (ts << 20);
             if (have) {
                 rec[lane] = f;
-                cadj[lane] = (int32_t)fc.c0 - (int32_t)pb;
+                // first column - first pair (low 16 bits, signed), first column
+                // (4 bits), visible columns (5 bits)
+                cadj[lane] = (int32_t)((((uint32_t)fc.c0 - pb) & 0xffffu) |
+                                       ((fc.c0 & 15u) << 16) | (nvis << 20));
                 for (uint32_t j = 0; j < span; j++) pair_fill[pb + j] = (uint8_t)lane;
             }
             // the next chunk's fills load meanwhile
@@ -2017,20 +2024,50 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             wave_lds_sync();
             for (uint32_t r0 = 0; r0 < pairs; r0 += 64u) {
                 uint32_t r1 = min(r0 + 64u, pairs);
+                // lane i: pair r0 + i, its fill k and pixel column pc (16 when
+                // the fill's window sees no column: its value is never read)
                 uint32_t p = r0 + lane;
+                uint32_t k = 0, pc = 16u;
                 if (p < r1) {
-                    uint32_t k = pair_fill[p];
-                    float x = (float)((int32_t)p + cadj[k]) + 0.5f;
-                    coverage_pair(lut, rec[k], x, res + lane);
+                    k = pair_fill[p];
+                    uint32_t ck = (uint32_t)cadj[k];
+                    uint32_t c = p + (uint32_t)(int32_t)(int16_t)(ck & 0xffffu);
+                    if (c - ((ck >> 16) & 15u) < (ck >> 20)) pc = c;
+                }
+                // The round's values are stored column by column: column q's
+                // pairs, in pair (= stream) order, from cstart[q] on. m_q: the
+                // lanes whose pair is in column q.
+                unsigned long long mine = 0, lanes_q = 0;
+#pragma unroll
+                for (uint32_t q = 0; q < 16; q++) {
+                    unsigned long long m = __ballot(pc == q);
+                    mine = pc == q ? m : mine;
+                    lanes_q = lane == q ? m : lanes_q;
+                }
+                uint32_t cnt = (uint32_t)__popcll(lanes_q);
+                uint32_t cs = wave_inclusive_scan(cnt) - cnt;
+                if (lane < 16u) {
+                    cmask[lane] = lanes_q;
+                    cstart[lane] = cs;
+                }
+                uint32_t rank = (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
+                wave_lds_sync();
+                if (pc < 16u) {
+                    float x = (float)pc + 0.5f;
+                    coverage_pair(lut, rec[k], x, res + cstart[pc] + rank);
                 }
                 wave_lds_sync();
-                // the fills with a pair in [r0, r1), in stream order
+                // this lane's column: which pairs of the round it holds and
+                // where its values start
+                unsigned long long my_lanes = cmask[col];
+                const float4 *mine_v = res + 64u * grp + cstart[col];
+                // the fills with a pair in [r0, r1), tile by tile in stream
+                // order; a tile's pairs in the round are [pa, pz) and this
+                // lane's of them are its column's values [s, e)
                 uint32_t ka = (uint32_t)__popcll(__ballot(have && pe <= r0));
                 uint32_t kb = (uint32_t)__popcll(__ballot(have && pb < r1));
-                uint32_t rw = r1 - r0;
-                for (uint32_t k = ka; k < kb; k++) {
-                    uint32_t fi = (uint32_t)__builtin_amdgcn_readlane((int)info, (int)k);
-                    int32_t t = (int32_t)(fi >> 20);
+                for (uint32_t kk = ka; kk < kb;) {
+                    int32_t t = (int32_t)__builtin_amdgcn_readlane((int)ts, (int)kk);
                     if (t != cur) {
                         if (cur >= 0)
                             store_mask(a, start + g0 + (uint32_t)cur,
@@ -2040,11 +2077,20 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                         cur = t;
                         acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     }
-                    uint32_t d = col - (fi & 15u);
-                    uint32_t pp = ((fi >> 9) & 0x7ffu) + d - r0;
-                    bool use = d < ((fi >> 4) & 31u) && pp < rw;
-                    float4 v = res[use ? pp + 64u * grp : kZeroSlot];
-                    acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+                    unsigned long long seg = __ballot(have && ts == (uint32_t)t);
+                    uint32_t ke = min(64u - (uint32_t)__clzll((long long)seg), kb);
+                    uint32_t pa = max((uint32_t)__builtin_amdgcn_readlane((int)pb, (int)kk), r0) - r0;
+                    uint32_t pz =
+                        min((uint32_t)__builtin_amdgcn_readlane((int)pe, (int)(ke - 1u)), r1) - r0;
+                    unsigned long long la = pa >= 64u ? ~0ull : (1ull << pa) - 1ull;
+                    unsigned long long lz = pz >= 64u ? ~0ull : (1ull << pz) - 1ull;
+                    uint32_t sj = (uint32_t)__popcll(my_lanes & la);
+                    uint32_t ej = (uint32_t)__popcll(my_lanes & lz);
+                    for (uint32_t jj = sj; jj < ej; jj++) {
+                        float4 v = mine_v[jj];
+                        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+                    }
+                    kk = ke;
                 }
                 // every lane is done with this round's values
                 wave_lds_sync();
