@@ -2086,9 +2086,13 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                     unsigned long long lz = pz >= 64u ? ~0ull : (1ull << pz) - 1ull;
                     uint32_t sj = (uint32_t)__popcll(my_lanes & la);
                     uint32_t ej = (uint32_t)__popcll(my_lanes & lz);
-                    for (uint32_t jj = sj; jj < ej; jj++) {
+                    // two values per step, read together (the second one is
+                    // the zero slot past the end)
+                    for (uint32_t jj = sj; jj < ej; jj += 2u) {
                         float4 v = mine_v[jj];
+                        float4 w = jj + 1u < ej ? mine_v[jj + 1u] : res[kZeroSlot];
                         acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+                        acc.x += w.x; acc.y += w.y; acc.z += w.z; acc.w += w.w;
                     }
                     kk = ke;
                 }
