@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output for the renderer's kernels.
 
-    python -m pathfinder_amd.profiling stats DIR [--command CMD]
+    python -m pathfinder_amd.profiling stats DIR [--frames K] [--command CMD]
     python -m pathfinder_amd.profiling traffic FETCH_DIR WRITE_DIR --copy-bytes N [--command CMD]
     python -m pathfinder_amd.profiling counters DIR [--command CMD]
 
@@ -48,10 +48,39 @@ def kernel_stats(directory):
     return out
 
 
-def stats_markdown(directory, command=None):
+def frame_stats(directory, frames):
+    """Per-kernel launches and duration over the last `frames` frames of a
+    kernel trace (from the frames-th last frame-reset dispatch on): the timed
+    frames of a bench.py run, which come last."""
+    path = _one(os.path.join(directory, "**", "*kernel_trace.csv"))
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if short_name(r["Kernel_Name"]) == FRAME_MARK]
+    if len(marks) < frames:
+        raise ValueError("fewer than %d frames in the trace" % frames)
+    per = {}
+    for r in rows[marks[-frames]:]:
+        k = short_name(r["Kernel_Name"])
+        e = per.setdefault(k, [0, 0.0])
+        e[0] += 1
+        e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    return per
+
+
+def stats_markdown(directory, command=None, frames=0):
     lines = ["# Kernel time, rocprofv3 --kernel-trace --stats", ""]
     if command:
         lines += ["Command (from the repository root):", "", "    " + command, ""]
+    if frames:
+        per = frame_stats(directory, frames)
+        lines += ["The last %d frames of the run (the timed frames of bench.py), from the kernel "
+                  "trace; us per frame and per launch:" % frames, "",
+                  "| kernel | launches | us per frame | average us per launch |",
+                  "|---|---:|---:|---:|"]
+        for k in sorted(per, key=lambda k: -per[k][1]):
+            n, us = per[k]
+            lines.append("| %s | %d | %.1f | %.1f |" % (k, n, us / frames, us / n))
+        lines += ["", "The whole run (autotune, warm-up and timed frames), rocprofv3 --stats:", ""]
     lines += ["| kernel | calls | total ms | average us | % |", "|---|---:|---:|---:|---:|"]
     for r in kernel_stats(directory):
         lines.append("| %s | %d | %.3f | %.1f | %.2f |" % (r["kernel"], r["calls"], r["total_ms"],
@@ -164,6 +193,7 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--command")
+    s.add_argument("--frames", type=int, default=0)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -174,7 +204,7 @@ def main(argv=None):
     c.add_argument("--command")
     args = ap.parse_args(argv)
     if args.what == "stats":
-        sys.stdout.write(stats_markdown(args.dir, args.command))
+        sys.stdout.write(stats_markdown(args.dir, args.command, args.frames))
     elif args.what == "counters":
         sys.stdout.write(counters_markdown(args.dir, args.command))
     else:
