@@ -1573,9 +1573,9 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
                           (uint32_t)(x - minx);
             if (x >= minx && x < maxx && ti < a.tile_count) {
                 t = (int32_t)ti;
-                uint4 w = a.tiles[t];
-                int32_t alpha = sext24(w.z);
-                int32_t bd = (int32_t)w.w >> 24;
+                uint2 w = reinterpret_cast<const uint2 *>(a.tiles + t)[1];  // words 2-3
+                int32_t alpha = sext24(w.x);
+                int32_t bd = (int32_t)w.y >> 24;
                 linked = bd != 0 || alpha >= 0;
                 alpha_tile = alpha >= 0;
                 occl = linked && s_zw[lane] != 0 && bd != 0 && alpha < 0;
