@@ -73,9 +73,7 @@ struct BatchArgs {
                                    // k_fill_link has written its head)
     uint32_t *tile_delta;          // tile_count: bin's backdrop deltas << 24 (tile word 2's top byte)
     uint32_t *col_alpha_offsets;   // column_count + 1
-    const uint2 *prop_segs;        // propagate's column segments {column, segment}, row
-                                   // segment major (16 tile rows per segment)
-    uint32_t prop_seg_count;
+    uint32_t prop_seg_rows;        // segments (16 tile rows each) of the tallest column
     int32_t *seg_delta;            // [segment][column]: backdrop delta sum of the segment
     uint32_t *seg_alpha;           // [segment][column]: alpha tiles the segment allocates
     uint2 *alpha_tiles;            // tile_count

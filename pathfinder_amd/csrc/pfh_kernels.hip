@@ -1403,13 +1403,24 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
     return count;
 }
 
-// Per column segment (the batch's segment list, row segment major): its
-// alpha tile count and backdrop delta sum, and the column's alpha count
-// (the scan's input) as the sum over its segments.
+// The number of kPropSeg-row segments of column col (one for a column whose
+// path index is out of range, which propagate_segment reports).
+__device__ __forceinline__ uint32_t column_segments(const BatchArgs &a, uint32_t col) {
+    uint32_t dpath = (uint32_t)a.backdrops[(size_t)col * 3 + 2];
+    if (dpath >= a.path_count) return 1;
+    const int32_t *md = a.prop_md + (size_t)dpath * 12;
+    int32_t height = md[3] - md[1];
+    return height > 0 ? ((uint32_t)height + kPropSeg - 1) / kPropSeg : 0u;
+}
+
+// Workgroups (x, y) cover columns 256x.. of row segment y (neighbouring
+// threads walk neighbouring columns); threads past their column's height
+// have nothing to do.
+// Per column segment: its alpha tile count and backdrop delta sum, and the
+// column's alpha count (the scan's input) as the sum over its segments.
 __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.prop_seg_count;
-         j += gridDim.x * blockDim.x) {
-        uint2 cs = a.prop_segs[j];
+    uint2 cs = make_uint2(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+    if (cs.x < a.column_count && cs.y < column_segments(a, cs.x)) {
         int32_t dsum = 0;
         uint32_t n = propagate_segment<false>(a, cs.x, cs.y * kPropSeg, 0, 0, 0, &dsum);
         size_t k = (size_t)cs.y * a.column_count + cs.x;
@@ -1444,9 +1455,8 @@ __global__ void k_propagate_finalize(BatchArgs a) {
 // row are the column's (after bin) plus the sums of the segments above it.
 __global__ void __launch_bounds__(kBlock) k_propagate_emit(BatchArgs a) {
     uint32_t first_alpha = a.counters->alpha_start;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < a.prop_seg_count;
-         j += gridDim.x * blockDim.x) {
-        uint2 cs = a.prop_segs[j];
+    uint2 cs = make_uint2(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+    if (cs.x < a.column_count && cs.y < column_segments(a, cs.x)) {
         int32_t backdrop = a.backdrops[(size_t)cs.x * 3 + 0];
         uint32_t alpha = a.col_alpha_offsets[cs.x];
         for (uint32_t q = 0; q < cs.y; q++) {
@@ -2713,10 +2723,11 @@ void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p) {
 }
 
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
-    uint32_t g = grid_for(a.prop_seg_count, kBlock, a.grid_cap);
-    if (a.prop_seg_count) {
+    dim3 g((a.column_count + kBlock - 1) / kBlock, a.prop_seg_rows);
+    bool any = a.column_count && a.prop_seg_rows;
+    if (any) {
         Scope t(p, s, K_PROPAGATE_COUNT);
-        hipLaunchKernelGGL(k_propagate_count, dim3(g), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_propagate_count, g, dim3(kBlock), 0, s, a);
     }
     {
         Scope t(p, s, K_PROPAGATE_SCAN);
@@ -2724,9 +2735,9 @@ void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
                        a.scan_partials, nullptr, s);
         hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
     }
-    if (a.prop_seg_count) {
+    if (any) {
         Scope t(p, s, K_PROPAGATE_EMIT);
-        hipLaunchKernelGGL(k_propagate_emit, dim3(g), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_propagate_emit, g, dim3(kBlock), 0, s, a);
     }
 }
 

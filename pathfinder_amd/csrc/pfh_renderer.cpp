@@ -186,9 +186,7 @@ struct Batch {
     pfh_backdrop_info *d_backdrops_init = nullptr;
     // device working buffers
     pfh_backdrop_info *d_backdrops = nullptr;
-    std::vector<uint2> prop_segs;      // propagate's column segments (host copy)
-    uint32_t prop_seg_rows = 0;        // segments of the tallest column
-    uint2 *d_prop_segs = nullptr;
+    uint32_t prop_seg_rows = 0;        // propagate segments of the tallest column
     int32_t *d_seg_delta = nullptr;
     uint32_t *d_seg_alpha = nullptr;
     uint32_t *d_seg_offsets = nullptr;
@@ -319,7 +317,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     }
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
-    dfree(ctx, b.d_prop_segs); dfree(ctx, b.d_seg_delta); dfree(ctx, b.d_seg_alpha);
+    dfree(ctx, b.d_seg_delta); dfree(ctx, b.d_seg_alpha);
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
     dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
     dfree(ctx, b.d_path_fills);
@@ -389,35 +387,18 @@ Batch *find_batch(pfh_context *ctx, uint32_t id) {
     return nullptr;
 }
 
-// Propagate's work list: every column cut into segments of
-// kPropSegmentRows tile rows,
-// listed row segment by row segment (segment s of every column that has
-// one, in column order), so that neighbouring threads walk neighbouring
-// columns. A column whose path index is out of range gets one segment (the
-// kernel reports it).
-void build_prop_segments(Batch &b) {
-    const pfh::BatchData &h = b.host;
-    size_t C = h.backdrops.size();
-    std::vector<uint32_t> n(C);
+// Propagate works on column segments of kPropSegmentRows tile rows; the
+// tallest path's height gives the segment rows of its grid.
+uint32_t prop_segment_rows(const pfh::BatchData &h) {
     uint32_t rows = 0;
-    for (size_t c = 0; c < C; c++) {
-        uint32_t p = h.backdrops[c].path_index;
-        uint32_t segs = 1;
-        if (p < h.propagate_metadata.size()) {
-            const int32_t *r = h.propagate_metadata[p].tile_rect;
-            int64_t height = (int64_t)r[3] - r[1];
-            segs = height > 0 ? (uint32_t)((height + pfh::kPropSegmentRows - 1) /
-                                           pfh::kPropSegmentRows)
-                              : 0;
-        }
-        n[c] = segs;
-        rows = std::max(rows, segs);
+    for (const pfh_propagate_metadata &m : h.propagate_metadata) {
+        int64_t height = (int64_t)m.tile_rect[3] - m.tile_rect[1];
+        if (height > 0)
+            rows = std::max(rows, (uint32_t)((height + pfh::kPropSegmentRows - 1) /
+                                             pfh::kPropSegmentRows));
     }
-    b.prop_segs.clear();
-    for (uint32_t sg = 0; sg < rows; sg++)
-        for (size_t c = 0; c < C; c++)
-            if (sg < n[c]) b.prop_segs.push_back(make_uint2((uint32_t)c, sg));
-    b.prop_seg_rows = rows;
+    // a column with an out-of-range path index takes one segment (reported)
+    return std::max(rows, 1u);
 }
 
 pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
@@ -466,8 +447,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.tile_flags = b.d_tile_flags;
     a.tile_delta = b.d_tile_delta;
     a.col_alpha_offsets = b.d_col_alpha;
-    a.prop_segs = b.d_prop_segs;
-    a.prop_seg_count = (uint32_t)b.prop_segs.size();
+    a.prop_seg_rows = b.prop_seg_rows;
     a.seg_delta = b.d_seg_delta;
     a.seg_alpha = b.d_seg_alpha;
     a.alpha_tiles = b.d_alpha;
@@ -759,8 +739,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = upload(ctx, &b.d_prop, h.propagate_metadata, front)) != PFH_OK) return s;
     if ((s = upload(ctx, &b.d_backdrops_init, h.backdrops, front)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_backdrops, h.backdrops.size())) != PFH_OK) return s;
-    build_prop_segments(b);
-    if ((s = upload(ctx, &b.d_prop_segs, b.prop_segs, front)) != PFH_OK) return s;
+    b.prop_seg_rows = prop_segment_rows(h);
     if ((s = dalloc(ctx, &b.d_seg_delta, (size_t)b.prop_seg_rows * h.backdrops.size())) != PFH_OK)
         return s;
     if ((s = dalloc(ctx, &b.d_seg_alpha, (size_t)b.prop_seg_rows * h.backdrops.size())) != PFH_OK)
