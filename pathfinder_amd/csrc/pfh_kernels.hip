@@ -427,36 +427,73 @@ __device__ __forceinline__ uint4 pack_microline(float x0, float y0, float x1, fl
                       path);
 }
 
+// Point at parameter t of a diced segment: nested mix as dice.cs.glsl:117-122
+// (the cubic's de Casteljau steps; a line is one mix).
+__device__ __forceinline__ float2 dice_eval(const float4 &pc, const float4 &pd, bool curve,
+                                            float t) {
+    float2 p0 = make_float2(pc.x, pc.y), c0 = make_float2(pc.z, pc.w);
+    float2 c1 = make_float2(pd.x, pd.y), p3 = make_float2(pd.z, pd.w);
+    if (!curve) return make_float2(mixf(p0.x, p3.x, t), mixf(p0.y, p3.y, t));
+    float2 p01 = make_float2(mixf(p0.x, c0.x, t), mixf(p0.y, c0.y, t));
+    float2 p12 = make_float2(mixf(c0.x, c1.x, t), mixf(c0.y, c1.y, t));
+    float2 p23 = make_float2(mixf(c1.x, p3.x, t), mixf(c1.y, p3.y, t));
+    float2 p012 = make_float2(mixf(p01.x, p12.x, t), mixf(p01.y, p12.y, t));
+    float2 p123 = make_float2(mixf(p12.x, p23.x, t), mixf(p12.y, p23.y, t));
+    return make_float2(mixf(p012.x, p123.x, t), mixf(p012.y, p123.y, t));
+}
+
+// dice.cs.glsl:204-216 for a block of 256 segments. The shader emits a
+// segment's n microlines in one thread, each starting where the previous one
+// ended (at t = i / n). Here the block's microlines are shared out evenly:
+// thread k takes microlines k, k + 256, ... of the block, finds its segment
+// among the block's output offsets in LDS and evaluates both ends, t = i / n
+// (the segment's start itself for i = 0) and t = (i + 1) / n, with the same
+// expressions, so every microline is the same.
 __global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
     __shared__ uint32_t zs[kBlock];
     __shared__ uint32_t qrange[2];
+    __shared__ float4 s_pc[kBlock], s_pd[kBlock];  // p0, c0 | c1, p3
+    __shared__ uint32_t s_first[kBlock];           // block-relative first microline
+    __shared__ uint32_t s_n[kBlock];               // n | curve << 31
+    __shared__ uint32_t s_path[kBlock];
     if (a.path_count == 0) return;
     for (uint32_t s0 = blockIdx.x * kBlock; s0 < a.segment_count; s0 += gridDim.x * kBlock) {
         uint32_t seg = s0 + threadIdx.x;
         uint32_t path = block_segment_path(a, s0, zs, qrange);
-        if (seg >= a.segment_count) continue;
+        uint32_t s_end = min(s0 + kBlock, a.segment_count);
+        uint32_t lo = a.seg_offsets[s0], hi = a.seg_offsets[s_end];
         DiceSeg d;
-        if (!dice_setup(a, seg, path, d)) continue;
-        uint32_t first = a.seg_offsets[seg];
-        float2 prev = d.p0;
-        for (int32_t i = 0; i < d.n; i++) {
-            float t = (float)(i + 1) / (float)d.n;
-            float2 next;
-            if (d.curve) {
-                float2 p01 = make_float2(mixf(d.p0.x, d.c0.x, t), mixf(d.p0.y, d.c0.y, t));
-                float2 p12 = make_float2(mixf(d.c0.x, d.c1.x, t), mixf(d.c0.y, d.c1.y, t));
-                float2 p23 = make_float2(mixf(d.c1.x, d.p3.x, t), mixf(d.c1.y, d.p3.y, t));
-                float2 p012 = make_float2(mixf(p01.x, p12.x, t), mixf(p01.y, p12.y, t));
-                float2 p123 = make_float2(mixf(p12.x, p23.x, t), mixf(p12.y, p23.y, t));
-                next = make_float2(mixf(p012.x, p123.x, t), mixf(p012.y, p123.y, t));
-            } else {
-                next = make_float2(mixf(d.p0.x, d.p3.x, t), mixf(d.p0.y, d.p3.y, t));
-            }
-            uint32_t o = first + (uint32_t)i;
+        bool ok = seg < a.segment_count && dice_setup(a, seg, path, d);
+        // a segment that failed the setup counted 0 microlines: an empty range
+        s_first[threadIdx.x] = (seg < a.segment_count ? a.seg_offsets[seg] : hi) - lo;
+        s_n[threadIdx.x] = ok ? ((uint32_t)d.n | (d.curve ? 0x80000000u : 0u)) : 0u;
+        s_pc[threadIdx.x] = ok ? make_float4(d.p0.x, d.p0.y, d.c0.x, d.c0.y)
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        s_pd[threadIdx.x] = ok ? make_float4(d.c1.x, d.c1.y, d.p3.x, d.p3.y)
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        s_path[threadIdx.x] = ok ? d.path : 0u;
+        __syncthreads();
+        uint32_t total = hi - lo, last = s_end - s0 - 1;
+        for (uint32_t m = threadIdx.x; m < total; m += kBlock) {
+            // the last segment of the block starting at or before m (an empty
+            // segment starts where the next one does, so this is never one)
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t step = kBlock / 2; step != 0; step >>= 1)
+                if (k + step <= last && s_first[k + step] <= m) k += step;
+            uint32_t nc = s_n[k];
+            if (nc == 0) continue;
+            int32_t n = (int32_t)(nc & 0x7fffffffu), i = (int32_t)(m - s_first[k]);
+            bool curve = (nc >> 31) != 0;
+            float4 pc = s_pc[k], pd = s_pd[k];
+            float2 from = i == 0 ? make_float2(pc.x, pc.y)
+                                 : dice_eval(pc, pd, curve, (float)i / (float)n);
+            float2 to = dice_eval(pc, pd, curve, (float)(i + 1) / (float)n);
+            uint32_t o = lo + m;
             if (o < a.microline_capacity)
-                a.microlines[o] = pack_microline(prev.x, prev.y, next.x, next.y, d.path);
-            prev = next;
+                a.microlines[o] = pack_microline(from.x, from.y, to.x, to.y, s_path[k]);
         }
+        __syncthreads();  // the next block of segments rewrites the LDS
     }
 }
 
