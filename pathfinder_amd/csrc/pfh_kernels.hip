@@ -1508,26 +1508,35 @@ __global__ void __launch_bounds__(kBlock) k_propagate_emit(BatchArgs a) {
 // ------------------------------------------------------------------ sort
 
 // Lists the paths whose tile rect covers each framebuffer tile row, in path
-// order (one workgroup per row).
+// order (one workgroup per row; a thread tests four consecutive paths per
+// step, so a step covers 1024 paths with one block scan).
 __global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
+    constexpr uint32_t kPer = 4;
     __shared__ uint32_t lds4[4];
     int32_t y = (int32_t)blockIdx.x;
     if (y >= a.fb_height_tiles) return;
     uint32_t carry = 0;
     uint32_t *out = a.row_paths + (size_t)y * a.path_count;
-    for (uint32_t base = 0; base < a.path_count; base += kBlock) {
-        uint32_t p = base + threadIdx.x;
-        uint32_t hit = 0;
-        if (p < a.path_count) {
-            uint4 info = a.tile_path_info[p];
+    for (uint32_t base = 0; base < a.path_count; base += kBlock * kPer) {
+        uint32_t p0 = base + threadIdx.x * kPer;
+        uint32_t hits = 0, n = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            uint32_t p = p0 + k;
+            uint4 info = a.tile_path_info[min(p, a.path_count - 1u)];
             int32_t miny = (int32_t)(info.x) >> 16, maxy = (int32_t)(info.y) >> 16;
             int32_t minx = (int32_t)(info.x << 16) >> 16, maxx = (int32_t)(info.y << 16) >> 16;
-            hit = (y >= miny && y < maxy && minx < maxx && maxx > 0 &&
-                   minx < a.fb_width_tiles) ? 1u : 0u;
+            bool hit = p < a.path_count && y >= miny && y < maxy && minx < maxx && maxx > 0 &&
+                       minx < a.fb_width_tiles;
+            hits |= (hit ? 1u : 0u) << k;
+            n += hit ? 1u : 0u;
         }
         uint32_t tot;
-        uint32_t ex = block_exclusive_scan(hit, lds4, &tot);
-        if (hit) out[carry + ex] = p;
+        uint32_t ex = block_exclusive_scan(n, lds4, &tot);
+        uint32_t o = carry + ex;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++)
+            if (hits >> k & 1u) out[o++] = p0 + k;
         carry += tot;
     }
     if (threadIdx.x == 0) a.row_counts[y] = carry;
