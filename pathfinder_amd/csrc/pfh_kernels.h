@@ -14,6 +14,10 @@ namespace pfh {
 // per frame (pfh_end_scene) and by later kernels of the same batch.
 // Tile rows per column segment of propagate's work list.
 constexpr uint32_t kPropSegmentRows = 16;
+// tile_status bits: the tile is linked into its framebuffer tile's list (a
+// backdrop or an alpha tile), it occludes the tiles below it (a backdrop, no
+// alpha tile and a path that writes z), it has an alpha tile
+constexpr uint8_t kTileLinked = 1, kTileOccluder = 2, kTileAlpha = 4;
 
 struct BatchCounters {
     uint32_t microlines;      // total microlines diced (may exceed capacity)
@@ -72,6 +76,8 @@ struct BatchArgs {
     uint8_t *tile_flags;           // tile_count: nonzero when bin gave the tile a fill (2 once
                                    // k_fill_link has written its head)
     uint32_t *tile_delta;          // tile_count: bin's backdrop deltas << 24 (tile word 2's top byte)
+    uint8_t *tile_status;          // tile_count, from propagate: kTileLinked | kTileOccluder |
+                                   // kTileAlpha, what k_sort_tiles reads of words 2-3
     uint32_t *col_alpha_offsets;   // column_count + 1
     uint32_t prop_seg_rows;        // segments (16 tile rows each) of the tallest column
     int32_t *seg_delta;            // [segment][column]: backdrop delta sum of the segment

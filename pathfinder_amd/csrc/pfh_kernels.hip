@@ -1323,6 +1323,7 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
     uint32_t height = ry1 - ry0, width = rx1 - rx0;
     uint32_t dofs = (uint32_t)md[4];
     int32_t cpi = md[7];
+    bool zwrite = md[6] != 0;
     uint32_t crx0 = 0, cry0 = 0, crx1 = 0, cry1 = 0, cofs = 0;
     bool clip = cpi >= 0 && a.clip_prop_md != nullptr && (uint32_t)cpi < a.clip_path_count;
     if (cpi >= 0 && !clip) atomicOr(&a.counters->error, 2u);
@@ -1424,6 +1425,11 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
                 // kernel, writes that word), alpha tile and delta, word 3
                 uint32_t z = ((uint32_t)draw_alpha & 0x00ffffffu) | ((uint32_t)delta << 24);
                 uint32_t w = (y == 0 ? word0 : word) | ((uint32_t)tile_backdrop << 24);
+                a.tile_status[t] =
+                    (uint8_t)((tile_backdrop != 0 || draw_alpha >= 0 ? kTileLinked : 0) |
+                              (tile_backdrop != 0 && draw_alpha < 0 && zwrite ? kTileOccluder
+                                                                               : 0) |
+                              (draw_alpha >= 0 ? kTileAlpha : 0));
                 if (have_mask) {
                     a.tiles[t].x = 0xffffffffu;
                     *reinterpret_cast<uint2 *>(&a.tiles[t].z) = make_uint2(z, w);
@@ -1550,15 +1556,17 @@ __global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
 // a tile's next is the previous linked tile (the next lower index); the sort
 // keeps tiles with index >= z in ascending order and leaves culled tiles with
 // their pre-sort next. One wavefront walks the covering paths from the top
-// down, 64 at a time, and writes exactly those words.
-// A workgroup takes kSortBand framebuffer tiles of one row (a wavefront each)
-// and stages the row's path list in LDS 64 entries at a time for all of them.
+// down, 64 at a time, and writes exactly those words; of each covering tile
+// it reads the status byte propagate wrote beside it.
+// A workgroup takes kSortBand framebuffer tiles of one row (a wavefront each),
+// lists in LDS the row's paths that overlap the band, and stages their rects
+// in LDS 64 at a time for all of its wavefronts.
 constexpr uint32_t kSortBand = 16;
 constexpr uint32_t kBandCap = 8192;  // band list entries held in LDS
 
 __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
     __shared__ int32_t s_miny[64], s_minx[64], s_maxx[64];
-    __shared__ uint32_t s_first[64], s_zw[64];
+    __shared__ uint32_t s_first[64];
     __shared__ uint32_t s_band[kBandCap];
     __shared__ uint32_t s_wave[kSortBand + 1];
     uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
@@ -1617,7 +1625,6 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
             s_minx[lane] = (int32_t)(info.x << 16) >> 16;
             s_maxx[lane] = (int32_t)(info.y << 16) >> 16;
             s_first[lane] = info.z;
-            s_zw[lane] = a.prop_md[(size_t)p * 12 + 6] != 0;
         }
         __syncthreads();
         bool valid = active && k < end;
@@ -1629,12 +1636,10 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
                           (uint32_t)(x - minx);
             if (x >= minx && x < maxx && ti < a.tile_count) {
                 t = (int32_t)ti;
-                uint2 w = reinterpret_cast<const uint2 *>(a.tiles + t)[1];  // words 2-3
-                int32_t alpha = sext24(w.x);
-                int32_t bd = (int32_t)w.y >> 24;
-                linked = bd != 0 || alpha >= 0;
-                alpha_tile = alpha >= 0;
-                occl = linked && s_zw[lane] != 0 && bd != 0 && alpha < 0;
+                uint32_t st = a.tile_status[t];  // words 2-3 as propagate wrote them
+                linked = (st & kTileLinked) != 0;
+                alpha_tile = (st & kTileAlpha) != 0;
+                occl = (st & kTileOccluder) != 0;
             }
         }
         __syncthreads();

@@ -201,6 +201,7 @@ struct Batch {
     uint4 *d_tiles = nullptr;
     uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
     uint8_t *d_tile_flags = nullptr;   // 1 per tile bin gave a fill
+    uint8_t *d_tile_status = nullptr;  // per tile: linked / solid / alpha bits
     uint32_t *d_tile_delta = nullptr;  // bin's backdrop deltas per tile
     uint32_t *d_col_alpha = nullptr;
     uint2 *d_alpha = nullptr;
@@ -324,6 +325,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_tile_runs);
     dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_tile_flags); dfree(ctx, b.d_tile_delta);
+    dfree(ctx, b.d_tile_status);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
     dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
@@ -445,6 +447,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.path_fills = b.d_path_fills;
     a.tiles = b.d_tiles;
     a.tile_flags = b.d_tile_flags;
+    a.tile_status = b.d_tile_status;
     a.tile_delta = b.d_tile_delta;
     a.col_alpha_offsets = b.d_col_alpha;
     a.prop_seg_rows = b.prop_seg_rows;
@@ -748,6 +751,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_flags, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tile_status, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_delta, h.tile_count)) != PFH_OK) return s;
     if (ctx->gradients && (s = dalloc(ctx, &b.d_tile_paths, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
