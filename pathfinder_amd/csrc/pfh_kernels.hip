@@ -2096,13 +2096,13 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                     if (c - ((ck >> 16) & 15u) < (ck >> 20)) pc = c;
                 }
                 // The round's values are stored column by column: column q's
-                // pairs, in pair (= stream) order, from cstart[q] on. m_q: the
-                // lanes whose pair is in column q.
-                unsigned long long mine = 0, lanes_q = 0;
+                // pairs, in pair (= stream) order, from cstart[q] on. Lane
+                // q < 16 gets the mask of the lanes whose pair is in column q
+                // (0 in lanes 16..63).
+                unsigned long long lanes_q = 0;
 #pragma unroll
                 for (uint32_t q = 0; q < 16; q++) {
                     unsigned long long m = __ballot(pc == q);
-                    mine = pc == q ? m : mine;
                     lanes_q = lane == q ? m : lanes_q;
                 }
                 uint32_t cnt = (uint32_t)__popcll(lanes_q);
@@ -2111,9 +2111,9 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                     cmask[lane] = lanes_q;
                     cstart[lane] = cs;
                 }
-                uint32_t rank = (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
                 wave_lds_sync();
                 if (pc < 16u) {
+                    uint32_t rank = (uint32_t)__popcll(cmask[pc] & ((1ull << lane) - 1ull));
                     float x = (float)pc + 0.5f;
                     coverage_pair(lut, rec[k], x, res + cstart[pc] + rank);
                 }
