@@ -1948,6 +1948,21 @@ __device__ __forceinline__ void fill_lookup(const BatchArgs &a, uint32_t pos, ui
     if (q < total) fw = a.sorted_fills[trx + tn - 1u - (q - tS)];
 }
 
+// The compiler's writelane intrinsic (v_writelane_b32: `value` into lane
+// `lane` of `old`), which this clang has no builtin for.
+__device__ int lane_write(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// Lane Q of (lo, hi) for Q = Q0..15: the ballot of pc == Q (one compare and
+// two lane writes per column; a select on lane == Q needs two moves and two
+// selects more).
+template <uint32_t Q>
+__device__ __forceinline__ void column_ballots(uint32_t pc, uint32_t &lo, uint32_t &hi) {
+    unsigned long long m = __ballot(pc == Q);
+    lo = (uint32_t)lane_write((int)(uint32_t)m, (int)Q, (int)lo);
+    hi = (uint32_t)lane_write((int)(uint32_t)(m >> 32), (int)Q, (int)hi);
+    if constexpr (Q + 1u < 16u) column_ballots<Q + 1u>(pc, lo, hi);
+}
+
 constexpr uint32_t kFillGroup = 64;  // alpha tiles per wavefront group
 constexpr uint32_t kZeroSlot = 256;  // res entry that holds float4(0)
 
@@ -2099,12 +2114,9 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                 // pairs, in pair (= stream) order, from cstart[q] on. Lane
                 // q < 16 gets the mask of the lanes whose pair is in column q
                 // (0 in lanes 16..63).
-                unsigned long long lanes_q = 0;
-#pragma unroll
-                for (uint32_t q = 0; q < 16; q++) {
-                    unsigned long long m = __ballot(pc == q);
-                    lanes_q = lane == q ? m : lanes_q;
-                }
+                uint32_t lq_lo = 0, lq_hi = 0;
+                column_ballots<0>(pc, lq_lo, lq_hi);
+                unsigned long long lanes_q = (unsigned long long)lq_hi << 32 | lq_lo;
                 uint32_t cnt = (uint32_t)__popcll(lanes_q);
                 uint32_t cs = wave_inclusive_scan(cnt) - cnt;
                 if (lane < 16u) {
