@@ -2070,6 +2070,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
         fill_lookup(a, 0u, total, S, rx, n, lane, ts, fw, tlast);
         for (uint32_t pos = 0; pos < total; pos += 64u) {
             bool have = pos + lane < total;
+            unsigned long long have_m = __ballot(have);
             FillRecord f;
             FillColumns fc = make_record(fw, f);
             // every fill takes at least one pair (a fill that sees no column
@@ -2137,8 +2138,8 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                 // the fills with a pair in [r0, r1), tile by tile in stream
                 // order; a tile's pairs in the round are [pa, pz) and this
                 // lane's of them are its column's values [s, e)
-                uint32_t ka = (uint32_t)__popcll(__ballot(have && pe <= r0));
-                uint32_t kb = (uint32_t)__popcll(__ballot(have && pb < r1));
+                uint32_t ka = (uint32_t)__popcll(__ballot(pe <= r0) & have_m);
+                uint32_t kb = (uint32_t)__popcll(__ballot(pb < r1) & have_m);
                 for (uint32_t kk = ka; kk < kb;) {
                     int32_t t = (int32_t)__builtin_amdgcn_readlane((int)ts, (int)kk);
                     if (t != cur) {
@@ -2150,8 +2151,9 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                         cur = t;
                         acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     }
-                    unsigned long long seg = __ballot(have && ts == (uint32_t)t);
-                    uint32_t ke = min(64u - (uint32_t)__clzll((long long)seg), kb);
+                    // nonzero: lane kk is in it
+                    unsigned long long seg = __ballot(ts == (uint32_t)t) & have_m;
+                    uint32_t ke = min(64u - (uint32_t)__builtin_clzll(seg), kb);
                     uint32_t pa = max((uint32_t)__builtin_amdgcn_readlane((int)pb, (int)kk), r0) - r0;
                     uint32_t pz =
                         min((uint32_t)__builtin_amdgcn_readlane((int)pe, (int)(ke - 1u)), r1) - r0;
