@@ -149,10 +149,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t *data,
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t *partials,
-                                                          const uint32_t *n_ptr, uint32_t n_host,
-                                                          uint32_t cap, uint32_t *total_out,
-                                                          uint32_t *data_end) {
+// The scan of the block partials (one workgroup); returns the total in every
+// thread.
+__device__ uint32_t scan_partials_total(uint32_t *partials, const uint32_t *n_ptr,
+                                        uint32_t n_host, uint32_t cap, uint32_t *total_out,
+                                        uint32_t *data_end) {
     __shared__ uint32_t lds4[4];
     uint32_t n = n_ptr ? min(*n_ptr, cap) : n_host;
     uint32_t nb = (n + kScanTile - 1) / kScanTile;
@@ -168,6 +169,28 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t *partials,
     if (threadIdx.x == 0) {
         if (total_out) *total_out = carry;
         data_end[n] = carry;
+    }
+    return carry;
+}
+
+// A count the scan's total gives a buffer: the total clamped to the buffer's
+// capacity goes to *clamped, and overflow gets `bit` when it does not fit.
+// clamped == nullptr: none.
+struct ScanCount {
+    uint32_t *clamped;
+    uint32_t cap;
+    uint32_t *overflow;
+    uint32_t bit;
+};
+
+__global__ void __launch_bounds__(kBlock) k_scan_partials(uint32_t *partials,
+                                                          const uint32_t *n_ptr, uint32_t n_host,
+                                                          uint32_t cap, uint32_t *total_out,
+                                                          uint32_t *data_end, ScanCount count) {
+    uint32_t t = scan_partials_total(partials, n_ptr, n_host, cap, total_out, data_end);
+    if (threadIdx.x == 0 && count.clamped) {
+        *count.clamped = t < count.cap ? t : count.cap;
+        if (t > count.cap) atomicOr(count.overflow, count.bit);
     }
 }
 
@@ -208,15 +231,20 @@ __global__ void __launch_bounds__(kBlock) k_scan_downsweep(uint32_t *data, const
     }
 }
 
-void scan_exclusive(uint32_t *data, const uint32_t *n_ptr, uint32_t n_host, uint32_t cap,
-                    uint32_t *partials, uint32_t *total_out, hipStream_t s) {
+uint32_t scan_blocks(const uint32_t *n_ptr, uint32_t n_host, uint32_t cap) {
     uint32_t max_n = n_ptr ? cap : n_host;
     uint32_t blocks = (max_n + kScanTile - 1) / kScanTile;
-    if (blocks < 1) blocks = 1;
+    return blocks < 1 ? 1 : blocks;
+}
+
+void scan_exclusive(uint32_t *data, const uint32_t *n_ptr, uint32_t n_host, uint32_t cap,
+                    uint32_t *partials, uint32_t *total_out, hipStream_t s,
+                    ScanCount count = ScanCount{nullptr, 0, nullptr, 0}) {
+    uint32_t blocks = scan_blocks(n_ptr, n_host, cap);
     hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kBlock), 0, s, data, n_ptr, n_host, cap,
                        partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, n_ptr, n_host, cap,
-                       total_out, data);
+                       total_out, data, count);
     hipLaunchKernelGGL(k_scan_downsweep, dim3(blocks), dim3(kBlock), 0, s, data, n_ptr, n_host,
                        cap, partials);
 }
@@ -494,16 +522,6 @@ __global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
                 a.microlines[o] = pack_microline(from.x, from.y, to.x, to.y, s_path[k]);
         }
         __syncthreads();  // the next block of segments rewrites the LDS
-    }
-}
-
-// Clamp a device-side total to the buffer capacity and flag overflow.
-__global__ void k_finalize_count(uint32_t *total, uint32_t cap, uint32_t *clamped,
-                                 uint32_t *overflow, uint32_t bit) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        uint32_t t = *total;
-        *clamped = t < cap ? t : cap;
-        if (t > cap) atomicOr(overflow, bit);
     }
 }
 
@@ -1477,10 +1495,9 @@ __global__ void __launch_bounds__(kBlock) k_propagate_count(BatchArgs a) {
 // d3d11/renderer.rs:624-625, 643-646) and writes the z-buffer header the
 // reference leaves behind: bin uploads [6,0,0,0,0,microlines,0,0], bin bumps
 // [1], propagate_tiles zeroes the first tile_area words and propagate bumps [4].
-__global__ void k_propagate_finalize(BatchArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// count: the batch's alpha tiles (the column scan's total).
+__device__ void propagate_finalize(const BatchArgs &a, uint32_t count) {
     BatchCounters *c = a.counters;
-    uint32_t count = a.col_alpha_offsets[a.column_count];
     uint32_t start = a.frame->alpha_count;
     c->alpha = count;
     c->alpha_start = start;
@@ -1492,6 +1509,13 @@ __global__ void k_propagate_finalize(BatchArgs a) {
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     for (uint32_t i = 0; i < 8; i++) a.zbuffer[i] = i < fbn ? 0 : hdr[i];
     a.zbuffer[4] += (int32_t)count;
+}
+
+// The partials of the column scan, then propagate_finalize with its total.
+__global__ void __launch_bounds__(kBlock) k_propagate_scan_partials(BatchArgs a) {
+    uint32_t t = scan_partials_total(a.scan_partials, nullptr, a.column_count, a.column_count,
+                                     nullptr, a.col_alpha_offsets);
+    if (threadIdx.x == 0) propagate_finalize(a, t);
 }
 
 // Per column segment: the running backdrop and alpha tile id at its first
@@ -2740,10 +2764,9 @@ void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
     {
         Scope t(p, s, K_DICE_SCAN);
         scan_exclusive(a.seg_offsets, nullptr, a.segment_count, a.segment_count, a.scan_partials,
-                       &a.counters->microlines, s);
-        hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->microlines,
-                           a.microline_capacity, &a.counters->microline_count,
-                           &a.counters->overflow, 1u);
+                       &a.counters->microlines, s,
+                       ScanCount{&a.counters->microline_count, a.microline_capacity,
+                                 &a.counters->overflow, 1u});
     }
     Scope t(p, s, K_DICE_EMIT);
     hipLaunchKernelGGL(k_dice_emit, dim3(g), dim3(kBlock), 0, s, a);
@@ -2765,9 +2788,8 @@ void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
     Scope t(p, s, K_BIN_SCAN);
     scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
-                   a.scan_partials, &a.counters->fills, s);
-    hipLaunchKernelGGL(k_finalize_count, dim3(1), dim3(64), 0, s, &a.counters->fills,
-                       a.fill_capacity, &a.counters->fill_count, &a.counters->overflow, 2u);
+                   a.scan_partials, &a.counters->fills, s,
+                   ScanCount{&a.counters->fill_count, a.fill_capacity, &a.counters->overflow, 2u});
 }
 
 void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {
@@ -2796,9 +2818,13 @@ void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
     }
     {
         Scope t(p, s, K_PROPAGATE_SCAN);
-        scan_exclusive(a.col_alpha_offsets, nullptr, a.column_count, a.column_count,
-                       a.scan_partials, nullptr, s);
-        hipLaunchKernelGGL(k_propagate_finalize, dim3(1), dim3(64), 0, s, a);
+        uint32_t blocks = scan_blocks(nullptr, a.column_count, a.column_count);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kBlock), 0, s, a.col_alpha_offsets,
+                           nullptr, a.column_count, a.column_count, a.scan_partials);
+        hipLaunchKernelGGL(k_propagate_scan_partials, dim3(1), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_scan_downsweep, dim3(blocks), dim3(kBlock), 0, s,
+                           a.col_alpha_offsets, nullptr, a.column_count, a.column_count,
+                           a.scan_partials);
     }
     if (any) {
         Scope t(p, s, K_PROPAGATE_EMIT);
