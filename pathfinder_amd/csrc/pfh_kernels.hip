@@ -2285,9 +2285,16 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
             for (int c = 0; c < 4; c++) d[sub][c] = (float)((v >> (8 * c)) & 0xffu) / 255.0f;
         }
     }
-    for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count;
-         guard++) {
-        uint4 w = a.tiles[t];
+    // the next tile's words load beside this tile's mask texel: inside the
+    // loop the load is unconditional, at index 0 past the list's end (in
+    // bounds: the loop runs only when tile_count > 0)
+    bool valid = t >= 0 && (uint32_t)t < a.tile_count;
+    uint4 w = make_uint4(0, 0, 0, 0);
+    if (valid) w = a.tiles[t];
+    for (uint32_t guard = 0; valid && guard < a.tile_count; guard++) {
+        int32_t tn = (int32_t)w.x;
+        bool next_valid = tn >= 0 && (uint32_t)tn < a.tile_count;
+        uint4 wn = a.tiles[next_valid ? (uint32_t)tn : 0u];
         int32_t alpha = sext24(w.z);
         uint32_t color = w.w & 0xffffu;
         int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
@@ -2323,7 +2330,8 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
             d[sub][2] = d[sub][2] * ia + base.z * al;
             d[sub][3] = d[sub][3] * ia + al;
         }
-        t = (int32_t)w.x;
+        w = wn;
+        valid = next_valid;
     }
 #pragma unroll
     for (int sub = 0; sub < 4; sub++)
