@@ -98,33 +98,71 @@ def copy_calibration(nbytes):
           flush=True)
 
 
-def cpu_baseline(cmds, size, bg, microlines):
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def median_run(fn, runs=5):
+    """One warm-up call, then the median wall-clock time of `runs` calls and
+    the per-stage split of that run (oracle.stage_seconds)."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
-    o = oracle.Oracle(threads=threads)
-    t0 = time.perf_counter()
-    o.render(cmds, size[0], size[1], bg)
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": round(microlines / dt / 1e6, 4), "unit": "Msegments/s", "cores": threads,
-            "kind": "port",
-            "sample": "one whole frame of the same workload through the oracle (the seven "
-                      "shaders restated in C, sequential dice/bin/propagate, threaded "
-                      "bound/sort/fill/tile): %.2f s" % dt}
+    fn()
+    results = []
+    for _ in range(runs):
+        oracle.stage_reset()
+        t0 = time.perf_counter()
+        fn()
+        results.append((time.perf_counter() - t0, oracle.stage_seconds()))
+    results.sort(key=lambda r: r[0])
+    return results[runs // 2]
 
 
-def cpu_baseline_d3d9(scene, size, bg, microlines):
-    from oracle import d3d9
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    d3d9.render(scene, size, bg, threads=threads)
-    dt = time.perf_counter() - t0
+def cpu_baseline(scene, size, bg, microlines, sample_scene, level):
+    """The CPU restatement of one level timed on the host: the whole frame at
+    the host's cores (at most 16, the box's CPU share) and a tenth-size sample
+    of the same generator at one thread; 1 warm-up + median of 5 each."""
+    import oracle
+    from oracle import d3d9, packing
+    nproc, model = cpu_info()
+    threads = min(16, nproc)
+
+    def runner(sc, sz, th):
+        if level == "d3d9":
+            return lambda: d3d9.render(sc, sz, bg, threads=th)
+        cmds = packing.build(sc)
+        o = oracle.Oracle(threads=th)
+        return lambda: o.render(cmds, sz[0], sz[1], bg)
+
+    def stages_ms(st):
+        keys = [k for k in oracle.STAGES if k.startswith("d3d9_") == (level == "d3d9")]
+        return {k: round(st[k] * 1e3, 1) for k in keys}
+
+    dt, st = median_run(runner(scene, size, threads))
+    s_scene, s_size, s_ml = sample_scene
+    dt1, st1 = median_run(runner(s_scene, s_size, 1))
+    what = ("the D3D11 oracle (the seven shaders restated in C; dice, bin and propagate "
+            "sequential, the other stages threaded)" if level == "d3d11" else
+            "the D3D9-level CPU restatement (tiler.rs flattening and tiling on a thread per "
+            "path, binary16 mask raster, per-tile RGBA8 composite), rated on the D3D11 "
+            "microline count")
     return {"value": round(microlines / dt / 1e6, 4), "unit": "Msegments/s", "cores": threads,
-            "kind": "port",
-            "sample": "one whole frame of the same workload through the D3D9-level CPU "
-                      "restatement (tiler.rs flattening and tiling on a thread per path, "
-                      "binary16 mask raster, per-tile RGBA8 composite), rated on the D3D11 "
-                      "microline count: %.2f s" % dt}
+            "kind": "port", "nproc": nproc, "cpu_model": model, "runs": 5, "warmup": 1,
+            "frame_ms": round(dt * 1e3, 1), "stages_ms": stages_ms(st),
+            "one_thread": {"value": round(s_ml / dt1 / 1e6, 4), "frame_ms": round(dt1 * 1e3, 1),
+                           "stages_ms": stages_ms(st1), "microlines": s_ml},
+            "sample": "whole frames of the same workload through %s, median of 5 after a "
+                      "warm-up at %d threads; one_thread: 2,000 paths and 50 clip paths of the "
+                      "same generator (a tenth of the frame), median of 5 after a warm-up"
+                      % (what, threads)}
 
 
 def diff_leg(steps, warmup):
@@ -141,26 +179,15 @@ def diff_leg(steps, warmup):
         target = dr.render(torch.tensor(d["target_points"]), torch.tensor(d["colors"]))
     p = torch.tensor(d["points"], device="cuda", requires_grad=True)
     c = torch.tensor(d["colors"], device="cuda", requires_grad=True)
-    opt = torch.optim.SGD([p, c], lr=1e-3)
-
-    def step():
-        opt.zero_grad()
-        img = diff.render(dr, p, c)
-        loss = ((img - target) ** 2).sum()
-        loss.backward()
-        opt.step()
-        return loss.item()
-
-    for _ in range(max(warmup, 1)):
-        step()
+    diff.fit(dr, p, c, target, max(warmup, 1))  # warm-up steps of the same fit
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = [step() for _ in range(steps)]
+    losses = diff.fit(dr, p, c, target, steps)
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) / steps * 1e3
     # device times of the last frame: forward (re-rendered) and backward
     img = diff.render(dr, p, c)
-    g = (2.0 * (img - target)).detach().contiguous()
+    g = (2.0 * (img - target) / img.numel()).detach().contiguous()
     gp = torch.zeros((len(shapes.packed), 2), dtype=torch.float32, device="cuda")
     gc = torch.zeros((shapes.path_count, 4), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
@@ -189,7 +216,10 @@ def diff_leg(steps, warmup):
             "fwd_ms": round(fwd_ms, 4), "bwd_ms": round(bwd_ms, 4),
             "fwd_bwd_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3),
             "step_ms": round(step_ms, 3),
-            "loss_first_last": [round(losses[0], 2), round(losses[-1], 2)]}
+            "optimizer": "adam, lr 0.2 px (points) and 0.005 (colours), mean squared error",
+            "loss_first_last": [round(losses[0], 6), round(losses[-1], 6)],
+            "loss_curve": [round(v, 6) for v in losses],
+            "loss_fell": bool(losses[-1] < losses[0])}
 
 
 def batch_leg(kind, steps, warmup, sample=32):
@@ -410,10 +440,18 @@ def main():
             print("bench: batch leg %s" % kind, file=sys.stderr, flush=True)
             line["batch_" + kind] = batch_leg(kind, min(args.steps, 10), args.warmup)
     if not args.no_cpu_baseline:
+        print("bench: cpu baselines", file=sys.stderr, flush=True)
+        s_scene, s_size, _ = synth.scene_4k(paths=2000, clips=50, clipped_paths=200)
         from oracle import packing
-        cmds = packing.build(scene)
-        line["cpu_baseline"] = cpu_baseline(cmds, size, bg, M)
-        line["cpu_baseline_d3d9"] = cpu_baseline_d3d9(scene, size, bg, M)
+        import oracle as _oracle
+        o = _oracle.Oracle(threads=min(16, os.cpu_count() or 1))
+        s_cmds = packing.build(s_scene)
+        o.render(s_cmds, s_size[0], s_size[1], bg)
+        s_ml = sum(len(o.read(b["batch_id"], _oracle.MICROLINES)) for _, b in s_cmds["batches"])
+        o.close()
+        line["cpu_baseline"] = cpu_baseline(scene, size, bg, M, (s_scene, s_size, s_ml), "d3d11")
+        line["cpu_baseline_d3d9"] = cpu_baseline(scene, size, bg, M, (s_scene, s_size, s_ml),
+                                                 "d3d9")
     r.close()
     if args.copy_calibration:
         copy_calibration(256 << 20)

@@ -321,7 +321,11 @@ pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst);
  * buffers: grad_points, 2 floats per point of the uploaded draw segments
  * (pfh_segments.points order, which repeats each contour's first point at
  * its end), and grad_colors, 4 floats (RGBA of the path's paint) per draw
- * path in push order; path_count bounds it. Waits for the result. */
+ * path in push order; path_count bounds it. Waits for the result.
+ * grad_image, grad_points and grad_colors must be ordinary (coarse-grained)
+ * device memory from hipMalloc on the context's device, as torch allocates
+ * it: the gradients are accumulated with hardware float atomics, which are
+ * not guaranteed on fine-grained or host-mapped memory. */
 pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_points,
                         uint32_t point_count, float *grad_colors, uint32_t path_count);
 

@@ -21,6 +21,7 @@ LUT_PATH = os.path.join(HERE, "..", "pathfinder_amd", "data", "area_lut.rgba")
 
 MICROLINES, FILLS, TILES, ZBUFFER, FIRST_TILE_MAP, ALPHA_TILES, BACKDROPS, ALPHA_RANGE = range(8)
 OPT_CLIP_STRIDE_LITERAL = 1
+OPT_PLANT_DICE = 2  # planted dice mistakes, only for tests that show a pin catches them
 _ELEM = {MICROLINES: (np.uint32, 4), FILLS: (np.uint32, 3), TILES: (np.uint32, 4),
          ZBUFFER: (np.int32, 1), FIRST_TILE_MAP: (np.int32, 1), ALPHA_TILES: (np.uint32, 2),
          BACKDROPS: (np.int32, 3), ALPHA_RANGE: (np.uint32, 1)}
@@ -59,6 +60,8 @@ def load():
         "orc_f32_to_f16": (C.c_uint16, [C.c_float]),
         "orc_f16_to_f32": (C.c_float, [C.c_uint16]),
         "orc_compute_coverage": (None, [P, C.c_float, C.c_float, C.c_float, C.c_float, P]),
+        "orc_stage_reset": (None, []),
+        "orc_stage_times": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -155,7 +158,9 @@ class Oracle:
         return out.view(np.uint8).reshape(self.height, self.width, 4)
 
     def render(self, cmds, width, height, background):
-        """Runs a packed command stream (packing.build or CommandList dicts)."""
+        """Runs a packed command stream (packing.build or CommandList dicts),
+        dropping the previous frame's batches."""
+        self.end_frame()
         self.begin_frame(width, height, background)
         self.upload_paints(cmds["paints"])
         self.upload_scene(cmds["draw"], cmds["clip"])
@@ -164,6 +169,22 @@ class Oracle:
             if kind == 4:
                 self.draw_batch(b["batch_id"])
         return self
+
+
+STAGES = ("dice", "bound", "bin", "propagate", "fill", "sort", "tile",
+          "d3d9_tiler", "d3d9_raster", "d3d9_composite")
+
+
+def stage_reset():
+    load().orc_stage_reset()
+
+
+def stage_seconds():
+    """Wall-clock seconds per stage since stage_reset() (D3D11 stages, then the
+    D3D9 level's phases)."""
+    out = np.zeros(len(STAGES), np.float64)
+    load().orc_stage_times(_p(out))
+    return dict(zip(STAGES, out.tolist()))
 
 
 def coverage(fx, fy, tx, ty, oracle=None):

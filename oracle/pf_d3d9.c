@@ -385,6 +385,7 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
     d9_path *clips = (d9_path *)calloc(nc ? nc : 1, sizeof(d9_path));
     d9_path *draws = (d9_path *)calloc(nd ? nd : 1, sizeof(d9_path));
     if (!clips || !draws) return -1;
+    double t_start = orc_now();
 
     /* builder.rs build_clip_paths then build_paths: tile every path */
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
@@ -422,6 +423,8 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
         d9_prepare(p, c);
     }
 
+    double t_tiled = orc_now();
+    orc_stage_seconds[ORC_STAGE_D3D9_TILER] += t_tiled - t_start;
     /* masks: binary16 RGBA, 64 texels per alpha tile, cleared */
     uint16_t *mask = (uint16_t *)calloc(((size_t)n_alpha + 1) * 256, sizeof(uint16_t));
     if (!mask) return -1;
@@ -433,6 +436,8 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
     memcpy(copy, mask, ((size_t)n_alpha + 1) * 256 * sizeof(uint16_t));
     for (uint32_t i = 0; i < nd; i++) d9_combine(&draws[i], mask, copy);
     free(copy);
+    double t_masked = orc_now();
+    orc_stage_seconds[ORC_STAGE_D3D9_RASTER] += t_masked - t_tiled;
 
     /* builder.rs:1013-1028: z-buffer of occluding solid tiles; per-framebuffer
      * tile lists in path order (tile.vs.glsl culls a tile whose path index is
@@ -520,6 +525,7 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
             }
         }
     }
+    orc_stage_seconds[ORC_STAGE_D3D9_COMPOSITE] += orc_now() - t_masked;
     free(base); free(lists); free(start); free(counts); free(z); free(mask);
     for (uint32_t i = 0; i < nc; i++) d9_free_path(&clips[i]);
     for (uint32_t i = 0; i < nd; i++) d9_free_path(&draws[i]);

@@ -43,6 +43,7 @@
 #include "pf_oracle.h"
 
 #include <math.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -92,10 +93,32 @@ struct orc_ctx {
     uint32_t *fb; /* fbw*16 x fbh*16 RGBA8 words */
     orc_batch batches[MAX_BATCHES];
     int opt_clip_stride_literal;
+    int opt_plant_dice;
 };
 
 static int g_threads = 1;
 void orc_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+
+/* Wall-clock seconds per stage since the last orc_stage_reset (CPU baseline
+ * timing): ORC_STAGE_* in pf_oracle.h. */
+double orc_stage_seconds[ORC_STAGE_COUNT];
+double orc_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+void orc_stage_reset(void) {
+    for (int i = 0; i < ORC_STAGE_COUNT; i++) orc_stage_seconds[i] = 0.0;
+}
+void orc_stage_times(double *out) {
+    for (int i = 0; i < ORC_STAGE_COUNT; i++) out[i] = orc_stage_seconds[i];
+}
+#define TIMED(stage, expr)                                     \
+    do {                                                       \
+        double t0_ = orc_now();                                \
+        expr;                                                  \
+        orc_stage_seconds[stage] += orc_now() - t0_;           \
+    } while (0)
 
 /* ---------------------------------------------------------------- helpers */
 
@@ -196,6 +219,7 @@ void orc_destroy(orc_ctx *ctx) {
 
 void orc_set_option(orc_ctx *ctx, int option, int value) {
     if (option == ORC_OPT_CLIP_STRIDE_LITERAL) ctx->opt_clip_stride_literal = value;
+    if (option == ORC_OPT_PLANT_DICE) ctx->opt_plant_dice = value;
 }
 
 void orc_begin_frame(orc_ctx *ctx, int32_t width, int32_t height, int32_t has_background,
@@ -275,13 +299,17 @@ static inline v2 get_point(const float *pts, uint32_t i, const float *T) {
 }
 
 /* dice.cs.glsl:117-124 sampleCurve. */
-static inline v2 sample_curve(v2 p0, v2 p1, v2 p2, v2 p3, float t) {
+static inline v2 sample_curve(v2 p0, v2 p1, v2 p2, v2 p3, float t, int plant_swap) {
     v2 a = {mixf(p0.x, p1.x, t), mixf(p0.y, p1.y, t)};
     v2 b = {mixf(p1.x, p2.x, t), mixf(p1.y, p2.y, t)};
     v2 c = {mixf(p2.x, p3.x, t), mixf(p2.y, p3.y, t)};
     v2 ab = {mixf(a.x, b.x, t), mixf(a.y, b.y, t)};
     v2 bc = {mixf(b.x, c.x, t), mixf(b.y, c.y, t)};
     v2 r = {mixf(ab.x, bc.x, t), mixf(ab.y, bc.y, t)};
+    if (plant_swap) {
+        r.x = mixf(bc.x, ab.x, t);
+        r.y = mixf(bc.y, ab.y, t);
+    }
     return r;
 }
 
@@ -343,6 +371,8 @@ static int stage_dice(orc_ctx *ctx, orc_batch *b) {
         }
         int32_t n = f2i_sat(ceilf(countf));
         if (n < 1) n = 1;
+        if (ctx->opt_plant_dice == 1) n *= 2;
+        if (ctx->opt_plant_dice == 2) n = (n + 1) / 2;
         if (n > (1 << 24)) return -2;
         uint32_t first = counter;
         counter += (uint32_t)n;
@@ -351,7 +381,7 @@ static int stage_dice(orc_ctx *ctx, orc_batch *b) {
             float t = (float)(i + 1) / (float)n;
             v2 next;
             if (curve) {
-                next = sample_curve(p0, c0, c1, p3, t);
+                next = sample_curve(p0, c0, c1, p3, t, ctx->opt_plant_dice == 3);
             } else {
                 next.x = mixf(p0.x, p3.x, t);
                 next.y = mixf(p0.y, p3.y, t);
@@ -837,27 +867,28 @@ int orc_prepare_batch(orc_ctx *ctx, uint32_t batch_id, uint32_t path_source, uin
     b->ftm = (int32_t *)malloc((size_t)(fbn ? fbn : 1) * 4);
     for (int64_t i = 0; i < fbn; i++) b->ftm[i] = -1;
 
-    int rc = stage_dice(ctx, b);
+    int rc;
+    TIMED(ORC_STAGE_DICE, rc = stage_dice(ctx, b));
     if (rc) return rc;
-    stage_bound(b);
+    TIMED(ORC_STAGE_BOUND, stage_bound(b));
     /* bin_segments uploads [6, 0, 0, 0, 0, microline_count, 0, 0] to the header
      * (d3d11/renderer.rs:174-180); bin bumps [1]. */
     int32_t hdr[8] = {6, 0, 0, 0, 0, (int32_t)b->n_microlines, 0, 0};
-    stage_bin(b);
+    TIMED(ORC_STAGE_BIN, stage_bin(b));
     hdr[1] = (int32_t)b->n_fills;
     /* propagate_tiles uploads tile_area zero words at offset 0 (:562-566). */
     for (int i = 0; i < 8; i++) b->zbuffer[i] = (i < fbn) ? 0 : hdr[i];
-    rc = stage_propagate(ctx, b, clip);
+    TIMED(ORC_STAGE_PROPAGATE, rc = stage_propagate(ctx, b, clip));
     if (rc) return rc;
-    stage_fill(ctx, b);
-    stage_sort(ctx, b);
+    TIMED(ORC_STAGE_FILL, stage_fill(ctx, b));
+    TIMED(ORC_STAGE_SORT, stage_sort(ctx, b));
     return 0;
 }
 
 int orc_draw_batch(orc_ctx *ctx, uint32_t batch_id) {
     orc_batch *b = find_batch(ctx, batch_id);
     if (!b) return -1;
-    stage_tile(ctx, b);
+    TIMED(ORC_STAGE_TILE, stage_tile(ctx, b));
     return 0;
 }
 

@@ -49,8 +49,25 @@ enum {
 enum {
     /* propagate.cs.glsl:121-122 reads clip metadata with a stride of two
      * uvec4; the struct is three uvec4 (gpu_data.rs:314-325). 0 = stride 3. */
-    ORC_OPT_CLIP_STRIDE_LITERAL = 1
+    ORC_OPT_CLIP_STRIDE_LITERAL = 1,
+    /* Planted mistakes, for tests that show a pin catches them (never set
+     * outside those tests): 1 = twice dice's segment count, 2 = half of it
+     * (rounded up), 3 = the last mix of sampleCurve with its ends swapped. */
+    ORC_OPT_PLANT_DICE = 2
 };
+
+/* Per-stage wall-clock seconds (steady clock) accumulated since
+ * orc_stage_reset, for the CPU baseline: the seven D3D11 stages, then the
+ * D3D9 level's tiling, mask raster and composite. */
+enum {
+    ORC_STAGE_DICE, ORC_STAGE_BOUND, ORC_STAGE_BIN, ORC_STAGE_PROPAGATE, ORC_STAGE_FILL,
+    ORC_STAGE_SORT, ORC_STAGE_TILE, ORC_STAGE_D3D9_TILER, ORC_STAGE_D3D9_RASTER,
+    ORC_STAGE_D3D9_COMPOSITE, ORC_STAGE_COUNT
+};
+extern double orc_stage_seconds[ORC_STAGE_COUNT];
+double orc_now(void);
+void orc_stage_reset(void);
+void orc_stage_times(double *out);
 
 orc_ctx *orc_create(const uint8_t *area_lut_rgba /* 256*256*4 */);
 void orc_destroy(orc_ctx *ctx);

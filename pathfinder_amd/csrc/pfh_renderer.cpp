@@ -626,17 +626,22 @@ pfh_status enqueue_frame(pfh_context *ctx) {
         pfh::launch_propagate(a, ctx->stream, p);
         HIP_TRY(ctx, hipEventRecord(b.prop_done, ctx->stream));
     }
+    // Fills first (the last batch's fill reads their masks: they are on the
+    // critical path), then the per-row path lists and sorts, which only the
+    // composites read.
+    for (size_t i = 0; i + 1 < n; i++) {
+        Batch &b = *ctx->batches[i];
+        hipStream_t side = ctx->side[i & 1];
+        HIP_TRY(ctx, hipStreamWaitEvent(side, b.prop_done, 0));
+        for (size_t j = 0; j < i; j++)  // earlier batches' masks (its clip masks)
+            HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[j]->fill_done, 0));
+        pfh::launch_fill(batch_args(ctx, b), side, p);
+        HIP_TRY(ctx, hipEventRecord(b.fill_done, side));
+    }
     for (size_t i = 0; i < n; i++) {
         Batch &b = *ctx->batches[i];
         pfh::BatchArgs a = batch_args(ctx, b);
         hipStream_t side = ctx->side[i & 1];
-        HIP_TRY(ctx, hipStreamWaitEvent(side, b.prop_done, 0));
-        if (i + 1 < n) {
-            for (size_t j = 0; j < i; j++)  // earlier batches' masks (its clip masks)
-                HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[j]->fill_done, 0));
-            pfh::launch_fill(a, side, p);
-            HIP_TRY(ctx, hipEventRecord(b.fill_done, side));
-        }
         // the per-row path lists and the sort run beside the last batch's
         // fill, not beside the propagate and fill lists it waits for
         HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[n - 1]->prop_done, 0));

@@ -138,3 +138,25 @@ def render(dr, points, colors):
     """Differentiable image (H, W, 4) of the paths: points (N, 2) and colors
     (P, 4) RGBA in [0, 1] are the leaves."""
     return _Render.apply(points, colors, dr)
+
+
+def fit(dr, points, colors, target, steps, lr_points=0.2, lr_colors=0.005, callback=None):
+    """Inverse rendering: Adam on the points (lr in pixels) and colours
+    towards `target` (H, W, 4) under the mean squared error. points and
+    colors are leaf tensors on the device (requires_grad); returns the loss
+    before each step."""
+    opt = torch.optim.Adam([{"params": [points], "lr": lr_points},
+                            {"params": [colors], "lr": lr_colors}])
+    losses = []
+    for _ in range(steps):
+        opt.zero_grad()
+        img = render(dr, points, colors)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            colors.clamp_(0.0, 1.0)
+        losses.append(loss.item())
+        if callback:
+            callback(len(losses))
+    return losses

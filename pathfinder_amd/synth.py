@@ -384,3 +384,98 @@ def tiny_scenes():
     add(s, [_contour([(-20, -10), (45, 5), (30, 60)], [0, 0, 0])], (10, 10, 10, 255))
     out["offscreen"] = (s, (50, 40), None)
     return out
+
+
+def _zigzag(x_lo, x_hi, y0, y1, n):
+    """n lines zig-zagging between x_lo and x_hi while y climbs from y0 to y1."""
+    ys = np.linspace(y0, y1, n + 1)
+    xs = np.where(np.arange(n + 1) % 2 == 0, x_lo, x_hi)
+    return np.stack([xs, ys], 1)
+
+
+def edge_scenes():
+    """Scenes at the limits the reference documents, one per limit:
+    name -> (scene, (w, h), background, transform or None).
+
+      dense_tile     tiles crossed by more than 1024 fills (fill_compute.inc.glsl:23
+                     stops after 1024), a path of ~5000 fills whose tiles' fills
+                     are spread over several 1024-fill chunks, and exactly 1024
+                     and 1025 fills in a tile
+      backdrop_wrap  200 and -200 net crossings per tile column, past the int8
+                     backdrop delta of bin.cs.glsl:140-141
+      huge_coords    points beyond +-32768 px (dice.cs.glsl:80 clamps microlines)
+      degenerate     zero-length lines, a cubic whose points coincide, a
+                     quadratic with its control point on an end, empty and
+                     one-point contours, and a path with no contour at all
+      mirrored       a 64-path small scene under a negative-determinant transform
+    """
+    from .scene import ColorU
+    out = {}
+
+    def make(w, h):
+        s = Scene()
+        s.set_view_box(RectF(0, 0, w, h))
+        return s
+
+    def add(scene, contours, color, rule=L.FILL_RULE_WINDING):
+        contours = [(np.asarray(p, np.float32).reshape(-1, 2),
+                     np.asarray(f, np.uint8).reshape(-1)) for p, f in contours]
+        pts = np.concatenate([c[0] for c in contours])
+        fl = np.concatenate([c[1] for c in contours])
+        ends = np.cumsum([len(c[0]) for c in contours]).astype(np.uint32)
+        pid = scene.push_paint(ColorU(*color))
+        return scene.push_draw_paths(PathArrays(pts, fl, ends, [len(ends)], [pid], [rule], [-1]))
+
+    def lines(pts):
+        pts = np.asarray(pts, np.float32)
+        return pts, np.zeros(len(pts), np.uint8)
+
+    s = make(96, 64)
+    # ~5000 fills: tile (1, 1), then tile (2, 1), then tile (1, 1) again, so
+    # tile (1, 1) has more than 1024 fills in separate chunks of the path
+    zz = np.concatenate([_zigzag(17.3, 30.7, 17.0, 24.0, 2000),
+                         _zigzag(33.1, 46.9, 17.0, 30.0, 1000)[1:],
+                         _zigzag(17.6, 30.2, 24.5, 31.0, 2000)[1:]])
+    add(s, [lines(zz)], (200, 40, 40, 255))
+    # exactly 1024 and 1025 fills in tiles (4, 1) and (4, 2)
+    add(s, [lines(_zigzag(66.2, 78.8, 17.0, 31.0, 1024))], (40, 200, 40, 160))
+    add(s, [lines(_zigzag(65.7, 79.4, 33.0, 47.0, 1025))], (40, 40, 200, 200))
+    out["dense_tile"] = (s, (96, 64), ColorF(1, 1, 1, 1), None)
+
+    s = make(128, 96)
+    sq = [(20.5, 10.25), (75.3, 10.25), (75.3, 50.75), (20.5, 50.75)]
+    add(s, [lines(sq)] * 200, (90, 20, 140, 255))
+    add(s, [lines(sq[::-1] + np.float32([30.0, 40.0]))] * 200, (255, 128, 0, 120))
+    # 131 and -129 crossings: just past the int8 range on either side
+    add(s, [lines([(3.5, 70.5), (40.5, 70.5), (40.5, 90.5), (3.5, 90.5)])] * 131, (0, 90, 0, 255))
+    add(s, [lines([(100.5, 60.5), (100.5, 90.5), (60.5, 90.5), (60.5, 60.5)])] * 129,
+        (0, 0, 90, 200))
+    out["backdrop_wrap"] = (s, (128, 96), ColorF(1, 1, 1, 1), None)
+
+    s = make(128, 96)
+    add(s, [lines([(-50000.5, 10.25), (100.7, 40.1), (20.3, 90.9)])], (30, 30, 30, 255))
+    add(s, [(np.float32([(10.5, 60.5), (0.0, 70000.0), (90000.0, -45000.0), (120.25, 80.5)]),
+             np.uint8([0, L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0]))], (200, 20, 90, 170))
+    add(s, [lines([(64.0, 5.0), (40000.0, 48.0), (64.0, 90.0), (-33000.25, 48.0)])],
+        (20, 120, 220, 140), rule=L.FILL_RULE_EVEN_ODD)
+    add(s, [lines([(40000.0, 40000.0), (40100.0, 40000.0), (40050.0, 40100.0)])], (0, 0, 0, 255))
+    out["huge_coords"] = (s, (128, 96), ColorF(1, 1, 1, 1), None)
+
+    s = make(64, 64)
+    add(s, [lines([(10, 10), (10, 10), (40.5, 12.25), (40.5, 12.25), (30.75, 40.5), (10, 10)])],
+        (150, 60, 20, 255))
+    add(s, [(np.float32([(50, 50), (50, 50), (50, 50), (50, 50)]),
+             np.uint8([0, L.POINT_CONTROL_0, L.POINT_CONTROL_1, 0])),
+            (np.float32([(5.5, 40.5), (5.5, 40.5), (25.25, 60.75)]),
+             np.uint8([0, L.POINT_CONTROL_0, 0])),
+            (np.float32([(44.5, 20.5), (60.25, 30.5), (60.25, 30.5)]),
+             np.uint8([0, 0, L.POINT_CONTROL_0])),
+            (np.zeros((0, 2), np.float32), np.zeros(0, np.uint8)),
+            (np.float32([(33.3, 33.3)]), np.uint8([0]))], (20, 150, 200, 200))
+    add(s, [(np.zeros((0, 2), np.float32), np.zeros(0, np.uint8))], (0, 0, 0, 255))
+    add(s, [lines([(2.5, 2.5), (60.5, 3.5), (2.5, 3.5)])], (60, 60, 60, 255))
+    out["degenerate"] = (s, (64, 64), ColorF(1, 1, 1, 1), None)
+
+    scene, size, bg = small_scene(paths=64, size=512)
+    out["mirrored"] = (scene, size, bg, (-1.0, 0.0, 0.0, 1.0, 512.0, 0.0))
+    return out

@@ -42,6 +42,12 @@ def test_packing_tiny(name):
     _compare(scene)
 
 
+@pytest.mark.parametrize("name", sorted(synth.edge_scenes()))
+def test_packing_edge_scenes(name):
+    scene, _, _, xf = synth.edge_scenes()[name]
+    _compare(scene, xf)
+
+
 def test_packing_small_and_transformed():
     scene, _, _ = synth.small_scene()
     _compare(scene)
