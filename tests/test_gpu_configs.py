@@ -64,38 +64,40 @@ def _render(scene, w, h, bg):
         r.close()
 
 
-def _alone(item, bg, phase):
-    px, py = phase
+def _alone(item, bg, cell):
+    """The item alone at its cell's position (points moved by the same origin,
+    so the same float32 values as in the packed frame), cropped to the cell."""
+    x, y, w, h = cell
     s = Scene()
-    s.set_view_box(RectF(0, 0, item.width + px, item.height + py))
+    s.set_view_box(RectF(0, 0, x + w, y + h))
     a = item.arrays
     paints = np.asarray([s.push_paint(ColorU(*c)) for c in item.colors], np.uint32)
-    moved = np.asarray(a.points, np.float32) + np.asarray([px, py], np.float32)
+    moved = np.asarray(a.points, np.float32) + np.asarray([x, y], np.float32)
     s.push_draw_paths(PathArrays(moved, a.flags, a.contour_ends, a.path_contour_ends, paints,
                                  a.fill_rules))
-    out = _render(s, item.width + px, item.height + py, bg)
-    return out[py:py + item.height, px:px + item.width]
+    return batch.crop(_render(s, x + w, y + h, bg), cell)
 
 
 @pytest.mark.parametrize("kind", ["glyphs", "canvases"])
 def test_every_frame_matches_items_alone(kind):
     """Every frame of the workload, with sampled items (8 per frame) rendered
-    alone at the same tile phase: within 1/255, >= 99.9 % identical (moving
-    points by the cell origin changes their float32 rounding)."""
+    alone at their own position: the crops must be identical (items do not
+    interfere). An item moved to another position is not compared: the new
+    float32 rounding of its points can flip dice's segment count of a curve
+    (a 0.25 px flattening difference, up to 65/255 on a pixel)."""
     items, fr = frames(kind)
     bg = (GLYPHS if kind == "glyphs" else CANVASES)["bg"]
     rng = np.random.default_rng(5)
-    diffs = []
+    compared = 0
     for k, f in enumerate(fr):
         _say("case", kind, "frame", k, "packed vs alone")
         px = _render(f.scene, f.width, f.height, bg)
         cells = sorted(f.cells)
         for i in rng.choice(cells, min(8, len(cells)), replace=False):
             cell = f.cells[int(i)]
-            d = np.abs(batch.crop(px, cell) -
-                       _alone(items[int(i)], bg, (cell[0] % 16, cell[1] % 16)))
-            diffs.append(d.ravel())
-    d = np.concatenate(diffs)
-    _say("max %d, identical %.5f" % (d.max(), np.mean(d == 0)))
-    assert d.max() <= 1
-    assert np.mean(d == 0) >= 0.999
+            got = batch.crop(px, cell)
+            want = _alone(items[int(i)], bg, cell)
+            assert np.array_equal(got, want), (kind, k, int(i), cell,
+                                               int(np.abs(got - want).max()))
+            compared += got.size
+    _say("compared", compared, "values")
