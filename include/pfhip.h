@@ -329,6 +329,22 @@ pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst);
 pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_points,
                         uint32_t point_count, float *grad_colors, uint32_t path_count);
 
+/* No counterpart in the reference: pfh_backward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight inverse-
+ * rendering loops, harnesses that forbid hipMalloc). Same arguments, checks
+ * and results as pfh_backward; it allocates nothing: its scratch (list
+ * offsets and their scan, per-slot transmittances, dL/dC per alpha tile and
+ * its stamps) lives in one static device array of PFH_NOALLOC_SCRATCH_BYTES
+ * per device, which each device holds until the process ends. Frames whose
+ * scratch would not fit (framebuffer tiles, composited list slots and alpha
+ * tiles; e.g. 512 x 512 with ~40k alpha tiles) return PFH_ERROR_LIMIT. Every
+ * call on a device shares that device's array, so calls that overlap on one
+ * GPU (two threads, two contexts) race; the array keeps the last call's
+ * contents, and each call clears what must start at zero. */
+#define PFH_NOALLOC_SCRATCH_BYTES (64u << 20)
+pfh_status pfh_backward_noalloc(pfh_context *ctx, const float *grad_image, float *grad_points,
+                                uint32_t point_count, float *grad_colors, uint32_t path_count);
+
 /* ------------------------------------------------------------ scene front-end */
 
 typedef struct pfh_scene pfh_scene;

@@ -49,6 +49,8 @@ DEBUG_BACKDROPS = 6
 DEBUG_ALPHA_RANGE = 7
 DEBUG_COVERAGE = 8
 
+NOALLOC_SCRATCH_BYTES = 64 << 20  # PFH_NOALLOC_SCRATCH_BYTES
+
 
 class Vector2F(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float)]
@@ -165,6 +167,7 @@ SIGNATURES = {
     "pfh_autotune": (I32, [P, U32, C.POINTER(LaunchShape)]),
     "pfh_framebuffer_to_float": (I32, [P, P]),
     "pfh_backward": (I32, [P, P, P, U32, P, U32]),
+    "pfh_backward_noalloc": (I32, [P, P, P, U32, P, U32]),
     "pfh_scene_create": (P, []),
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),

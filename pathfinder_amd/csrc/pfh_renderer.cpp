@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -775,6 +776,143 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     return run_batch(ctx, b, front);
 }
 
+// ---------------------------------------------------------------- adjoint
+
+// Device buffers of one backward pass beyond its inputs and outputs.
+struct BackwardScratch {
+    uint32_t *list_off = nullptr;    // list slots per framebuffer tile (+ total), fbn + 1
+    uint32_t *list_scan = nullptr;   // scan partials of list_off
+    float *trans = nullptr;          // 256 floats per list slot
+    uint32_t *list_tiles = nullptr;  // the tile of every list slot
+    float *grad_cov = nullptr;       // dL/dC, 256 floats per alpha tile
+    uint32_t *grad_stamp = nullptr;  // per alpha tile: the epoch whose grad_cov it holds
+    uint32_t epoch = 0;
+};
+
+// The checks both backward entry points share; returns the frame's draw batch.
+pfh_status backward_check(pfh_context *ctx, const float *grad_image, const float *grad_points,
+                          uint32_t point_count, const float *grad_colors, Batch **out) {
+    if (!ctx || !grad_image || !grad_points || !grad_colors)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "backward inside a scene");
+    if (!ctx->gradients || !ctx->d_coverage)
+        return fail(ctx, PFH_ERROR_STATE, "the frame was rendered without pfh_set_gradients");
+    Batch *draw = nullptr;
+    for (auto &b : ctx->batches) {
+        if (!b->draw) continue;
+        if (draw) return fail(ctx, PFH_ERROR_STATE, "backward supports one draw batch per frame");
+        draw = b.get();
+    }
+    if (!draw) return fail(ctx, PFH_ERROR_STATE, "no draw batch in the frame");
+    if (draw->load)
+        return fail(ctx, PFH_ERROR_STATE, "backward needs a frame drawn over a clear colour");
+    if (point_count < ctx->draw_points_host.size())
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "grad_points holds %u points, the scene %zu",
+                    point_count, ctx->draw_points_host.size());
+    if (!draw->d_tile_paths)
+        return fail(ctx, PFH_ERROR_STATE, "the draw batch was prepared before pfh_set_gradients");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    *out = draw;
+    return PFH_OK;
+}
+
+uint32_t backward_alpha_end(pfh_context *ctx) {
+    uint32_t alpha_end = 0;
+    for (auto &x : ctx->batches) alpha_end = std::max(alpha_end, x->counters.frame_alpha_end);
+    return std::min(alpha_end, ctx->cap_a);
+}
+
+// Exclusive scan of the sorted lists' lengths into sc.list_off; returns the total.
+pfh_status backward_list_slots(pfh_context *ctx, Batch &b, const BackwardScratch &sc,
+                               uint32_t *slots) {
+    hipStream_t s = ctx->stream;
+    uint32_t fbn = (uint32_t)(ctx->fbw * ctx->fbh);
+    pfh::launch_list_lengths(b.d_list_stats, sc.list_off, fbn, s);
+    pfh::launch_scan(sc.list_off, fbn, sc.list_scan, s);
+    HIP_TRY(ctx, hipMemcpyAsync(slots, sc.list_off + fbn, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return PFH_OK;
+}
+
+pfh_status backward_launch(pfh_context *ctx, Batch &b, const float *grad_image, float *grad_points,
+                           float *grad_colors, uint32_t path_count, const BackwardScratch &sc,
+                           uint32_t slots, uint32_t alpha_end) {
+    hipStream_t s = ctx->stream;
+    pfh::TileGradArgs tg;
+    tg.f = tile_args(ctx, b);
+    tg.coverage = ctx->d_coverage;
+    tg.alpha_tiles = b.d_alpha;
+    tg.alpha_start = b.counters.alpha_start;
+    tg.alpha_count = b.counters.alpha;
+    tg.grad_image = grad_image;
+    tg.width = (uint32_t)ctx->width;
+    tg.height = (uint32_t)ctx->height;
+    tg.list_offsets = sc.list_off;
+    tg.slot_capacity = slots;
+    tg.trans = sc.trans;
+    tg.list_tiles = sc.list_tiles;
+    tg.grad_stamp = sc.grad_stamp;
+    tg.epoch = sc.epoch;
+    tg.tile_paths = b.d_tile_paths;
+    tg.dice_md = (const uint4 *)b.d_dice;
+    tg.path_count = b.host.path_count;
+    tg.grad_cov = sc.grad_cov;
+    tg.grad_colors = grad_colors;
+    tg.color_count = path_count;
+    pfh::launch_tile_backward(tg, s);
+    pfh::MicrolineGradArgs mg;
+    mg.b = batch_args(ctx, b);
+    mg.microline_count = b.counters.microline_count;
+    mg.grad_cov = sc.grad_cov;
+    mg.grad_stamp = sc.grad_stamp;
+    mg.epoch = sc.epoch;
+    mg.alpha_capacity = alpha_end;
+    mg.grad_points = grad_points;
+    pfh::launch_microline_backward(mg, s);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return PFH_OK;
+}
+
+// ------------------------------------------- scratch of the allocation-free entry point
+
+// pfh_backward_noalloc's device buffers: one array per device, shared by every
+// call on that device and kept until the process ends. Host code never uses
+// the array's name as an address: each device's address comes from
+// hipGetSymbolAddress (scratch_setup, at pfh_context_create).
+constexpr size_t kNoAllocScratchBytes = (size_t)PFH_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_known[kMaxDevices] = {};
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Once per device, with that device current.
+void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_known[device]) return;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? static_cast<uint8_t *>(p) : nullptr;
+    g_scratch_known[device] = true;
+}
+
+uint8_t *scratch_pointer(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+hipError_t scratch_error(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_known[device] ? g_scratch_err[device] : hipErrorNotInitialized;
+}
+
 }  // namespace
 
 extern "C" {
@@ -799,6 +937,7 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, opt.device) != hipSuccess) return PFH_ERROR_NO_DEVICE;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PFH_ERROR_NO_DEVICE;
+    scratch_setup(opt.device);  // pfh_backward_noalloc's scratch address on this device
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return PFH_ERROR_HIP;
     for (hipStream_t &side : ctx->side)
@@ -1216,29 +1355,11 @@ pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst) {
 
 pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_points,
                         uint32_t point_count, float *grad_colors, uint32_t path_count) {
-    if (!ctx || !grad_image || !grad_points || !grad_colors)
-        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
-    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "backward inside a scene");
-    if (!ctx->gradients || !ctx->d_coverage)
-        return fail(ctx, PFH_ERROR_STATE, "the frame was rendered without pfh_set_gradients");
     Batch *draw = nullptr;
-    for (auto &b : ctx->batches) {
-        if (!b->draw) continue;
-        if (draw) return fail(ctx, PFH_ERROR_STATE, "backward supports one draw batch per frame");
-        draw = b.get();
-    }
-    if (!draw) return fail(ctx, PFH_ERROR_STATE, "no draw batch in the frame");
-    if (draw->load) return fail(ctx, PFH_ERROR_STATE, "backward needs a frame drawn over a clear colour");
-    if (point_count < ctx->draw_points_host.size())
-        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "grad_points holds %u points, the scene %zu",
-                    point_count, ctx->draw_points_host.size());
-
-    if (!draw->d_tile_paths)
-        return fail(ctx, PFH_ERROR_STATE, "the draw batch was prepared before pfh_set_gradients");
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    pfh_status st = backward_check(ctx, grad_image, grad_points, point_count, grad_colors, &draw);
+    if (st != PFH_OK) return st;
     Batch &b = *draw;
     hipStream_t s = ctx->stream;
-    pfh_status st;
     // list slots per framebuffer tile: exclusive scan of the sorted lists' lengths
     uint32_t fbn = (uint32_t)(ctx->fbw * ctx->fbh);
     if (ctx->list_off_cap < (size_t)fbn + 1) {
@@ -1251,11 +1372,11 @@ pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_p
             return st;
         ctx->list_off_cap = (size_t)fbn + 1;
     }
-    pfh::launch_list_lengths(b.d_list_stats, ctx->d_list_off, fbn, s);
-    pfh::launch_scan(ctx->d_list_off, fbn, ctx->d_list_scan, s);
+    BackwardScratch sc;
+    sc.list_off = ctx->d_list_off;
+    sc.list_scan = ctx->d_list_scan;
     uint32_t slots = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&slots, ctx->d_list_off + fbn, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if ((st = backward_list_slots(ctx, b, sc, &slots)) != PFH_OK) return st;
     if (ctx->trans_cap < (size_t)slots * 256 || !ctx->d_trans) {
         dfree(ctx, ctx->d_trans);
         dfree(ctx, ctx->d_list_tiles);
@@ -1267,9 +1388,7 @@ pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_p
         if ((st = dalloc(ctx, &ctx->d_list_tiles, n)) != PFH_OK) return st;
         ctx->trans_cap = n * 256;
     }
-    uint32_t alpha_end = 0;
-    for (auto &x : ctx->batches) alpha_end = std::max(alpha_end, x->counters.frame_alpha_end);
-    alpha_end = std::min(alpha_end, ctx->cap_a);
+    uint32_t alpha_end = backward_alpha_end(ctx);
     if (ctx->grad_cov_cap < (size_t)alpha_end * 256 || !ctx->d_grad_cov) {
         dfree(ctx, ctx->d_grad_cov);
         dfree(ctx, ctx->d_grad_stamp);
@@ -1287,40 +1406,66 @@ pfh_status pfh_backward(pfh_context *ctx, const float *grad_image, float *grad_p
         pfh::launch_fill_zero((float *)ctx->d_grad_stamp, ctx->grad_cov_cap / 256, s);
         ctx->grad_epoch = 1;
     }
-    pfh::TileGradArgs tg;
-    tg.f = tile_args(ctx, b);
-    tg.coverage = ctx->d_coverage;
-    tg.alpha_tiles = b.d_alpha;
-    tg.alpha_start = b.counters.alpha_start;
-    tg.alpha_count = b.counters.alpha;
-    tg.grad_image = grad_image;
-    tg.width = (uint32_t)ctx->width;
-    tg.height = (uint32_t)ctx->height;
-    tg.list_offsets = ctx->d_list_off;
-    tg.slot_capacity = slots;
-    tg.trans = ctx->d_trans;
-    tg.list_tiles = ctx->d_list_tiles;
-    tg.grad_stamp = ctx->d_grad_stamp;
-    tg.epoch = ctx->grad_epoch;
-    tg.tile_paths = b.d_tile_paths;
-    tg.dice_md = (const uint4 *)b.d_dice;
-    tg.path_count = b.host.path_count;
-    tg.grad_cov = ctx->d_grad_cov;
-    tg.grad_colors = grad_colors;
-    tg.color_count = path_count;
-    pfh::launch_tile_backward(tg, s);
-    pfh::MicrolineGradArgs mg;
-    mg.b = batch_args(ctx, b);
-    mg.microline_count = b.counters.microline_count;
-    mg.grad_cov = ctx->d_grad_cov;
-    mg.grad_stamp = ctx->d_grad_stamp;
-    mg.epoch = ctx->grad_epoch;
-    mg.alpha_capacity = alpha_end;
-    mg.grad_points = grad_points;
-    pfh::launch_microline_backward(mg, s);
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipStreamSynchronize(s));
-    return PFH_OK;
+    sc.trans = ctx->d_trans;
+    sc.list_tiles = ctx->d_list_tiles;
+    sc.grad_cov = ctx->d_grad_cov;
+    sc.grad_stamp = ctx->d_grad_stamp;
+    sc.epoch = ctx->grad_epoch;
+    return backward_launch(ctx, b, grad_image, grad_points, grad_colors, path_count, sc, slots,
+                           alpha_end);
+}
+
+pfh_status pfh_backward_noalloc(pfh_context *ctx, const float *grad_image, float *grad_points,
+                                uint32_t point_count, float *grad_colors, uint32_t path_count) {
+    Batch *draw = nullptr;
+    pfh_status st = backward_check(ctx, grad_image, grad_points, point_count, grad_colors, &draw);
+    if (st != PFH_OK) return st;
+    uint8_t *base = scratch_pointer(ctx->device);
+    if (!base)
+        return fail(ctx, PFH_ERROR_HIP, "the scratch array's device address is unknown (%s)",
+                    hipGetErrorString(scratch_error(ctx->device)));
+    Batch &b = *draw;
+    uint32_t fbn = (uint32_t)(ctx->fbw * ctx->fbh);
+    uint32_t alpha_end = backward_alpha_end(ctx);
+    // the scratch array, carved in 256-byte aligned regions
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    size_t at_off = carve(((size_t)fbn + 1) * 4);
+    size_t at_scan = carve(pfh::scan_partials_words(fbn) * 4);
+    size_t at_cov = carve((size_t)std::max<uint32_t>(alpha_end, 1) * 1024);
+    size_t at_stamp = carve((size_t)std::max<uint32_t>(alpha_end, 1) * 4);
+    if (off > kNoAllocScratchBytes)
+        return fail(ctx, PFH_ERROR_LIMIT,
+                    "backward_noalloc: %u framebuffer tiles and %u alpha tiles need %zu bytes "
+                    "of scratch before the list slots; the limit is %zu",
+                    fbn, alpha_end, off, kNoAllocScratchBytes);
+    BackwardScratch sc;
+    sc.list_off = reinterpret_cast<uint32_t *>(base + at_off);
+    sc.list_scan = reinterpret_cast<uint32_t *>(base + at_scan);
+    uint32_t slots = 0;
+    if ((st = backward_list_slots(ctx, b, sc, &slots)) != PFH_OK) return st;
+    size_t at_trans = carve((size_t)std::max<uint32_t>(slots, 1) * 1024);
+    size_t at_tiles = carve((size_t)std::max<uint32_t>(slots, 1) * 4);
+    if (off > kNoAllocScratchBytes)
+        return fail(ctx, PFH_ERROR_LIMIT,
+                    "backward_noalloc: %u framebuffer tiles, %u list slots and %u alpha tiles "
+                    "need %zu bytes of scratch; the limit is %zu",
+                    fbn, slots, alpha_end, off, kNoAllocScratchBytes);
+    sc.trans = reinterpret_cast<float *>(base + at_trans);
+    sc.list_tiles = reinterpret_cast<uint32_t *>(base + at_tiles);
+    sc.grad_cov = reinterpret_cast<float *>(base + at_cov);
+    sc.grad_stamp = reinterpret_cast<uint32_t *>(base + at_stamp);
+    // the array keeps the last call's contents (any context on this device):
+    // every stamp starts at zero, and this pass's epoch is 1
+    pfh::launch_fill_zero(reinterpret_cast<float *>(sc.grad_stamp),
+                          std::max<uint32_t>(alpha_end, 1), ctx->stream);
+    sc.epoch = 1;
+    return backward_launch(ctx, b, grad_image, grad_points, grad_colors, path_count, sc, slots,
+                           alpha_end);
 }
 
 pfh_status pfh_get_stats(pfh_context *ctx, pfh_render_stats *stats) {

@@ -588,6 +588,15 @@ class Renderer:
                                            C.c_void_p(grad_points_ptr), point_count,
                                            C.c_void_p(grad_colors_ptr), path_count))
 
+    def backward_noalloc(self, grad_image_ptr, grad_points_ptr, point_count, grad_colors_ptr,
+                         path_count):
+        """pfh_backward_noalloc: pfh_backward with its scratch in the library's
+        static per-device array (allocates nothing; PFH_ERROR_LIMIT when the
+        frame's scratch does not fit)."""
+        self._check(self._lib.pfh_backward_noalloc(self._h, C.c_void_p(grad_image_ptr),
+                                                   C.c_void_p(grad_points_ptr), point_count,
+                                                   C.c_void_p(grad_colors_ptr), path_count))
+
     def read_pixels(self):
         d = self.options.dest
         out = np.zeros((d.height, d.width, 4), np.uint8)
