@@ -65,14 +65,19 @@ struct BatchArgs {
     uint4 *microlines;             // microline_capacity
     uint32_t microline_capacity;
     uint32_t *ml_fill_offsets;     // microline_capacity + 1
-    uint32_t *fills;               // 3 words per fill, fill_capacity
+    uint2 *fill_words;             // Fill words 0-1 (the 8.8 ends), fill_capacity
+    uint32_t *fill_links;          // Fill word 2 (the list link), fill_capacity
     uint32_t *fill_tiles;          // tile index per fill
     uint32_t *path_fills;          // path_count + 1: first fill of each path
     uint32_t fill_capacity;
     uint2 *sorted_fills;           // fill words 0-1 sorted by (tile, fill index), fill_capacity
     uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)
     uint2 *run_prev;               // at a run's first sorted index: the previous run or (0, 0)
-    uint4 *tiles;                  // tile_count
+    uint4 *tiles;                  // tile_count; words 0 and 1 are not kept here
+                                   // (tile_next, tile_first)
+    uint32_t *tile_next;           // tile word 0 of linked tiles (k_sort_tiles; propagate for
+                                   // linked tiles outside the framebuffer)
+    uint32_t *tile_first;          // tile word 1 of tiles with fills (k_fill_link)
     uint8_t *tile_flags;           // tile_count: nonzero when bin gave the tile a fill (2 once
                                    // k_fill_link has written its head)
     uint32_t *tile_delta;          // tile_count: bin's backdrop deltas << 24 (tile word 2's top byte)
@@ -106,6 +111,7 @@ struct BatchArgs {
 
 struct TileArgs {
     const uint4 *tiles;
+    const uint32_t *tile_next;     // tile word 0 (the sorted list link)
     uint32_t tile_count;
     uint32_t mask_capacity;        // alpha tiles held by `mask`
     const int32_t *first_tile_map;

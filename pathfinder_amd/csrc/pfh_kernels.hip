@@ -683,8 +683,7 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
             if (EMIT) {
                 uint32_t f = fill_base + count;
                 if (f < a.fill_capacity) {
-                    a.fills[(size_t)f * 3 + 0] = fw.x;
-                    a.fills[(size_t)f * 3 + 1] = fw.y;
+                    a.fill_words[f] = fw;
                     a.fill_tiles[f] = tile;
                     a.tile_flags[tile] = 1;
                 }
@@ -704,8 +703,7 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
             if (EMIT) {
                 uint32_t f = fill_base + count;
                 if (f < a.fill_capacity) {
-                    a.fills[(size_t)f * 3 + 0] = fw.x;
-                    a.fills[(size_t)f * 3 + 1] = fw.y;
+                    a.fill_words[f] = fw;
                     a.fill_tiles[f] = tile;
                     a.tile_flags[tile] = 1;
                 }
@@ -905,10 +903,10 @@ __device__ __forceinline__ void link_finish(const BatchArgs &a, LinkLds &L, cons
             // flag 2: an earlier chunk of this path gave the tile a head
             // (bin set the flag to 1; tiles are not initialised otherwise)
             if (!first_chunk && a.tile_flags[tile] == 2) {
-                link = a.tiles[tile].y;
+                link = a.tile_first[tile];
                 prev = a.tile_runs[tile];
             }
-            a.tiles[tile].y = base + end - 1;
+            a.tile_first[tile] = base + end - 1;
             a.tile_flags[tile] = 2;
             uint32_t slot = L.out[p];
             uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
@@ -920,7 +918,7 @@ __device__ __forceinline__ void link_finish(const BatchArgs &a, LinkLds &L, cons
         } else {
             link = base + L.run_first[K::run(sk[p - 1]) + 1] - 1;
         }
-        a.fills[(size_t)(base + first) * 3 + 2] = link;
+        a.fill_links[base + first] = link;
     }
     // Every fill to its slot; a fill inside a run links to the one before.
 #pragma unroll
@@ -929,7 +927,7 @@ __device__ __forceinline__ void link_finish(const BatchArgs &a, LinkLds &L, cons
         if (i < n) {
             uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
             a.sorted_fills[base + slot] = fw[k];
-            if (!(start_bits >> k & 1u)) a.fills[(size_t)(base + i) * 3 + 2] = base + i - 1;
+            if (!(start_bits >> k & 1u)) a.fill_links[base + i] = base + i - 1;
         }
     }
     __syncthreads();
@@ -957,8 +955,7 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
             uint32_t i = min(k * kBlock + tid, n - 1);
             tl[k] = a.fill_tiles[base + i];
             tp[k] = a.fill_tiles[base + (i > 0 ? i - 1 : 0)];
-            const uint32_t *w = a.fills + (size_t)(base + i) * 3;
-            fw[k] = make_uint2(w[0], w[1]);
+            fw[k] = a.fill_words[base + i];
         }
         // Runs: a run starts where the tile changes. Its index is the number
         // of starts before it, counted by ballots per (element row, wave).
@@ -1190,8 +1187,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
         uint32_t i = min(k * 64u + lane, n - 1);
         tl[k] = a.fill_tiles[f0 + i];
         tp[k] = a.fill_tiles[f0 + (i > 0 ? i - 1 : 0)];
-        const uint32_t *w = a.fills + (size_t)(f0 + i) * 3;
-        fw[k] = make_uint2(w[0], w[1]);
+        fw[k] = a.fill_words[f0 + i];
     }
     uint32_t runs = 0, rid[kQ], start_bits = 0;
 #pragma unroll
@@ -1292,14 +1288,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
             uint32_t end = L.run_first[rq + 1];
             uint32_t tile = tile_base + lt;
             link = 0xffffffffu;
-            a.tiles[tile].y = f0 + end - 1;
+            a.tile_first[tile] = f0 + end - 1;
             uint32_t slot = L.out[p];
             uint32_t count = (uint32_t)L.out[lo] + (end - L.run_first[rq]) - slot;
             a.tile_runs[tile] = make_uint2(f0 + slot, count);
         } else {
             link = f0 + L.run_first[(sk[p - 1] & 0xffu) + 1] - 1;
         }
-        a.fills[(size_t)(f0 + first) * 3 + 2] = link;
+        a.fill_links[f0 + first] = link;
     }
     // every fill to its slot; a fill inside a run links to the one before
 #pragma unroll
@@ -1308,7 +1304,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
         if (i < n) {
             uint32_t slot = (uint32_t)L.out[L.rank[rid[k]]] + (i - L.run_first[rid[k]]);
             a.sorted_fills[f0 + slot] = fw[k];
-            if (!(start_bits >> k & 1u)) a.fills[(size_t)(f0 + i) * 3 + 2] = f0 + i - 1;
+            if (!(start_bits >> k & 1u)) a.fill_links[f0 + i] = f0 + i - 1;
         }
     }
 }
@@ -1448,12 +1444,15 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
                               (tile_backdrop != 0 && draw_alpha < 0 && zwrite ? kTileOccluder
                                                                                : 0) |
                               (draw_alpha >= 0 ? kTileAlpha : 0));
-                if (have_mask) {
-                    a.tiles[t].x = 0xffffffffu;
-                    *reinterpret_cast<uint2 *>(&a.tiles[t].z) = make_uint2(z, w);
-                } else {
-                    a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, z, w);
-                }
+                // words 0 and 1 live in tile_next / tile_first; the tile's
+                // 16 bytes are written whole (no partial-sector writes)
+                a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, z, w);
+                // k_sort_tiles links the linked tiles inside the framebuffer
+                // grid; the others keep an empty link
+                int32_t gx = (int32_t)(rx0 + (uint32_t)tile_x), gy = (int32_t)(ry0 + y);
+                if ((tile_backdrop != 0 || draw_alpha >= 0) &&
+                    (gx < 0 || gy < 0 || gx >= a.fb_width_tiles || gy >= a.fb_height_tiles))
+                    a.tile_next[t] = 0xffffffffu;
             }
             backdrop += delta;
             dsum += delta;
@@ -1684,7 +1683,7 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
             // the pending culled tile's next is this chunk's highest linked tile
             int32_t hi_lane = 63 - __clzll((long long)lmask);
             int32_t hi_t = __shfl(t, hi_lane, 64);
-            if (pending >= 0 && lane == 0) a.tiles[pending].x = (uint32_t)hi_t;
+            if (pending >= 0 && lane == 0) a.tile_next[pending] = (uint32_t)hi_t;
             if (pending >= 0) pending = -1;
             int32_t lo_lane = __ffsll((long long)lmask) - 1;
             int32_t lo_t = __shfl(t, lo_lane, 64);
@@ -1696,9 +1695,9 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
                 int32_t up_t = __shfl(t, up_lane & 63, 64);
                 int32_t down_t = __shfl(t, down_lane & 63, 64);
                 if ((int32_t)lane >= zlane) {
-                    a.tiles[t].x = (uint32_t)(up_lane < 64 ? up_t : succ);
+                    a.tile_next[t] = (uint32_t)(up_lane < 64 ? up_t : succ);
                 } else if (down_lane >= 0) {
-                    a.tiles[t].x = (uint32_t)down_t;
+                    a.tile_next[t] = (uint32_t)down_t;
                 }
             } else {
                 // keep the shuffles uniform across the wave
@@ -1717,7 +1716,7 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
         end = begin;
     }
     if (lane == 0 && active) {
-        if (pending >= 0) a.tiles[pending].x = 0xffffffffu;
+        if (pending >= 0) a.tile_next[pending] = 0xffffffffu;
         a.first_tile_map[wave] = lowest_survivor;
         a.zbuffer[8 + wave] = z_found ? z : 0;
         a.list_stats[wave] = make_uint4(n_linked, n_survivors, n_survivor_alpha, 0);
@@ -2290,11 +2289,17 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
     // bounds: the loop runs only when tile_count > 0)
     bool valid = t >= 0 && (uint32_t)t < a.tile_count;
     uint4 w = make_uint4(0, 0, 0, 0);
-    if (valid) w = a.tiles[t];
+    uint32_t w_next = 0xffffffffu;
+    if (valid) {
+        w = a.tiles[t];
+        w_next = a.tile_next[t];
+    }
     for (uint32_t guard = 0; valid && guard < a.tile_count; guard++) {
-        int32_t tn = (int32_t)w.x;
+        int32_t tn = (int32_t)w_next;
         bool next_valid = tn >= 0 && (uint32_t)tn < a.tile_count;
-        uint4 wn = a.tiles[next_valid ? (uint32_t)tn : 0u];
+        uint32_t tc = next_valid ? (uint32_t)tn : 0u;
+        uint4 wn = a.tiles[tc];
+        uint32_t wn_next = a.tile_next[tc];
         int32_t alpha = sext24(w.z);
         uint32_t color = w.w & 0xffffu;
         int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
@@ -2331,6 +2336,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
             d[sub][3] = d[sub][3] * ia + al;
         }
         w = wn;
+        w_next = wn_next;
         valid = next_valid;
     }
 #pragma unroll
@@ -2433,7 +2439,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
             }
             ma[sub] = m;
         }
-        return (int32_t)w.x;
+        return (int32_t)a.tile_next[t];
     };
 
     // walk 1: each slot's tile (stored by every lane, so each lane later reads

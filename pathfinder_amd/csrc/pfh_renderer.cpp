@@ -193,13 +193,16 @@ struct Batch {
     uint32_t *d_seg_offsets = nullptr;
     uint4 *d_microlines = nullptr;
     uint32_t *d_ml_fill_offsets = nullptr;
-    uint32_t *d_fills = nullptr;
+    uint2 *d_fill_words = nullptr;     // Fill words 0-1
+    uint32_t *d_fill_links = nullptr;  // Fill word 2
     uint32_t *d_fill_tiles = nullptr;
     uint32_t *d_path_fills = nullptr;
     uint2 *d_sorted_fills = nullptr;
     uint2 *d_run_prev = nullptr;
     uint2 *d_tile_runs = nullptr;
     uint4 *d_tiles = nullptr;
+    uint32_t *d_tile_next = nullptr;   // tile word 0 (linked tiles)
+    uint32_t *d_tile_first = nullptr;  // tile word 1 (tiles with fills)
     uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
     uint8_t *d_tile_flags = nullptr;   // 1 per tile bin gave a fill
     uint8_t *d_tile_status = nullptr;  // per tile: linked / solid / alpha bits
@@ -320,11 +323,12 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_dice); dfree(ctx, b.d_tpi); dfree(ctx, b.d_prop);
     dfree(ctx, b.d_backdrops_init); dfree(ctx, b.d_backdrops); dfree(ctx, b.d_seg_offsets);
     dfree(ctx, b.d_seg_delta); dfree(ctx, b.d_seg_alpha);
-    dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fills);
-    dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
+    dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets); dfree(ctx, b.d_fill_words);
+    dfree(ctx, b.d_fill_links); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills); dfree(ctx, b.d_run_prev);
     dfree(ctx, b.d_path_fills);
     dfree(ctx, b.d_tile_runs);
-    dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_next); dfree(ctx, b.d_tile_first);
+    dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_tile_flags); dfree(ctx, b.d_tile_delta);
     dfree(ctx, b.d_tile_status);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
@@ -352,14 +356,16 @@ pfh_status upload(pfh_context *ctx, T **dst, const std::vector<T> &src,
 pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if (b.cap_m == ctx->cap_m && b.cap_f == ctx->cap_f && b.d_microlines) return PFH_OK;
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets);
-    dfree(ctx, b.d_fills); dfree(ctx, b.d_fill_tiles); dfree(ctx, b.d_sorted_fills);
+    dfree(ctx, b.d_fill_words); dfree(ctx, b.d_fill_links); dfree(ctx, b.d_fill_tiles);
+    dfree(ctx, b.d_sorted_fills);
     dfree(ctx, b.d_run_prev); dfree(ctx, b.d_scan);
     b.cap_m = ctx->cap_m;
     b.cap_f = ctx->cap_f;
     pfh_status s;
     if ((s = dalloc(ctx, &b.d_microlines, b.cap_m)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_ml_fill_offsets, (size_t)b.cap_m + 1)) != PFH_OK) return s;
-    if ((s = dalloc(ctx, &b.d_fills, (size_t)b.cap_f * 3)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_fill_words, b.cap_f)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_fill_links, b.cap_f)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_fill_tiles, b.cap_f)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_sorted_fills, b.cap_f)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_run_prev, b.cap_f)) != PFH_OK) return s;
@@ -439,7 +445,8 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.microlines = b.d_microlines;
     a.microline_capacity = b.cap_m;
     a.ml_fill_offsets = b.d_ml_fill_offsets;
-    a.fills = b.d_fills;
+    a.fill_words = b.d_fill_words;
+    a.fill_links = b.d_fill_links;
     a.fill_tiles = b.d_fill_tiles;
     a.fill_capacity = b.cap_f;
     a.sorted_fills = b.d_sorted_fills;
@@ -447,6 +454,8 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.run_prev = b.d_run_prev;
     a.path_fills = b.d_path_fills;
     a.tiles = b.d_tiles;
+    a.tile_next = b.d_tile_next;
+    a.tile_first = b.d_tile_first;
     a.tile_flags = b.d_tile_flags;
     a.tile_status = b.d_tile_status;
     a.tile_delta = b.d_tile_delta;
@@ -488,6 +497,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
 pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
     pfh::TileArgs t;
     t.tiles = b.d_tiles;
+    t.tile_next = b.d_tile_next;
     t.tile_count = b.host.tile_count;
     t.mask_capacity = ctx->cap_a;
     t.first_tile_map = b.d_ftm;
@@ -756,6 +766,8 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_seg_offsets, (size_t)h.segment_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_path_fills, (size_t)h.path_count + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tiles, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tile_next, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_tile_first, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_flags, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_status, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_delta, h.tile_count)) != PFH_OK) return s;
@@ -1287,8 +1299,8 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
     uint32_t range[2] = {b->counters.alpha_start, b->counters.alpha_start + b->counters.alpha};
     switch (kind) {
     case PFH_DEBUG_MICROLINES: src = b->d_microlines; bytes = (size_t)b->counters.microline_count * 16; break;
-    case PFH_DEBUG_FILLS: src = b->d_fills; bytes = (size_t)b->counters.fill_count * 12; break;
-    case PFH_DEBUG_TILES: src = b->d_tiles; bytes = (size_t)b->host.tile_count * 16; break;
+    case PFH_DEBUG_FILLS: bytes = (size_t)b->counters.fill_count * 12; break;
+    case PFH_DEBUG_TILES: bytes = (size_t)b->host.tile_count * 16; break;
     case PFH_DEBUG_Z_BUFFER: src = b->d_zbuffer; bytes = (fbn + 8) * 4; break;
     case PFH_DEBUG_FIRST_TILE_MAP: src = b->d_ftm; bytes = fbn * 4; break;
     case PFH_DEBUG_ALPHA_TILES: src = b->d_alpha; bytes = (size_t)b->counters.alpha * 8; break;
@@ -1307,6 +1319,35 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
     if (capacity < bytes) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "buffer too small");
     if (kind == PFH_DEBUG_ALPHA_RANGE) {
         std::memcpy(dst, range, 8);
+    } else if (kind == PFH_DEBUG_FILLS && bytes) {
+        // the reference's 12-byte Fill from its two device arrays
+        size_t n = b->counters.fill_count;
+        std::vector<uint2> words(n);
+        std::vector<uint32_t> links(n);
+        HIP_TRY(ctx, hipMemcpy(words.data(), b->d_fill_words, n * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(links.data(), b->d_fill_links, n * 4, hipMemcpyDeviceToHost));
+        uint32_t *out = static_cast<uint32_t *>(dst);
+        for (size_t i = 0; i < n; i++) {
+            out[3 * i] = words[i].x;
+            out[3 * i + 1] = words[i].y;
+            out[3 * i + 2] = links[i];
+        }
+    } else if (kind == PFH_DEBUG_TILES && bytes) {
+        // TileD3D11: words 0 and 1 come from their own arrays, where the tile
+        // has them (linked: a list link; fills: a fill list head), else ~0
+        size_t n = b->host.tile_count;
+        std::vector<uint32_t> next(n), first(n);
+        std::vector<uint8_t> status(n), flags(n);
+        HIP_TRY(ctx, hipMemcpy(dst, b->d_tiles, n * 16, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(next.data(), b->d_tile_next, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(first.data(), b->d_tile_first, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(status.data(), b->d_tile_status, n, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(flags.data(), b->d_tile_flags, n, hipMemcpyDeviceToHost));
+        uint32_t *out = static_cast<uint32_t *>(dst);
+        for (size_t i = 0; i < n; i++) {
+            out[4 * i] = (status[i] & pfh::kTileLinked) ? next[i] : 0xffffffffu;
+            out[4 * i + 1] = flags[i] ? first[i] : 0xffffffffu;
+        }
     } else if (bytes) {
         HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     }
