@@ -353,8 +353,41 @@ typedef struct pfh_command_list pfh_command_list;
 /* FillRule (content/src/fill.rs:18-23) */
 #define PFH_FILL_RULE_WINDING 0
 #define PFH_FILL_RULE_EVEN_ODD 1
-/* BlendMode: SrcOver is the only mode on this path. */
+/* BlendMode (content/src/effects.rs:99-160). SrcOver keeps the value 0; the
+ * others follow the reference's declaration order. At the D3D11 level a draw
+ * path's blend mode decides its tile rect (destructive modes use the view
+ * box, builder.rs:430-436) and whether it writes the z-buffer (opaque paint
+ * and SrcOver or Clear, effects.rs:202-233, builder.rs:83); the composite
+ * itself is src-over for every mode, as in the reference, whose paint
+ * metadata always carries SrcOver (paint.rs:573). */
 #define PFH_BLEND_MODE_SRC_OVER 0
+#define PFH_BLEND_MODE_CLEAR 1
+#define PFH_BLEND_MODE_COPY 2
+#define PFH_BLEND_MODE_SRC_IN 3
+#define PFH_BLEND_MODE_SRC_OUT 4
+#define PFH_BLEND_MODE_SRC_ATOP 5
+#define PFH_BLEND_MODE_DEST_IN 6
+#define PFH_BLEND_MODE_DEST_OUT 7
+#define PFH_BLEND_MODE_DEST_OVER 8
+#define PFH_BLEND_MODE_DEST_ATOP 9
+#define PFH_BLEND_MODE_XOR 10
+#define PFH_BLEND_MODE_LIGHTER 11
+#define PFH_BLEND_MODE_DARKEN 12
+#define PFH_BLEND_MODE_LIGHTEN 13
+#define PFH_BLEND_MODE_MULTIPLY 14
+#define PFH_BLEND_MODE_SCREEN 15
+#define PFH_BLEND_MODE_HARD_LIGHT 16
+#define PFH_BLEND_MODE_OVERLAY 17
+#define PFH_BLEND_MODE_COLOR_DODGE 18
+#define PFH_BLEND_MODE_COLOR_BURN 19
+#define PFH_BLEND_MODE_SOFT_LIGHT 20
+#define PFH_BLEND_MODE_DIFFERENCE 21
+#define PFH_BLEND_MODE_EXCLUSION 22
+#define PFH_BLEND_MODE_HUE 23
+#define PFH_BLEND_MODE_SATURATION 24
+#define PFH_BLEND_MODE_COLOR 25
+#define PFH_BLEND_MODE_LUMINOSITY 26
+#define PFH_BLEND_MODE_COUNT 27
 /* PointFlags (content/src/outline.rs:50-60) */
 #define PFH_POINT_CONTROL_0 0x01u
 #define PFH_POINT_CONTROL_1 0x02u

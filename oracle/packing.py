@@ -223,6 +223,11 @@ def segments_vectorized(arrays):
              "indices": indices}, ranges)
 
 
+# BlendMode values (PFH_BLEND_MODE_*): effects.rs:202-262
+DESTRUCTIVE = {1, 2, 3, 6, 4, 9}  # Clear, Copy, SrcIn, DestIn, SrcOut, DestAtop
+OCCLUDING = {0, 1}                 # SrcOver, Clear
+
+
 def build(scene, transform=(1, 0, 0, 1, 0, 0)):
     """SceneBuilder::build for the D3D11 level. Returns
     {"paints": (N, 4) u8, "draw": {...}, "clip": {...}, "batches": [(kind, dict)]}."""
@@ -262,14 +267,18 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
         clipped = _intersect(bounds, vb)
         if clipped is None:
             continue
+        mode = int(draw.blend_modes[i])
         rule = int(draw.fill_rules[i])
         ctrl = 0x2 if rule == 1 else 0x1
         if draw_batch is None:
             draw_batch = _Batch(MAX_CLIP_BATCHES, 0, t)
         cl = add_clip(int(draw.clip_paths[i]), 0)
         paint = int(draw.paints[i])
-        opaque = paint < len(paints) and paints[paint, 3] == 255
-        draw_batch.push(_tile_bounds(clipped), paint, ctrl, i, draw_ranges[i], cl, opaque)
+        # builder.rs:83 occludes = opaque && occludes_backdrop; builder.rs:430-436
+        # destructive modes tile the view box
+        opaque = paint < len(paints) and paints[paint, 3] == 255 and mode in OCCLUDING
+        rect = _tile_bounds(vb if mode in DESTRUCTIVE else clipped)
+        draw_batch.push(rect, paint, ctrl, i, draw_ranges[i], cl, opaque)
 
     batches = []
     for b in reversed(prepare_batches):

@@ -113,6 +113,18 @@ struct RectI {
     int32_t height() const { return y1 - y0; }
 };
 
+// BlendMode::is_destructive (effects.rs:234-262).
+bool blend_is_destructive(int32_t mode) {
+    return mode == PFH_BLEND_MODE_CLEAR || mode == PFH_BLEND_MODE_COPY ||
+           mode == PFH_BLEND_MODE_SRC_IN || mode == PFH_BLEND_MODE_DEST_IN ||
+           mode == PFH_BLEND_MODE_SRC_OUT || mode == PFH_BLEND_MODE_DEST_ATOP;
+}
+
+// BlendMode::occludes_backdrop (effects.rs:202-233).
+bool blend_occludes_backdrop(int32_t mode) {
+    return mode == PFH_BLEND_MODE_SRC_OVER || mode == PFH_BLEND_MODE_CLEAR;
+}
+
 // round_rect_out_to_tile_bounds (tiles.rs:64-66): (rect * 1/16).round_out().to_i32().
 RectI tile_bounds(const RectF &r) {
     const float s = 1.0f / 16.0f;
@@ -428,16 +440,21 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
         RectF bounds = xform_rect(st.transform, dp.bounds);
         RectF clipped;
         if (!intersect(bounds, st.view_box, &clipped)) continue;
-        // Only SrcOver is on this path: not destructive, occludes the backdrop.
+        // BuiltPath::new (builder.rs:430-436): a destructive blend mode tiles
+        // the whole view box, anything else its clipped bounds
         uint8_t ctrl = dp.fill_rule == PFH_FILL_RULE_EVEN_ODD ? 0x2 : 0x1;
-        BuiltPathGPU built{tile_bounds(clipped), dp.paint, ctrl};
+        BuiltPathGPU built{tile_bounds(blend_is_destructive(dp.blend_mode) ? st.view_box : clipped),
+                           dp.paint, ctrl};
         if (!have_draw_batch) {
             draw_batch = new_batch(MAX_CLIP_BATCHES, PFH_PATH_SOURCE_DRAW, st.transform);
             have_draw_batch = true;
         }
         uint32_t cb = 0, ci = 0;
         bool has_clip = add_clip_path_to_batch(st, dp.clip_path, 0, &cb, &ci);
-        bool opaque = dp.paint < scene->paints.size() && scene->paints[dp.paint].a == 255;
+        // BuiltDrawPath::new (builder.rs:83): occludes = opaque paint and a
+        // mode that occludes the backdrop
+        bool opaque = dp.paint < scene->paints.size() && scene->paints[dp.paint].a == 255 &&
+                      blend_occludes_backdrop(dp.blend_mode);
         batch_push(draw_batch, built, (uint32_t)i, draw_seg_begin[i], draw_seg_end[i], has_clip,
                    cb, ci, opaque);
     }

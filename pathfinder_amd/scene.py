@@ -24,7 +24,34 @@ class FillRule(enum.IntEnum):
 
 
 class BlendMode(enum.IntEnum):
+    """BlendMode (content/src/effects.rs:99-160); SrcOver is 0 (PFH_BLEND_MODE_*)."""
     SRC_OVER = 0
+    CLEAR = 1
+    COPY = 2
+    SRC_IN = 3
+    SRC_OUT = 4
+    SRC_ATOP = 5
+    DEST_IN = 6
+    DEST_OUT = 7
+    DEST_OVER = 8
+    DEST_ATOP = 9
+    XOR = 10
+    LIGHTER = 11
+    DARKEN = 12
+    LIGHTEN = 13
+    MULTIPLY = 14
+    SCREEN = 15
+    HARD_LIGHT = 16
+    OVERLAY = 17
+    COLOR_DODGE = 18
+    COLOR_BURN = 19
+    SOFT_LIGHT = 20
+    DIFFERENCE = 21
+    EXCLUSION = 22
+    HUE = 23
+    SATURATION = 24
+    COLOR = 25
+    LUMINOSITY = 26
 
 
 class RendererLevel(enum.IntEnum):
@@ -193,7 +220,7 @@ class PathArrays:
     """Many paths in flat arrays (the bulk form of pfh_scene_push_*_paths)."""
 
     def __init__(self, points, flags, contour_ends, path_contour_ends, paints=None,
-                 fill_rules=None, clip_paths=None):
+                 fill_rules=None, clip_paths=None, blend_modes=None):
         self.points = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
         self.flags = np.ascontiguousarray(flags, np.uint8)
         self.contour_ends = np.ascontiguousarray(contour_ends, np.uint32)
@@ -205,6 +232,8 @@ class PathArrays:
             np.ascontiguousarray(fill_rules, np.int32)
         self.clip_paths = np.full(n, -1, np.int32) if clip_paths is None else \
             np.ascontiguousarray(clip_paths, np.int32)
+        self.blend_modes = np.zeros(n, np.int32) if blend_modes is None else \
+            np.ascontiguousarray(blend_modes, np.int32)
 
     def __len__(self):
         return len(self.path_contour_ends)
@@ -220,7 +249,7 @@ class PathArrays:
 
     @staticmethod
     def concat(parts):
-        pts, fl, ce, pce, pa, fr, cp = [], [], [], [], [], [], []
+        pts, fl, ce, pce, pa, fr, cp, bm = [], [], [], [], [], [], [], []
         npts = ncont = 0
         for a in parts:
             pts.append(a.points)
@@ -230,12 +259,14 @@ class PathArrays:
             pa.append(a.paints)
             fr.append(a.fill_rules)
             cp.append(a.clip_paths)
+            bm.append(a.blend_modes)
             npts += len(a.points)
             ncont += len(a.contour_ends)
         cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
         return PathArrays(np.concatenate(pts) if pts else np.zeros((0, 2), np.float32),
                           cat(fl, np.uint8), cat(ce, np.uint32), cat(pce, np.uint32),
-                          cat(pa, np.uint32), cat(fr, np.int32), cat(cp, np.int32))
+                          cat(pa, np.uint32), cat(fr, np.int32), cat(cp, np.int32),
+                          cat(bm, np.int32))
 
 
 def _ptr(a):
@@ -269,11 +300,11 @@ class Scene:
         self._paint_cache[key] = pid
         return pid
 
-    def _single(self, outline, paint, fill_rule, clip_path):
+    def _single(self, outline, paint, fill_rule, clip_path, blend_mode=0):
         return PathArrays(outline.points, outline.flags, outline.contour_ends,
                           np.asarray([len(outline.contour_ends)], np.uint32),
                           np.asarray([paint], np.uint32), np.asarray([int(fill_rule)], np.int32),
-                          np.asarray([clip_path], np.int32))
+                          np.asarray([clip_path], np.int32), np.asarray([int(blend_mode)], np.int32))
 
     def draw_path_count(self):
         return sum(len(p) for p in self._draw_parts)
@@ -284,7 +315,8 @@ class Scene:
     def push_draw_path(self, draw_path):
         pid = self.draw_path_count()
         self._draw_parts.append(self._single(draw_path.outline, draw_path.paint,
-                                             draw_path.fill_rule, draw_path.clip_path))
+                                             draw_path.fill_rule, draw_path.clip_path,
+                                             draw_path.blend_mode))
         return pid
 
     def push_clip_path(self, clip_path):
@@ -328,7 +360,8 @@ class Scene:
             lib.pfh_scene_push_draw_paths(h, _ptr(a.points), _ptr(a.flags), len(a.points),
                                           _ptr(a.contour_ends), len(a.contour_ends),
                                           _ptr(a.path_contour_ends), len(a), _ptr(a.paints),
-                                          _ptr(a.fill_rules), _ptr(a.clip_paths), None)
+                                          _ptr(a.fill_rules), _ptr(a.clip_paths),
+                                          _ptr(a.blend_modes))
         return h
 
     def build(self, options=None):

@@ -479,3 +479,26 @@ def edge_scenes():
     scene, size, bg = small_scene(paths=64, size=512)
     out["mirrored"] = (scene, size, bg, (-1.0, 0.0, 0.0, 1.0, 512.0, 0.0))
     return out
+
+
+def blend_scene(seed=SEED_BASE + 5, width=160, height=120):
+    """One draw path per blend mode (27), overlapping, opaque and translucent
+    paints, some clipped: destructive modes tile the whole view box
+    (builder.rs:430-436) and only opaque SrcOver/Clear paths write z."""
+    from .scene import ColorU
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.set_view_box(RectF(0, 0, width, height))
+    cp, cf = _clip_shape(1, width * 0.5, height * 0.5, width * 0.3, height * 0.3)
+    clip = s.push_clip_paths(PathArrays(cp, cf, [len(cp)], [1], None, [0], [-1]))
+    parts = []
+    for mode in range(27):
+        cx, cy = rng.uniform(0.2, 0.8) * width, rng.uniform(0.2, 0.8) * height
+        r = rng.uniform(10, 40)
+        pts, fl = _clip_shape(mode % 2, cx, cy, r, r * rng.uniform(0.6, 1.2))
+        a = 255 if mode % 3 != 2 else int(rng.integers(60, 230))
+        pid = s.push_paint(ColorU(*[int(v) for v in rng.integers(0, 256, 3)], a))
+        parts.append(PathArrays(pts, fl, [len(pts)], [1], [pid],
+                                [int(rng.integers(2))], [clip if mode % 5 == 4 else -1], [mode]))
+    s.push_draw_paths(PathArrays.concat(parts))
+    return s, (width, height), ColorF(1, 1, 1, 1)

@@ -61,3 +61,16 @@ def test_4k_frame_against_d3d9_level(renderer_factory):
     assert d.mean() < 0.05
     assert np.mean(px <= 1) >= 0.997
     assert d.max() <= 64
+
+
+def test_blend_modes_bit_exact(renderer_factory):
+    """A path per blend mode: destructive modes tile the view box, only opaque
+    SrcOver/Clear paths write z; the composite is src-over for every mode."""
+    scene, size, bg = synth.blend_scene()
+    _say("case blend modes", size)
+    cmds, o = oracle_frame(scene, size, bg)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)

@@ -48,6 +48,24 @@ def test_packing_edge_scenes(name):
     _compare(scene, xf)
 
 
+def test_packing_blend_modes():
+    """Every blend mode: destructive ones tile the view box, only opaque
+    SrcOver/Clear paths write z (builder.rs:83, 430-436)."""
+    scene, size, _ = synth.blend_scene()
+    cmds = _compare(scene)
+    _compare(scene, (0.9, 0.2, -0.1, 1.1, 3.5, -7.25))
+    (_, b), = [kb for kb in cmds["batches"] if kb[0] == packing.DRAW]
+    vb = (0, 0, (size[0] + 15) // 16, (size[1] + 15) // 16)
+    modes = scene.draw_arrays().blend_modes
+    paints = cmds["paints"]
+    for i, pm in enumerate(b["propagate_metadata"]):
+        mode, path = int(modes[i]), int(pm[5])
+        assert path == i
+        assert (tuple(int(v) for v in pm[:4]) == vb) == (mode in packing.DESTRUCTIVE)
+        opaque = paints[scene.draw_arrays().paints[i], 3] == 255
+        assert pm[6] == int(opaque and mode in packing.OCCLUDING)
+
+
 def test_packing_small_and_transformed():
     scene, _, _ = synth.small_scene()
     _compare(scene)
