@@ -432,6 +432,64 @@ uint32_t pfh_scene_push_clip_paths(pfh_scene *scene, const pfh_vector2f *points,
                                    const int32_t *fill_rules, const int32_t *clip_paths);
 uint32_t pfh_scene_draw_path_count(const pfh_scene *scene);
 
+/* The scene's paths as flat arrays (the layout pfh_scene_push_*_paths takes:
+ * contour_ends are absolute point indices, path i owns contours
+ * [path_contour_ends[i-1], path_contour_ends[i])). Pointers stay valid until
+ * the scene is modified or exported again. */
+typedef struct pfh_path_arrays {
+    const pfh_vector2f *points;
+    const uint8_t *flags;
+    uint32_t point_count;
+    const uint32_t *contour_ends;
+    uint32_t contour_count;
+    const uint32_t *path_contour_ends;
+    uint32_t path_count;
+    const uint32_t *paints;       /* draw paths only */
+    const int32_t *fill_rules;
+    const int32_t *clip_paths;
+    const int32_t *blend_modes;   /* draw paths only */
+} pfh_path_arrays;
+pfh_status pfh_scene_path_arrays(pfh_scene *scene, uint32_t path_source, pfh_path_arrays *out);
+pfh_status pfh_scene_paints(const pfh_scene *scene, const pfh_color_u **colors, uint32_t *count);
+pfh_status pfh_scene_get_view_box(const pfh_scene *scene, pfh_rect_f *out);
+
+/* SVG ingest: SVGScene::from_tree (svg/src/lib.rs:65-290) over this library's
+ * own reader (paths, basic shapes, groups, transforms, solid fills and
+ * strokes; gradients, clip paths, masks, text and images are skipped and
+ * counted). Returns NULL on a malformed document, like
+ * PFSVGSceneCreateWithMemory (c/src/lib.rs:804-815). info may be NULL. */
+typedef struct pfh_svg_info {
+    float view_box[4];            /* x, y, width, height */
+    uint32_t fills, strokes;      /* draw paths pushed */
+    uint32_t skipped_elements;    /* elements (and their subtrees) not supported */
+    uint32_t unsupported_paints;  /* url() or unknown paints (drawn as none) */
+} pfh_svg_info;
+pfh_scene *pfh_svg_scene_create(const char *bytes, size_t byte_len, pfh_svg_info *info);
+
+/* Stroking: OutlineStrokeToFill (content/src/stroke.rs) with StrokeStyle
+ * {line_width, line_cap, line_join (miter limit)}. The outline is stroked in
+ * its own space, then transformed (transform may be NULL); closed[k] != 0
+ * joins contour k's end to its start (NULL: every contour closed). The
+ * result is a fill outline for the nonzero rule. */
+#define PFH_LINE_CAP_BUTT 0
+#define PFH_LINE_CAP_SQUARE 1
+#define PFH_LINE_CAP_ROUND 2
+#define PFH_LINE_JOIN_MITER 0
+#define PFH_LINE_JOIN_BEVEL 1
+#define PFH_LINE_JOIN_ROUND 2
+typedef struct pfh_stroke_style {
+    float line_width;
+    int32_t line_cap;
+    int32_t line_join;
+    float miter_limit;  /* for PFH_LINE_JOIN_MITER; <= 0 means 4 */
+} pfh_stroke_style;
+typedef struct pfh_outline_buffer pfh_outline_buffer;
+pfh_status pfh_stroke_outline(const pfh_outline *outline, const uint8_t *closed,
+                              const pfh_stroke_style *style, const pfh_transform2f *transform,
+                              pfh_outline_buffer **out);
+const pfh_outline *pfh_outline_buffer_view(const pfh_outline_buffer *buffer);
+void pfh_outline_buffer_destroy(pfh_outline_buffer *buffer);
+
 /* BuildOptions (options.rs:53-61): a 2D transform only. */
 typedef struct pfh_build_options {
     pfh_transform2f transform;

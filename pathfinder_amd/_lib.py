@@ -52,6 +52,24 @@ DEBUG_COVERAGE = 8
 NOALLOC_SCRATCH_BYTES = 64 << 20  # PFH_NOALLOC_SCRATCH_BYTES
 
 
+class PathArraysC(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("flags", C.c_void_p), ("point_count", C.c_uint32),
+                ("contour_ends", C.c_void_p), ("contour_count", C.c_uint32),
+                ("path_contour_ends", C.c_void_p), ("path_count", C.c_uint32),
+                ("paints", C.c_void_p), ("fill_rules", C.c_void_p), ("clip_paths", C.c_void_p),
+                ("blend_modes", C.c_void_p)]
+
+
+class SvgInfo(C.Structure):
+    _fields_ = [("view_box", C.c_float * 4), ("fills", C.c_uint32), ("strokes", C.c_uint32),
+                ("skipped_elements", C.c_uint32), ("unsupported_paints", C.c_uint32)]
+
+
+class StrokeStyle(C.Structure):
+    _fields_ = [("line_width", C.c_float), ("line_cap", C.c_int32), ("line_join", C.c_int32),
+                ("miter_limit", C.c_float)]
+
+
 class Vector2F(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float)]
 
@@ -177,6 +195,13 @@ SIGNATURES = {
     "pfh_scene_push_draw_paths": (U32, [P, P, P, U32, P, U32, P, U32, P, P, P, P]),
     "pfh_scene_push_clip_paths": (U32, [P, P, P, U32, P, U32, P, U32, P, P]),
     "pfh_scene_draw_path_count": (U32, [P]),
+    "pfh_scene_path_arrays": (I32, [P, U32, C.POINTER(PathArraysC)]),
+    "pfh_scene_paints": (I32, [P, C.POINTER(P), C.POINTER(U32)]),
+    "pfh_scene_get_view_box": (I32, [P, C.POINTER(RectF)]),
+    "pfh_svg_scene_create": (P, [C.c_char_p, SZ, C.POINTER(SvgInfo)]),
+    "pfh_stroke_outline": (I32, [P, P, C.POINTER(StrokeStyle), P, C.POINTER(P)]),
+    "pfh_outline_buffer_view": (C.POINTER(Outline), [P]),
+    "pfh_outline_buffer_destroy": (None, [P]),
     "pfh_scene_build": (I32, [P, C.POINTER(BuildOptions), C.POINTER(P)]),
     "pfh_command_list_destroy": (None, [P]),
     "pfh_command_list_len": (U32, [P]),

@@ -143,6 +143,9 @@ struct pfh_scene {
     std::vector<pfh_color_u> paints;
     std::map<uint32_t, uint32_t> paint_cache;  // packed rgba -> PaintId
     PathSet draw, clip;
+    // pfh_scene_path_arrays: the per-path columns of the last export
+    std::vector<uint32_t> ex_path_contour_ends, ex_paints;
+    std::vector<int32_t> ex_fill_rules, ex_clip_paths, ex_blend_modes;
 };
 
 struct pfh_command {
@@ -394,6 +397,51 @@ uint32_t pfh_scene_push_clip_paths(pfh_scene *scene, const pfh_vector2f *points,
         return 0xffffffffu;
     return push_paths(scene->clip, points, flags, point_count, contour_ends, contour_count,
                       path_contour_ends, path_count, nullptr, fill_rules, clip_paths, nullptr);
+}
+
+pfh_status pfh_scene_path_arrays(pfh_scene *scene, uint32_t path_source, pfh_path_arrays *out) {
+    if (!scene || !out || path_source > PFH_PATH_SOURCE_CLIP) return PFH_ERROR_INVALID_ARGUMENT;
+    const PathSet &set = path_source == PFH_PATH_SOURCE_CLIP ? scene->clip : scene->draw;
+    size_t n = set.paths.size();
+    scene->ex_path_contour_ends.resize(n);
+    scene->ex_paints.resize(n);
+    scene->ex_fill_rules.resize(n);
+    scene->ex_clip_paths.resize(n);
+    scene->ex_blend_modes.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const PathRecord &r = set.paths[i];
+        scene->ex_path_contour_ends[i] = r.contour_end;
+        scene->ex_paints[i] = r.paint;
+        scene->ex_fill_rules[i] = r.fill_rule;
+        scene->ex_clip_paths[i] = r.clip_path;
+        scene->ex_blend_modes[i] = r.blend_mode;
+    }
+    out->points = set.points.data();
+    out->flags = set.flags.data();
+    out->point_count = (uint32_t)set.points.size();
+    out->contour_ends = set.contour_ends.data();
+    out->contour_count = (uint32_t)set.contour_ends.size();
+    out->path_contour_ends = scene->ex_path_contour_ends.data();
+    out->path_count = (uint32_t)n;
+    out->paints = scene->ex_paints.data();
+    out->fill_rules = scene->ex_fill_rules.data();
+    out->clip_paths = scene->ex_clip_paths.data();
+    out->blend_modes = scene->ex_blend_modes.data();
+    return PFH_OK;
+}
+
+pfh_status pfh_scene_paints(const pfh_scene *scene, const pfh_color_u **colors, uint32_t *count) {
+    if (!scene || !colors || !count) return PFH_ERROR_INVALID_ARGUMENT;
+    *colors = scene->paints.data();
+    *count = (uint32_t)scene->paints.size();
+    return PFH_OK;
+}
+
+pfh_status pfh_scene_get_view_box(const pfh_scene *scene, pfh_rect_f *out) {
+    if (!scene || !out) return PFH_ERROR_INVALID_ARGUMENT;
+    *out = pfh_rect_f{{scene->view_box.x0, scene->view_box.y0},
+                      {scene->view_box.x1, scene->view_box.y1}};
+    return PFH_OK;
 }
 
 uint32_t pfh_scene_draw_path_count(const pfh_scene *scene) {

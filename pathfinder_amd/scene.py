@@ -678,3 +678,105 @@ class Renderer:
         got = C.c_uint32()
         self._check(self._lib.pfh_profile_read(self._h, arr, n, C.byref(got)))
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n)}
+
+
+def _arrays_from_native(h, source):
+    """The native scene's paths (pfh_scene_path_arrays) as PathArrays."""
+    lib = L.load()
+    pa = L.PathArraysC()
+    L.check(lib.pfh_scene_path_arrays(h, source, C.byref(pa)))
+
+    def arr(ptr, n, dt, shape=None):
+        if n == 0 or not ptr:
+            return np.zeros((0,) + (shape or ()), dt)
+        ct = {np.float32: C.c_float, np.uint8: C.c_uint8, np.uint32: C.c_uint32,
+              np.int32: C.c_int32}[dt]
+        count = n * (int(np.prod(shape)) if shape else 1)
+        a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+        return a.reshape((n,) + shape) if shape else a
+
+    n = pa.path_count
+    out = PathArrays(arr(pa.points, pa.point_count, np.float32, (2,)),
+                     arr(pa.flags, pa.point_count, np.uint8),
+                     arr(pa.contour_ends, pa.contour_count, np.uint32),
+                     arr(pa.path_contour_ends, n, np.uint32),
+                     arr(pa.paints, n, np.uint32) if source == L.PATH_SOURCE_DRAW else None,
+                     arr(pa.fill_rules, n, np.int32), arr(pa.clip_paths, n, np.int32),
+                     arr(pa.blend_modes, n, np.int32) if source == L.PATH_SOURCE_DRAW else None)
+    return out
+
+
+def scene_from_native(h):
+    """A Scene holding the same view box, paints and paths as native scene h."""
+    lib = L.load()
+    s = Scene()
+    vb = L.RectF()
+    L.check(lib.pfh_scene_get_view_box(h, C.byref(vb)))
+    s.set_view_box(RectF(vb.origin.x, vb.origin.y, vb.lower_right.x, vb.lower_right.y))
+    colors, count = C.c_void_p(), C.c_uint32()
+    L.check(lib.pfh_scene_paints(h, C.byref(colors), C.byref(count)))
+    if count.value:
+        rgba = np.ctypeslib.as_array(C.cast(colors, C.POINTER(C.c_uint8)),
+                                     shape=(count.value * 4,)).reshape(-1, 4)
+        for c in rgba:
+            s.push_paint(ColorU(int(c[0]), int(c[1]), int(c[2]), int(c[3])))
+    clip = _arrays_from_native(h, L.PATH_SOURCE_CLIP)
+    draw = _arrays_from_native(h, L.PATH_SOURCE_DRAW)
+    if len(clip):
+        s.push_clip_paths(clip)
+    if len(draw):
+        s.push_draw_paths(draw)
+    return s
+
+
+def svg_scene(data):
+    """SVGScene::from_tree (svg/src/lib.rs:65-290) through pfh_svg_scene_create:
+    returns (Scene, info dict); raises ValueError on a malformed document."""
+    lib = L.load()
+    if isinstance(data, str):
+        data = data.encode()
+    info = L.SvgInfo()
+    h = lib.pfh_svg_scene_create(data, len(data), C.byref(info))
+    if not h:
+        raise ValueError("malformed SVG document")
+    try:
+        s = scene_from_native(h)
+    finally:
+        lib.pfh_scene_destroy(h)
+    return s, {"view_box": list(info.view_box), "fills": info.fills, "strokes": info.strokes,
+               "skipped_elements": info.skipped_elements,
+               "unsupported_paints": info.unsupported_paints}
+
+
+def stroke_outline(outline, width, cap=0, join=0, miter_limit=4.0, closed=None, transform=None):
+    """OutlineStrokeToFill (content/src/stroke.rs) through pfh_stroke_outline:
+    the fill Outline (nonzero) of the stroke of `outline` (cap/join:
+    PFH_LINE_CAP_* / PFH_LINE_JOIN_*)."""
+    lib = L.load()
+    pts = np.ascontiguousarray(outline.points, np.float32)
+    fl = np.ascontiguousarray(outline.flags, np.uint8)
+    ends = np.ascontiguousarray(outline.contour_ends, np.uint32)
+    o = L.Outline(pts.ctypes.data_as(C.c_void_p), fl.ctypes.data_as(C.c_void_p), len(pts),
+                  ends.ctypes.data_as(C.c_void_p), len(ends))
+    cl = None if closed is None else np.ascontiguousarray(closed, np.uint8)
+    st = L.StrokeStyle(width, cap, join, miter_limit)
+    xf = None if transform is None else Transform2F(*transform).to_c()
+    buf = C.c_void_p()
+    L.check(lib.pfh_stroke_outline(C.byref(o), cl.ctypes.data_as(C.c_void_p) if cl is not None
+                                   else None, C.byref(st),
+                                   C.byref(xf) if xf is not None else None, C.byref(buf)))
+    try:
+        v = lib.pfh_outline_buffer_view(buf).contents
+        n, m = v.point_count, v.contour_count
+        out = Outline()
+        out.points = np.ctypeslib.as_array(C.cast(v.points, C.POINTER(C.c_float)),
+                                           shape=(2 * n,)).reshape(-1, 2).copy() if n else \
+            np.zeros((0, 2), np.float32)
+        out.flags = np.ctypeslib.as_array(C.cast(v.flags, C.POINTER(C.c_uint8)),
+                                          shape=(n,)).copy() if n else np.zeros(0, np.uint8)
+        out.contour_ends = np.ctypeslib.as_array(C.cast(v.contour_ends, C.POINTER(C.c_uint32)),
+                                                 shape=(m,)).copy() if m else \
+            np.zeros(0, np.uint32)
+        return out
+    finally:
+        lib.pfh_outline_buffer_destroy(buf)

@@ -74,3 +74,24 @@ def test_blend_modes_bit_exact(renderer_factory):
     report = compare_frame(r, o, cmds)
     _say(report)
     assert_bit_exact(report)
+
+
+def test_svg_tiger_bit_exact(renderer_factory):
+    """The Ghostscript tiger (the reference's resources/svg test image) read by
+    pfh_svg_scene_create, fills and strokes, at 4x on 800 x 800."""
+    import os
+    from pathfinder_amd import RectF
+    from pathfinder_amd.scene import svg_scene
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "Ghostscript_Tiger.svg")
+    scene, info = svg_scene(open(path, "rb").read())
+    scene.set_view_box(RectF(0, 0, 800, 800))
+    from pathfinder_amd import ColorF
+    bg, size, t = ColorF(1, 1, 1, 1), (800, 800), (4, 0, 0, 4, 0, 0)
+    _say("case svg tiger", info)
+    cmds, o = oracle_frame(scene, size, bg, transform=t)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene, transform=t)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
