@@ -60,11 +60,13 @@ def build(verbose=False):
           "-o", obj("pfh_scene")], verbose)
     _run([cxx, *COMMON, "-c", os.path.join(CSRC, "pfh_svg.cpp"), "-o", obj("pfh_svg")],
          verbose)
+    _run([cxx, *COMMON, "-c", os.path.join(CSRC, "pfh_capi.cpp"), "-o", obj("pfh_capi")],
+         verbose)
     _run([cxx, "-fPIC", "-c", os.path.join(CSRC, "pfh_lut.cpp"),
           '-DPFH_AREA_LUT_PATH="%s"' % LUT, "-o", obj("pfh_lut")], verbose)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-Wl,--no-undefined"], verbose)
+          "-lz", "-Wl,--no-undefined"], verbose)
     os.replace(tmp, LIB)
     return LIB
 
