@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define PFHIP_ABI_VERSION 1
+#define PFHIP_ABI_VERSION 2
 
 typedef enum pfh_status {
     PFH_OK = 0,
@@ -146,7 +146,50 @@ typedef struct pfh_tile_batch {
     const pfh_backdrop_info *backdrops;
     uint32_t backdrop_count;
     int32_t clip_batch_id; /* -1: no clipped paths */
+    /* DrawTileBatchD3D11.color_texture (gpu_data.rs; TileBatchTexture): the
+     * colour texture page of a draw batch (-1: none) and its sampling flags */
+    int32_t color_texture_page;
+    uint32_t color_texture_sampling_flags;
 } pfh_tile_batch;
+
+/* TextureSamplingFlags (gpu/src/lib.rs:523-528) */
+#define PFH_SAMPLING_REPEAT_U 0x01u
+#define PFH_SAMPLING_REPEAT_V 0x02u
+#define PFH_SAMPLING_NEAREST_MIN 0x04u
+#define PFH_SAMPLING_NEAREST_MAG 0x08u
+
+/* ColorCombineMode (gpu/blend.rs:204-212) */
+#define PFH_COMBINE_NONE 0
+#define PFH_COMBINE_SRC_IN 1
+#define PFH_COMBINE_DEST_IN 2
+
+/* Filter (renderer/src/paint.rs, effects.rs PatternFilter) and its parameters:
+ *   PFH_FILTER_RADIAL_GRADIENT: line from x, y, line to x, y, radii r0, r1,
+ *     uv origin x, y (Filter::RadialGradient {line, radii, uv_origin});
+ *   PFH_FILTER_BLUR: direction (0 = x, 1 = y), sigma;
+ *   PFH_FILTER_COLOR_MATRIX: the five columns of ColorMatrix (20 floats).
+ * The text filter needs the gamma LUT and font rendering, which are not on
+ * this path: entries with it are refused. */
+#define PFH_FILTER_NONE 0
+#define PFH_FILTER_RADIAL_GRADIENT 1
+#define PFH_FILTER_BLUR 2
+#define PFH_FILTER_COLOR_MATRIX 3
+typedef struct pfh_filter {
+    int32_t kind;
+    float params[20];
+} pfh_filter;
+
+/* TextureMetadataEntry (gpu_data.rs): per paint, the transform from device
+ * pixels to colour texture coordinates (color_0_transform), how the colour
+ * texture combines with the base colour, the base colour, the filter, and
+ * the blend mode (BlendMode::to_composite_ctrl picks the composite op). */
+typedef struct pfh_texture_metadata_entry {
+    pfh_transform2f color_0_transform;
+    int32_t color_0_combine_mode;  /* PFH_COMBINE_* */
+    pfh_color_u base_color;
+    pfh_filter filter;
+    int32_t blend_mode;            /* PFH_BLEND_MODE_* */
+} pfh_texture_metadata_entry;
 
 /* ------------------------------------------------------------ context */
 
@@ -182,6 +225,23 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame);
  * base_color, paint.rs:323-331; encoded as f16 per gpu/renderer.rs:702-775). */
 pfh_status pfh_upload_texture_metadata(pfh_context *ctx, const pfh_color_u *base_colors,
                                        uint32_t paint_count);
+
+/* UploadTextureMetadata (gpu/renderer.rs:702-775, compute_filter_params
+ * :967-1049): each entry becomes ten RGBA16F texels, rounded to binary16 as
+ * the reference rounds them (the composite reads these values). */
+pfh_status pfh_upload_texture_metadata_entries(pfh_context *ctx,
+                                               const pfh_texture_metadata_entry *entries,
+                                               uint32_t count);
+
+/* AllocateTexturePage / UploadTexelData (gpu_data.rs:37-105; gpu/renderer.rs
+ * allocate_pattern_texture_page, upload_texel_data): RGBA8 colour texture
+ * pages, zero when allocated; texels fill `rect` (origin, lower right) row by
+ * row. Pages live until the context is destroyed or reallocated. */
+typedef struct pfh_rect_i { int32_t min_x, min_y, max_x, max_y; } pfh_rect_i;
+pfh_status pfh_allocate_texture_page(pfh_context *ctx, uint32_t page_id, int32_t width,
+                                     int32_t height);
+pfh_status pfh_upload_texel_data(pfh_context *ctx, uint32_t page_id, const pfh_rect_i *rect,
+                                 const pfh_color_u *texels);
 
 /* RendererD3D11::upload_scene (d3d11/renderer.rs:236-241). */
 pfh_status pfh_upload_scene(pfh_context *ctx, const pfh_segments *draw_segments,

@@ -22,6 +22,7 @@ LUT_PATH = os.path.join(HERE, "..", "pathfinder_amd", "data", "area_lut.rgba")
 MICROLINES, FILLS, TILES, ZBUFFER, FIRST_TILE_MAP, ALPHA_TILES, BACKDROPS, ALPHA_RANGE = range(8)
 OPT_CLIP_STRIDE_LITERAL = 1
 OPT_PLANT_DICE = 2  # planted dice mistakes, only for tests that show a pin catches them
+OPT_PLANT_PAINT = 3  # planted paint mistakes, likewise
 _ELEM = {MICROLINES: (np.uint32, 4), FILLS: (np.uint32, 3), TILES: (np.uint32, 4),
          ZBUFFER: (np.int32, 1), FIRST_TILE_MAP: (np.int32, 1), ALPHA_TILES: (np.uint32, 2),
          BACKDROPS: (np.int32, 3), ALPHA_RANGE: (np.uint32, 1)}
@@ -60,6 +61,10 @@ def load():
         "orc_f32_to_f16": (C.c_uint16, [C.c_float]),
         "orc_f16_to_f32": (C.c_float, [C.c_uint16]),
         "orc_compute_coverage": (None, [P, C.c_float, C.c_float, C.c_float, C.c_float, P]),
+        "orc_upload_metadata": (I32, [P, P, U32]),
+        "orc_allocate_page": (None, [P, U32, I32, I32]),
+        "orc_upload_texels": (I32, [P, U32, I32, I32, I32, I32, P]),
+        "orc_set_batch_color_texture": (I32, [P, U32, I32, U32]),
         "orc_stage_reset": (None, []),
         "orc_stage_times": (None, [P]),
     }
@@ -68,6 +73,13 @@ def load():
         fn.restype, fn.argtypes = res, args
     _lib = lib
     return lib
+
+
+# orc_metadata_entry: colour transform (m00 m01 m10 m11 tx ty), combine mode,
+# base colour, filter kind and 20 parameters, blend mode
+METADATA_DTYPE = np.dtype([("transform", np.float32, 6), ("combine", np.int32),
+                           ("base_color", np.uint8, 4), ("filter_kind", np.int32),
+                           ("filter_params", np.float32, 20), ("blend_mode", np.int32)])
 
 
 def area_lut():
@@ -109,6 +121,19 @@ class Oracle:
         a = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 4)
         self.lib.orc_upload_paints(self.h, _p(a), len(a))
 
+    def upload_metadata(self, entries):
+        a = np.ascontiguousarray(entries, METADATA_DTYPE)
+        if self.lib.orc_upload_metadata(self.h, _p(a), len(a)) != 0:
+            raise ValueError("oracle: unknown filter kind")
+
+    def allocate_page(self, page, width, height):
+        self.lib.orc_allocate_page(self.h, page, width, height)
+
+    def upload_texels(self, page, x, y, rgba):
+        a = np.ascontiguousarray(rgba, np.uint8)
+        if self.lib.orc_upload_texels(self.h, page, x, y, a.shape[1], a.shape[0], _p(a)) != 0:
+            raise ValueError("oracle: texel rect outside page %d" % page)
+
     def upload_scene(self, draw, clip):
         arrs = []
         for s in (draw, clip):
@@ -127,6 +152,10 @@ class Oracle:
                                         len(arrs[4]), b["clip_batch_id"])
         if rc != 0:
             raise RuntimeError("oracle prepare_batch failed: %d" % rc)
+        if self.lib.orc_set_batch_color_texture(self.h, b["batch_id"],
+                                                int(b.get("color_texture_page", -1)),
+                                                int(b.get("color_texture_sampling_flags", 0))):
+            raise RuntimeError("oracle set_batch_color_texture failed")
 
     def draw_batch(self, batch_id):
         if self.lib.orc_draw_batch(self.h, batch_id) != 0:
@@ -162,7 +191,14 @@ class Oracle:
         dropping the previous frame's batches."""
         self.end_frame()
         self.begin_frame(width, height, background)
-        self.upload_paints(cmds["paints"])
+        for page, (w, h) in cmds.get("pages", {}).items():
+            self.allocate_page(page, w, h)
+        for page, x, y, rgba in cmds.get("texels", ()):
+            self.upload_texels(page, x, y, rgba)
+        if "metadata" in cmds:
+            self.upload_metadata(cmds["metadata"])
+        else:
+            self.upload_paints(cmds["paints"])
         self.upload_scene(cmds["draw"], cmds["clip"])
         for kind, b in cmds["batches"]:
             self.prepare_batch(b)

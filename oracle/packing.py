@@ -132,6 +132,8 @@ class _Batch:
             "propagate_metadata": np.asarray(self.prop, np.int64).astype(np.int32).reshape(-1, 12),
             "backdrops": np.asarray(self.backdrops, np.int32).reshape(-1, 3),
             "clip_batch_id": self.clip_batch_id,
+            # DrawTileBatchD3D11.color_texture: none for solid-colour paints
+            "color_texture_page": -1, "color_texture_sampling_flags": 0,
         }
 
 

@@ -48,6 +48,9 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* ORC_OPT_PLANT_PAINT (0: none); read by the paint functions */
+static int g_plant_paint = 0;
 #endif
 
 #define TILE_SIZE 16
@@ -76,7 +79,14 @@ typedef struct {
     int32_t *ftm;        /* fb tiles */
     uint32_t *alpha;     /* 2 per alpha tile */
     uint32_t n_alpha, alpha_start;
+    int32_t color_page;        /* colour texture 0 of a draw batch, -1 none */
+    uint32_t color_flags;      /* its TextureSamplingFlags */
 } orc_batch;
+
+typedef struct {
+    int32_t width, height;     /* 0: not allocated */
+    uint8_t *rgba;
+} orc_page;
 
 struct orc_ctx {
     float lut[256 * 256 * 4]; /* texel (u, v) channel c at (v * 256 + u) * 4 + c */
@@ -84,6 +94,8 @@ struct orc_ctx {
     int has_background, dest_dirty;
     float background[4];
     uint16_t *paints; uint32_t paint_count;   /* f16 rgba per paint */
+    float *metadata; uint32_t metadata_count; /* 40 f16-rounded floats per entry */
+    orc_page *pages; uint32_t page_count;
     float *draw_points; uint32_t n_draw_points;
     uint32_t *draw_indices; uint32_t n_draw_segments;
     float *clip_points; uint32_t n_clip_points;
@@ -212,6 +224,9 @@ orc_ctx *orc_create(const uint8_t *lut) {
 void orc_destroy(orc_ctx *ctx) {
     if (!ctx) return;
     orc_end_frame(ctx);
+    free(ctx->metadata);
+    for (uint32_t i = 0; i < ctx->page_count; i++) free(ctx->pages[i].rgba);
+    free(ctx->pages);
     free(ctx->paints); free(ctx->draw_points); free(ctx->draw_indices);
     free(ctx->clip_points); free(ctx->clip_indices); free(ctx->mask); free(ctx->fb);
     free(ctx);
@@ -220,6 +235,7 @@ void orc_destroy(orc_ctx *ctx) {
 void orc_set_option(orc_ctx *ctx, int option, int value) {
     if (option == ORC_OPT_CLIP_STRIDE_LITERAL) ctx->opt_clip_stride_literal = value;
     if (option == ORC_OPT_PLANT_DICE) ctx->opt_plant_dice = value;
+    if (option == ORC_OPT_PLANT_PAINT) g_plant_paint = value;
 }
 
 void orc_begin_frame(orc_ctx *ctx, int32_t width, int32_t height, int32_t has_background,
@@ -242,6 +258,338 @@ void orc_upload_paints(orc_ctx *ctx, const uint8_t *rgba8, uint32_t n) {
     ctx->paint_count = n;
     const float inv255 = 1.0f / 255.0f; /* ColorU::to_f32 multiplies by splat(1/255) */
     for (uint32_t i = 0; i < 4 * n; i++) ctx->paints[i] = orc_f32_to_f16((float)rgba8[i] * inv255);
+    /* a solid paint's TextureMetadataEntry (paint.rs:641-659): identity colour
+     * transform, no combine, no filter, SrcOver */
+    orc_metadata_entry *e = (orc_metadata_entry *)calloc(n ? n : 1, sizeof(orc_metadata_entry));
+    for (uint32_t i = 0; i < n; i++) {
+        e[i].transform[0] = 1.0f;
+        e[i].transform[3] = 1.0f;
+        memcpy(e[i].base_color, rgba8 + 4 * (size_t)i, 4);
+    }
+    orc_upload_metadata(ctx, e, n);
+    free(e);
+}
+
+/* ---------------------------------------------------------------- paints */
+
+/* BlendMode::to_composite_ctrl (gpu/blend.rs:172-202), PFH_BLEND_MODE_* numbering. */
+static int composite_ctrl(int32_t blend_mode) {
+    static const int table[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, /* SrcOver .. Lighter */
+                                  4, 5, 1, 2, 8, 3, 6, 7, 9, 0xa, 0xb, 0xc, 0xd, 0xe, 0xf};
+    return blend_mode >= 0 && blend_mode < 27 ? table[blend_mode] : 0;
+}
+
+int orc_upload_metadata(orc_ctx *ctx, const orc_metadata_entry *entries, uint32_t n) {
+    float *m = (float *)calloc((size_t)(n ? n : 1) * 40, sizeof(float));
+    const float inv255 = 1.0f / 255.0f;
+    for (uint32_t i = 0; i < n; i++) {
+        const orc_metadata_entry *e = &entries[i];
+        float t[40];
+        memset(t, 0, sizeof(t));
+        /* texels 0-1: the colour transform (m11 m21 m12 m22 | m13 m23) */
+        t[0] = e->transform[0]; t[1] = e->transform[2]; t[2] = e->transform[1];
+        if (g_plant_paint == 1) { t[1] = e->transform[1]; t[2] = e->transform[2]; }
+        t[3] = e->transform[3]; t[4] = e->transform[4]; t[5] = e->transform[5];
+        for (int c = 0; c < 4; c++) t[8 + c] = (float)e->base_color[c] * inv255;
+        /* compute_filter_params (gpu/renderer.rs:967-1049) */
+        int ctrl = (composite_ctrl(e->blend_mode) << 10) | ((e->combine & 3) << 8);
+        const float *p = e->filter_params;
+        switch (e->filter_kind) {
+        case 0: break;
+        case 1: /* RadialGradient: p0 = line.from, line.vector; p1 = radii, uv_origin */
+            t[12] = p[0]; t[13] = p[1]; t[14] = p[2] - p[0]; t[15] = p[3] - p[1];
+            t[16] = p[4]; t[17] = p[5]; t[18] = p[6]; t[19] = p[7];
+            ctrl |= 1 << 4;
+            break;
+        case 2: { /* Blur */
+            const float sqrt_2_pi_inv = 0.3989422804014327f;
+            float sigma_inv = 1.0f / p[1];
+            float gx = sqrt_2_pi_inv * sigma_inv;
+            float gy = expf(-0.5f * sigma_inv * sigma_inv);
+            float gz = gy * gy;
+            int y_dir = p[0] != 0.0f;
+            t[12] = y_dir ? 0.0f : 1.0f; t[13] = y_dir ? 1.0f : 0.0f;
+            t[14] = ceilf(1.5f * p[1]) * 2.0f;
+            t[16] = gx; t[17] = gy; t[18] = gz;
+            ctrl |= 3 << 4;
+            break;
+        }
+        case 3: /* ColorMatrix: p0..p4 = the five columns */
+            for (int k = 0; k < 20; k++) t[12 + k] = p[k];
+            ctrl |= 4 << 4;
+            break;
+        default:
+            free(m);
+            return -1;
+        }
+        t[32] = (float)ctrl;
+        for (int k = 0; k < 40; k++) m[(size_t)i * 40 + k] = orc_f16_to_f32(orc_f32_to_f16(t[k]));
+    }
+    free(ctx->metadata);
+    ctx->metadata = m;
+    ctx->metadata_count = n;
+    return 0;
+}
+
+void orc_allocate_page(orc_ctx *ctx, uint32_t page, int32_t width, int32_t height) {
+    if (page >= ctx->page_count) {
+        uint32_t n = page + 1;
+        ctx->pages = (orc_page *)realloc(ctx->pages, n * sizeof(orc_page));
+        memset(ctx->pages + ctx->page_count, 0, (n - ctx->page_count) * sizeof(orc_page));
+        ctx->page_count = n;
+    }
+    orc_page *pg = &ctx->pages[page];
+    free(pg->rgba);
+    pg->width = width > 0 ? width : 0;
+    pg->height = height > 0 ? height : 0;
+    pg->rgba = (uint8_t *)calloc((size_t)(pg->width ? pg->width : 1) * (pg->height ? pg->height : 1), 4);
+}
+
+int orc_upload_texels(orc_ctx *ctx, uint32_t page, int32_t x, int32_t y, int32_t w, int32_t h,
+                      const uint8_t *rgba8) {
+    if (page >= ctx->page_count || !ctx->pages[page].rgba) return -1;
+    orc_page *pg = &ctx->pages[page];
+    if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > pg->width || y + h > pg->height) return -1;
+    for (int32_t r = 0; r < h; r++)
+        memcpy(pg->rgba + ((size_t)(y + r) * pg->width + x) * 4, rgba8 + (size_t)r * w * 4,
+               (size_t)w * 4);
+    return 0;
+}
+
+int orc_set_batch_color_texture(orc_ctx *ctx, uint32_t batch_id, int32_t page, uint32_t flags) {
+    orc_batch *b = find_batch(ctx, batch_id);
+    if (!b) return -1;
+    b->color_page = page;
+    b->color_flags = flags;
+    return 0;
+}
+
+typedef struct {
+    const orc_page *page;  /* NULL: no texture (samples are 0) */
+    uint32_t flags;
+} orc_tex;
+
+static inline float lerp_fma(float w, float a, float b) { return fmaf(w, b - a, a); }
+static inline float unorm8_load(uint8_t c) { return (float)c / 255.0f; }
+
+/* floor(x) as an index, saturated to +-1e9 first (NaN to -1e9) so that a
+ * wild coordinate converts the same way on every target */
+static inline int32_t floor_index(float x) {
+    return (int32_t)fminf(fmaxf(floorf(x), -1e9f), 1e9f);
+}
+
+static inline int32_t wrap_index(int32_t i, int32_t n, int repeat) {
+    if (repeat) {
+        int32_t m = i % n;
+        return m < 0 ? m + n : m;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+/* texture(colorTexture, uv) on an RGBA8 page: GL_LINEAR (or GL_NEAREST with
+ * the nearest flags), REPEAT or CLAMP_TO_EDGE per axis; the bilinear form is
+ * the one the area LUT samples with (lerp(w, a, b) = fma(w, b - a, a)). */
+static void tex_sample(const orc_tex *t, float u, float v, float out[4]) {
+    for (int c = 0; c < 4; c++) out[c] = 0.0f;
+    if (!t->page || !t->page->width || !t->page->height) return;
+    const orc_page *pg = t->page;
+    int ru = (t->flags & 1) != 0, rv = (t->flags & 2) != 0;
+    float W = (float)pg->width, H = (float)pg->height;
+    if (t->flags & 8) {
+        int32_t i = wrap_index(floor_index(u * W), pg->width, ru);
+        int32_t j = wrap_index(floor_index(v * H), pg->height, rv);
+        const uint8_t *p = pg->rgba + ((size_t)j * pg->width + i) * 4;
+        for (int c = 0; c < 4; c++) out[c] = unorm8_load(p[c]);
+        return;
+    }
+    float sx = u * W - 0.5f, sy = v * H - 0.5f;
+    float fx = floorf(sx), fy = floorf(sy);
+    float a = sx - fx, b = sy - fy;
+    if (g_plant_paint == 2) { float w = a; a = b; b = w; }
+    int32_t ix = floor_index(sx), iy = floor_index(sy);
+    int32_t i0 = wrap_index(ix, pg->width, ru), i1 = wrap_index(ix + 1, pg->width, ru);
+    int32_t j0 = wrap_index(iy, pg->height, rv), j1 = wrap_index(iy + 1, pg->height, rv);
+    const uint8_t *p00 = pg->rgba + ((size_t)j0 * pg->width + i0) * 4;
+    const uint8_t *p10 = pg->rgba + ((size_t)j0 * pg->width + i1) * 4;
+    const uint8_t *p01 = pg->rgba + ((size_t)j1 * pg->width + i0) * 4;
+    const uint8_t *p11 = pg->rgba + ((size_t)j1 * pg->width + i1) * 4;
+    for (int c = 0; c < 4; c++) {
+        float A = lerp_fma(a, unorm8_load(p00[c]), unorm8_load(p10[c]));
+        float B = lerp_fma(a, unorm8_load(p01[c]), unorm8_load(p11[c]));
+        out[c] = lerp_fma(b, A, B);
+    }
+}
+
+#define FRAC_6_PI_F 1.9098593171027443f
+#define FRAC_PI_3_F 1.0471975511965976f
+
+static inline float comp_div(float n, float d) { return d != 0.0f ? n / d : 0.0f; }
+static inline float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
+
+static void rgb_to_hsl(const float rgb[3], float hsl[3]) {
+    float v = maxf_glsl(maxf_glsl(rgb[0], rgb[1]), rgb[2]);
+    float xmin = minf_glsl(minf_glsl(rgb[0], rgb[1]), rgb[2]);
+    float c = v - xmin, l = mixf(xmin, v, 0.5f);
+    float tx, ty, tz;
+    if (rgb[0] == v) { tx = 0.0f; ty = rgb[1]; tz = rgb[2]; }
+    else if (rgb[1] == v) { tx = 2.0f; ty = rgb[2]; tz = rgb[0]; }
+    else { tx = 4.0f; ty = rgb[0]; tz = rgb[1]; }
+    hsl[0] = FRAC_PI_3_F * comp_div((tx * c + ty) - tz, c);
+    hsl[1] = comp_div(c, v);
+    hsl[2] = l;
+}
+
+static void hsl_to_rgb(const float hsl[3], float rgb[3]) {
+    float a = hsl[1] * minf_glsl(hsl[2], 1.0f - hsl[2]);
+    static const float k0[3] = {0.0f, 8.0f, 4.0f};
+    for (int i = 0; i < 3; i++) {
+        float ks = glsl_mod(k0[i] + hsl[0] * FRAC_6_PI_F, 12.0f);
+        float m = minf_glsl(ks - 3.0f, 9.0f - ks);
+        rgb[i] = hsl[2] - clampf_glsl(m, -1.0f, 1.0f) * a;
+    }
+}
+
+static inline float screen1(float d, float s) { return (d + s) - d * s; }
+static inline float hard_light1(float d, float s) {
+    return s <= 0.5f ? (d * 2.0f) * s : screen1(d, 2.0f * s - 1.0f);
+}
+static inline float dodge1(float d, float s) {
+    return d == 0.0f ? 0.0f : (s == 1.0f ? 1.0f : d / (1.0f - s));
+}
+
+/* compositeRGB (tile_fragment.inc.glsl:465-501). */
+static void composite_rgb(const float d[3], const float s[3], int op, float out[3]) {
+    if (op >= 0xc) {
+        float dh[3], sh[3], mix3[3];
+        rgb_to_hsl(d, dh);
+        rgb_to_hsl(s, sh);
+        switch (op) {
+        case 0xc: mix3[0] = g_plant_paint == 5 ? dh[0] : sh[0]; mix3[1] = dh[1]; mix3[2] = dh[2]; break;
+        case 0xd: mix3[0] = dh[0]; mix3[1] = sh[1]; mix3[2] = dh[2]; break;
+        case 0xe: mix3[0] = sh[0]; mix3[1] = sh[1]; mix3[2] = dh[2]; break;
+        default:  mix3[0] = dh[0]; mix3[1] = dh[1]; mix3[2] = sh[2]; break;
+        }
+        hsl_to_rgb(mix3, out);
+        return;
+    }
+    for (int i = 0; i < 3; i++) {
+        float x = d[i], y = s[i], r;
+        switch (op) {
+        case 0x1: r = x * y; break;
+        case 0x2: r = screen1(x, y); break;
+        case 0x3: r = g_plant_paint == 3 ? hard_light1(x, y) : hard_light1(y, x); break;
+        case 0x4: r = minf_glsl(x, y); break;
+        case 0x5: r = maxf_glsl(x, y); break;
+        case 0x6: r = dodge1(x, y); break;
+        case 0x7: r = 1.0f - dodge1(1.0f - x, 1.0f - y); break;
+        case 0x8: r = hard_light1(x, y); break;
+        case 0x9: {
+            float dark = x <= 0.25f ? ((16.0f * x - 12.0f) * x + 4.0f) * x : sqrtf(x);
+            float factor = y <= 0.5f ? x * (1.0f - x) : dark - x;
+            r = x + (y * 2.0f - 1.0f) * factor;
+            break;
+        }
+        case 0xa: r = fabsf(x - y); break;
+        case 0xb: r = (x + y) - (2.0f * x) * y; break;
+        default: r = y; break;
+        }
+        out[i] = r;
+    }
+}
+
+/* calculateColor (tile_fragment.inc.glsl:567-614) for one pixel and layer,
+ * after the mask: the premultiplied source colour. m is the entry's 40
+ * metadata values; (fx, fy) the pixel centre. */
+static void calculate_color(const orc_ctx *ctx, const orc_tex *tex, const float *m, float fx,
+                            float fy, float mask_alpha, float out[4]) {
+    float color[4] = {m[8], m[9], m[10], m[11]};
+    int ctrl = (int)m[32];
+    int combine = (ctrl >> 8) & 0x3;
+    if (combine != 0) {
+        /* computeTileVaryings: mat2(texel 0) * fragCoord + texel 1 */
+        float u = (m[0] * fx + m[2] * fy) + m[4];
+        float v = (m[1] * fx + m[3] * fy) + m[5];
+        int filter = (ctrl >> 4) & 0xf;
+        float c0[4];
+        if (filter == 1) {
+            /* filterRadialGradient (:280-306) */
+            float dpx = u - m[12], dpy = v - m[13], dcx = m[14], dcy = m[15];
+            float dr = m[17] - m[16];
+            float a = (dcx * dcx + dcy * dcy) - dr * dr;
+            float b = (dpx * dcx + dpy * dcy) + m[16] * dr;
+            float c = (dpx * dpx + dpy * dpy) - m[16] * m[16];
+            float discrim = b * b - a * c;
+            for (int k = 0; k < 4; k++) c0[k] = 0.0f;
+            if (discrim != 0.0f) {
+                float sq = sqrtf(discrim);
+                float t0 = (sq + b) / a, t1 = (-sq + b) / a;
+                if (t0 > t1) { float w = t0; t0 = t1; t1 = w; }
+                float t = t0 >= 0.0f && g_plant_paint != 6 ? t0 : t1;
+                tex_sample(tex, m[18] + t, m[19] + 0.0f, c0);
+            }
+        } else if (filter == 3) {
+            /* filterBlur (:313-351) */
+            float W = tex->page ? (float)tex->page->width : 0.0f;
+            float H = tex->page ? (float)tex->page->height : 0.0f;
+            float ox = m[12] / W, oy = m[13] / H;
+            int support = (int)m[14];
+            float gx = m[16], gy = m[17], gz = m[18];
+            float sum = gx;
+            tex_sample(tex, u, v, c0);
+            for (int k = 0; k < 4; k++) c0[k] *= gx;
+            { float nx = gx * gy, ny = gy * gz; gx = nx; gy = ny; }
+            for (int i = 1; i <= support; i += 2) {
+                float partial = gx;
+                { float nx = gx * gy, ny = gy * gz; gx = nx; gy = ny; }
+                partial += gx;
+                float k = (float)i + gx / partial;
+                float sx = ox * k, sy = oy * k;
+                float lo[4], hi[4];
+                tex_sample(tex, u - sx, v - sy, lo);
+                tex_sample(tex, u + sx, v + sy, hi);
+                for (int q = 0; q < 4; q++) c0[q] += (lo[q] + hi[q]) * partial;
+                sum += (g_plant_paint == 4 ? 1.0f : 2.0f) * partial;
+                { float nx = gx * gy, ny = gy * gz; gx = nx; gy = ny; }
+            }
+            for (int k = 0; k < 4; k++) c0[k] /= sum;
+        } else if (filter == 4) {
+            /* filterColorMatrix (:353-363): mat4(p0..p3) * src + p4 */
+            float src[4];
+            tex_sample(tex, u, v, src);
+            for (int k = 0; k < 4; k++)
+                c0[k] = g_plant_paint == 7
+                            ? (((m[12 + 4 * k] * src[0] + m[13 + 4 * k] * src[1]) +
+                                m[14 + 4 * k] * src[2]) + m[15 + 4 * k] * src[3]) + m[28 + k]
+                            : (((m[12 + k] * src[0] + m[16 + k] * src[1]) + m[20 + k] * src[2]) +
+                               m[24 + k] * src[3]) + m[28 + k];
+        } else {
+            tex_sample(tex, u, v, c0);
+        }
+        /* combineColor0 (:81-89) */
+        if (combine == 1 || (combine == 2 && g_plant_paint == 8)) {
+            float a = c0[3] * color[3];
+            color[0] = c0[0]; color[1] = c0[1]; color[2] = c0[2]; color[3] = a;
+        } else if (combine == 2) {
+            color[3] = c0[3] * color[3];
+        }
+    }
+    color[3] *= mask_alpha;
+    int op = (ctrl >> 10) & 0xf;
+    if (op != 0) {
+        /* composite (:508-523): the destination is sampled from colour
+         * texture 0 at fragCoord / framebufferSize, the texture the D3D11
+         * tile shader binds there (d3d11/tile.cs.glsl:135-152) */
+        float dest[4], blended[3];
+        tex_sample(tex, fx / (float)ctx->width, fy / (float)ctx->height, dest);
+        composite_rgb(dest, color, op, blended);
+        float sa = color[3];
+        for (int k = 0; k < 3; k++)
+            color[k] = ((sa * (1.0f - dest[3])) * color[k] + (sa * dest[3]) * blended[k]) +
+                       (1.0f - sa) * dest[k];
+        color[3] = 1.0f;
+    }
+    for (int k = 0; k < 3; k++) color[k] *= color[3];
+    for (int k = 0; k < 4; k++) out[k] = color[k];
 }
 
 static void *dup(const void *src, size_t bytes) {
@@ -774,6 +1122,10 @@ static void stage_tile(orc_ctx *ctx, orc_batch *b) {
     int32_t fbw = ctx->fbw, fbh = ctx->fbh;
     int64_t fbn = (int64_t)fbw * fbh;
     int64_t pitch = (int64_t)fbw * 16;
+    orc_tex tex = {NULL, b->color_flags};
+    if (b->color_page >= 0 && (uint32_t)b->color_page < ctx->page_count &&
+        ctx->pages[b->color_page].rgba)
+        tex.page = &ctx->pages[b->color_page];
 #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 8)
     for (int64_t g = 0; g < fbn; g++) {
         int32_t first = b->ftm[g];
@@ -803,9 +1155,6 @@ static void stage_tile(orc_ctx *ctx, orc_batch *b) {
                         backdrop = (float)((int32_t)word >> 24);
                         ctrl &= ~0x3;
                     }
-                    float base[4];
-                    for (int c = 0; c < 4; c++)
-                        base[c] = color < ctx->paint_count ? orc_f16_to_f32(ctx->paints[color * 4 + c]) : 0.0f;
                     float mask_alpha = 1.0f;
                     int mask_ctrl = ctrl & 0x3;
                     if (mask_ctrl != 0) {
@@ -819,9 +1168,12 @@ static void stage_tile(orc_ctx *ctx, orc_batch *b) {
                         }
                         mask_alpha = minf_glsl(mask_alpha, cov);
                     }
-                    float a = base[3] * mask_alpha;
-                    float s[4] = {base[0] * a, base[1] * a, base[2] * a, a};
-                    float ia = 1.0f - a;
+                    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (color < ctx->metadata_count)
+                        calculate_color(ctx, &tex, ctx->metadata + (size_t)color * 40,
+                                        (float)(gx * 16 + lx) + 0.5f,
+                                        (float)(gy * 16 + py) + 0.5f, mask_alpha, s);
+                    float ia = 1.0f - s[3];
                     for (int c = 0; c < 4; c++) d[c] = d[c] * ia + s[c];
                     t = (int32_t)tile[0];
                 }
@@ -852,6 +1204,7 @@ int orc_prepare_batch(orc_ctx *ctx, uint32_t batch_id, uint32_t path_source, uin
     }
     memset(b, 0, sizeof(*b));
     b->used = 1;
+    b->color_page = -1;
     b->batch_id = batch_id; b->path_source = path_source; b->path_count = path_count;
     b->tile_count = tile_count; b->segment_count = segment_count; b->column_count = column_count;
     b->clip_batch_id = clip_batch_id;

@@ -53,7 +53,13 @@ enum {
     /* Planted mistakes, for tests that show a pin catches them (never set
      * outside those tests): 1 = twice dice's segment count, 2 = half of it
      * (rounded up), 3 = the last mix of sampleCurve with its ends swapped. */
-    ORC_OPT_PLANT_DICE = 2
+    ORC_OPT_PLANT_DICE = 2,
+    /* Planted paint mistakes, likewise: 1 = colour transform transposed,
+     * 2 = bilinear weights of the two axes swapped, 3 = overlay's operands
+     * unswapped, 4 = blur sum without the pair factor 2, 5 = hue taken from
+     * the destination, 6 = radial gradient always on the larger root,
+     * 7 = colour matrix transposed, 8 = combine DestIn as SrcIn. */
+    ORC_OPT_PLANT_PAINT = 3
 };
 
 /* Per-stage wall-clock seconds (steady clock) accumulated since
@@ -82,6 +88,38 @@ void orc_begin_frame(orc_ctx *ctx, int32_t width, int32_t height, int32_t has_ba
  * ColorU::to_f32 (color/src/lib.rs:70-73) and f16::from_f32
  * (gpu/renderer.rs:712-727). */
 void orc_upload_paints(orc_ctx *ctx, const uint8_t *rgba8, uint32_t paint_count);
+
+/* TextureMetadataEntry (renderer/src/gpu_data.rs) as the seam takes it;
+ * byte-identical to pfh_texture_metadata_entry. filter_kind: 0 none,
+ * 1 radial gradient (params: line from x, y, to x, y, radii r0, r1, uv
+ * origin x, y), 2 blur (params: direction 0 x / 1 y, sigma), 3 colour
+ * matrix (params: the five columns). combine: 0 none, 1 src-in, 2 dest-in.
+ * blend_mode: PFH_BLEND_MODE_* numbering (SrcOver 0). */
+typedef struct orc_metadata_entry {
+    float transform[6];  /* m00, m01, m10, m11, tx, ty: device pixel -> colour UV */
+    int32_t combine;
+    uint8_t base_color[4];
+    int32_t filter_kind;
+    float filter_params[20];
+    int32_t blend_mode;
+} orc_metadata_entry;
+
+/* UploadTextureMetadata (gpu/renderer.rs:702-775 with compute_filter_params
+ * :967-1049): each entry becomes 10 RGBA16F texels. Returns 0, or -1 for an
+ * unknown filter. */
+int orc_upload_metadata(orc_ctx *ctx, const orc_metadata_entry *entries, uint32_t count);
+
+/* AllocateTexturePage / UploadTexelData (gpu_data.rs:37-105): RGBA8 colour
+ * texture pages, zeroed when allocated. */
+void orc_allocate_page(orc_ctx *ctx, uint32_t page, int32_t width, int32_t height);
+int orc_upload_texels(orc_ctx *ctx, uint32_t page, int32_t x, int32_t y, int32_t width,
+                      int32_t height, const uint8_t *rgba8);
+
+/* The colour texture of a draw batch (DrawTileBatchD3D11.color_texture:
+ * TileBatchTexture {page, sampling_flags}); page < 0 for none. Sampling
+ * flags: 1 repeat u, 2 repeat v, 4 nearest min, 8 nearest mag. */
+int orc_set_batch_color_texture(orc_ctx *ctx, uint32_t batch_id, int32_t page,
+                                uint32_t sampling_flags);
 
 /* RendererD3D11::upload_scene (d3d11/renderer.rs:236-241). indices are
  * (first_point_index, flags) pairs (SegmentIndicesD3D11). */

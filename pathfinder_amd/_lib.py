@@ -49,6 +49,10 @@ DEBUG_BACKDROPS = 6
 DEBUG_ALPHA_RANGE = 7
 DEBUG_COVERAGE = 8
 
+SAMPLING_REPEAT_U, SAMPLING_REPEAT_V, SAMPLING_NEAREST_MIN, SAMPLING_NEAREST_MAG = 1, 2, 4, 8
+COMBINE_NONE, COMBINE_SRC_IN, COMBINE_DEST_IN = 0, 1, 2
+FILTER_NONE, FILTER_RADIAL_GRADIENT, FILTER_BLUR, FILTER_COLOR_MATRIX = 0, 1, 2, 3
+
 NOALLOC_SCRATCH_BYTES = 64 << 20  # PFH_NOALLOC_SCRATCH_BYTES
 
 
@@ -102,7 +106,23 @@ class TileBatch(C.Structure):
                 ("segment_count", C.c_uint32), ("transform", Transform2F),
                 ("dice_metadata", C.c_void_p), ("tile_path_info", C.c_void_p),
                 ("propagate_metadata", C.c_void_p), ("backdrops", C.c_void_p),
-                ("backdrop_count", C.c_uint32), ("clip_batch_id", C.c_int32)]
+                ("backdrop_count", C.c_uint32), ("clip_batch_id", C.c_int32),
+                ("color_texture_page", C.c_int32),
+                ("color_texture_sampling_flags", C.c_uint32)]
+
+
+class Filter(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("params", C.c_float * 20)]
+
+
+class TextureMetadataEntry(C.Structure):
+    _fields_ = [("color_0_transform", Transform2F), ("color_0_combine_mode", C.c_int32),
+                ("base_color", ColorU), ("filter", Filter), ("blend_mode", C.c_int32)]
+
+
+class RectI(C.Structure):
+    _fields_ = [("min_x", C.c_int32), ("min_y", C.c_int32), ("max_x", C.c_int32),
+                ("max_y", C.c_int32)]
 
 
 class ContextOptions(C.Structure):
@@ -186,6 +206,9 @@ SIGNATURES = {
     "pfh_framebuffer_to_float": (I32, [P, P]),
     "pfh_backward": (I32, [P, P, P, U32, P, U32]),
     "pfh_backward_noalloc": (I32, [P, P, P, U32, P, U32]),
+    "pfh_upload_texture_metadata_entries": (I32, [P, C.POINTER(TextureMetadataEntry), U32]),
+    "pfh_allocate_texture_page": (I32, [P, U32, I32, I32]),
+    "pfh_upload_texel_data": (I32, [P, U32, C.POINTER(RectI), P]),
     "pfh_scene_create": (P, []),
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),

@@ -20,6 +20,8 @@ struct BatchData {
     std::vector<pfh_propagate_metadata> propagate_metadata;
     std::vector<pfh_backdrop_info> backdrops;
     int32_t clip_batch_id = -1;
+    int32_t color_texture_page = -1;  // DrawTileBatchD3D11.color_texture
+    uint32_t color_texture_sampling_flags = 0;
 };
 
 inline void batch_view(const BatchData &b, pfh_tile_batch *out) {
@@ -35,6 +37,8 @@ inline void batch_view(const BatchData &b, pfh_tile_batch *out) {
     out->backdrops = b.backdrops.data();
     out->backdrop_count = (uint32_t)b.backdrops.size();
     out->clip_batch_id = b.clip_batch_id;
+    out->color_texture_page = b.color_texture_page;
+    out->color_texture_sampling_flags = b.color_texture_sampling_flags;
 }
 
 }  // namespace pfh

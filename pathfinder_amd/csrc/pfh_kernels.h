@@ -116,8 +116,14 @@ struct TileArgs {
     uint32_t mask_capacity;        // alpha tiles held by `mask`
     const int32_t *first_tile_map;
     const uint32_t *mask;
-    const float4 *paints;          // f16-rounded base colours
+    const float4 *metadata;        // 10 float4 (f16-rounded texels) per paint, the
+                                   // TextureMetadataEntry layout of gpu/renderer.rs:702-775
     uint32_t paint_count;
+    // colour texture 0 of the batch (RGBA8 words, rows of color_width), or null
+    const uint32_t *color_texture;
+    int32_t color_width, color_height;
+    uint32_t color_flags;          // TextureSamplingFlags
+    float fb_width, fb_height;     // the draw viewport in pixels (uFramebufferSize)
     uint32_t *framebuffer;         // fb_width_tiles*16 x fb_height_tiles*16 RGBA8 words
     int32_t fb_width_tiles, fb_height_tiles;
     int32_t load;                  // 1: LOAD_ACTION_LOAD, 0: clear
@@ -200,7 +206,8 @@ void launch_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words, uint3
 // Exclusive scan of n u32 values in place; out[n] receives the total.
 void launch_scan(uint32_t *data, uint32_t n, uint32_t *partials, hipStream_t s);
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s);
-void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s);
+// binary16 metadata texels to floats (exact)
+void launch_metadata_expand(const uint16_t *halves, float4 *metadata, uint32_t texels, hipStream_t s);
 
 // Scratch words needed by the scans of a batch.
 size_t scan_partials_words(uint32_t max_elements);

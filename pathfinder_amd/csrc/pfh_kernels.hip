@@ -2257,6 +2257,254 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ paints
+//
+// calculateColor (tile_fragment.inc.glsl:567-614) for paints that are more
+// than a solid colour: the colour texture (texture() with GL_LINEAR or
+// GL_NEAREST, REPEAT or CLAMP_TO_EDGE; the bilinear form lerp(w, a, b) =
+// fma(w, b - a, a) the area LUT uses too), the radial gradient, blur and
+// colour matrix filters, SrcIn/DestIn combining, and the composite ops.
+// Every operation is the oracle's, in the same order, without contraction.
+
+// floor(x) as an index, saturated to +-1e9 first (NaN to -1e9): every index
+// then stays in range of wrap_index and converts as on the host
+__device__ __forceinline__ int32_t floor_index(float x) {
+    return (int32_t)fminf(fmaxf(floorf(x), -1e9f), 1e9f);
+}
+
+__device__ __forceinline__ int32_t wrap_index(int32_t i, int32_t n, bool repeat) {
+    if (repeat) {
+        int32_t m = i % n;
+        return m < 0 ? m + n : m;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+__device__ __forceinline__ float4 texel_unit(uint32_t v) {
+    return make_float4((float)(v & 0xffu) / 255.0f, (float)((v >> 8) & 0xffu) / 255.0f,
+                       (float)((v >> 16) & 0xffu) / 255.0f, (float)(v >> 24) / 255.0f);
+}
+
+__device__ float4 color_texture_sample(const TileArgs &a, float u, float v) {
+    if (!a.color_texture || a.color_width <= 0 || a.color_height <= 0)
+        return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool ru = (a.color_flags & 1u) != 0, rv = (a.color_flags & 2u) != 0;
+    float W = (float)a.color_width, H = (float)a.color_height;
+    if (a.color_flags & 8u) {
+        int32_t i = wrap_index(floor_index(u * W), a.color_width, ru);
+        int32_t j = wrap_index(floor_index(v * H), a.color_height, rv);
+        return texel_unit(a.color_texture[(size_t)j * a.color_width + i]);
+    }
+    float sx = u * W - 0.5f, sy = v * H - 0.5f;
+    float fx = floorf(sx), fy = floorf(sy);
+    float wa = sx - fx, wb = sy - fy;
+    int32_t ix = floor_index(sx), iy = floor_index(sy);
+    int32_t i0 = wrap_index(ix, a.color_width, ru);
+    int32_t i1 = wrap_index(ix + 1, a.color_width, ru);
+    int32_t j0 = wrap_index(iy, a.color_height, rv);
+    int32_t j1 = wrap_index(iy + 1, a.color_height, rv);
+    float4 t00 = texel_unit(a.color_texture[(size_t)j0 * a.color_width + i0]);
+    float4 t10 = texel_unit(a.color_texture[(size_t)j0 * a.color_width + i1]);
+    float4 t01 = texel_unit(a.color_texture[(size_t)j1 * a.color_width + i0]);
+    float4 t11 = texel_unit(a.color_texture[(size_t)j1 * a.color_width + i1]);
+    float4 A, B, r;
+    A.x = __builtin_fmaf(wa, t10.x - t00.x, t00.x); B.x = __builtin_fmaf(wa, t11.x - t01.x, t01.x);
+    A.y = __builtin_fmaf(wa, t10.y - t00.y, t00.y); B.y = __builtin_fmaf(wa, t11.y - t01.y, t01.y);
+    A.z = __builtin_fmaf(wa, t10.z - t00.z, t00.z); B.z = __builtin_fmaf(wa, t11.z - t01.z, t01.z);
+    A.w = __builtin_fmaf(wa, t10.w - t00.w, t00.w); B.w = __builtin_fmaf(wa, t11.w - t01.w, t01.w);
+    r.x = __builtin_fmaf(wb, B.x - A.x, A.x);
+    r.y = __builtin_fmaf(wb, B.y - A.y, A.y);
+    r.z = __builtin_fmaf(wb, B.z - A.z, A.z);
+    r.w = __builtin_fmaf(wb, B.w - A.w, A.w);
+    return r;
+}
+
+constexpr float kFrac6Pi = 1.9098593171027443f;
+constexpr float kFracPi3 = 1.0471975511965976f;
+
+__device__ __forceinline__ float comp_div(float n, float d) { return d != 0.0f ? n / d : 0.0f; }
+__device__ __forceinline__ float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
+
+__device__ void rgb_to_hsl(const float rgb[3], float hsl[3]) {
+    float v = max_glsl(max_glsl(rgb[0], rgb[1]), rgb[2]);
+    float xmin = min_glsl(min_glsl(rgb[0], rgb[1]), rgb[2]);
+    float c = v - xmin, l = mixf(xmin, v, 0.5f);
+    float tx, ty, tz;
+    if (rgb[0] == v) { tx = 0.0f; ty = rgb[1]; tz = rgb[2]; }
+    else if (rgb[1] == v) { tx = 2.0f; ty = rgb[2]; tz = rgb[0]; }
+    else { tx = 4.0f; ty = rgb[0]; tz = rgb[1]; }
+    hsl[0] = kFracPi3 * comp_div((tx * c + ty) - tz, c);
+    hsl[1] = comp_div(c, v);
+    hsl[2] = l;
+}
+
+__device__ void hsl_to_rgb(const float hsl[3], float rgb[3]) {
+    float a = hsl[1] * min_glsl(hsl[2], 1.0f - hsl[2]);
+    const float k0[3] = {0.0f, 8.0f, 4.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float ks = glsl_mod(k0[i] + hsl[0] * kFrac6Pi, 12.0f);
+        float m = min_glsl(ks - 3.0f, 9.0f - ks);
+        rgb[i] = hsl[2] - clamp_glsl(m, -1.0f, 1.0f) * a;
+    }
+}
+
+__device__ __forceinline__ float screen1(float d, float s) { return (d + s) - d * s; }
+__device__ __forceinline__ float hard_light1(float d, float s) {
+    return s <= 0.5f ? (d * 2.0f) * s : screen1(d, 2.0f * s - 1.0f);
+}
+__device__ __forceinline__ float dodge1(float d, float s) {
+    return d == 0.0f ? 0.0f : (s == 1.0f ? 1.0f : d / (1.0f - s));
+}
+
+// compositeRGB (tile_fragment.inc.glsl:465-501)
+__device__ void composite_rgb(const float d[3], const float s[3], int op, float out[3]) {
+    if (op >= 0xc) {
+        float dh[3], sh[3], m[3];
+        rgb_to_hsl(d, dh);
+        rgb_to_hsl(s, sh);
+        m[0] = (op == 0xc || op == 0xe) ? sh[0] : dh[0];
+        m[1] = (op == 0xd || op == 0xe) ? sh[1] : dh[1];
+        m[2] = op == 0xf ? sh[2] : dh[2];
+        hsl_to_rgb(m, out);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float x = d[i], y = s[i], r;
+        switch (op) {
+        case 0x1: r = x * y; break;
+        case 0x2: r = screen1(x, y); break;
+        case 0x3: r = hard_light1(y, x); break;
+        case 0x4: r = min_glsl(x, y); break;
+        case 0x5: r = max_glsl(x, y); break;
+        case 0x6: r = dodge1(x, y); break;
+        case 0x7: r = 1.0f - dodge1(1.0f - x, 1.0f - y); break;
+        case 0x8: r = hard_light1(x, y); break;
+        case 0x9: {
+            float dark = x <= 0.25f ? ((16.0f * x - 12.0f) * x + 4.0f) * x : sqrtf(x);
+            float factor = y <= 0.5f ? x * (1.0f - x) : dark - x;
+            r = x + (y * 2.0f - 1.0f) * factor;
+            break;
+        }
+        case 0xa: r = fabsf(x - y); break;
+        case 0xb: r = (x + y) - (2.0f * x) * y; break;
+        default: r = y; break;
+        }
+        out[i] = r;
+    }
+}
+
+// calculateColor after the mask for one pixel centre (fx, fy) and a paint
+// whose metadata is m[0..9]: the premultiplied source colour.
+__device__ float4 paint_color(const TileArgs &a, const float4 *m, float fx, float fy,
+                              float mask_alpha) {
+    float4 base = m[2];
+    float col[4] = {base.x, base.y, base.z, base.w};
+    int ctrl = (int)m[8].x;
+    int combine = (ctrl >> 8) & 0x3;
+    if (combine != 0) {
+        float4 t0 = m[0], t1 = m[1];
+        float u = (t0.x * fx + t0.z * fy) + t1.x;
+        float v = (t0.y * fx + t0.w * fy) + t1.y;
+        int filter = (ctrl >> 4) & 0xf;
+        float4 p0 = m[3], p1 = m[4];
+        float4 c0;
+        if (filter == 1) {
+            // filterRadialGradient (tile_fragment.inc.glsl:280-306)
+            float dpx = u - p0.x, dpy = v - p0.y, dcx = p0.z, dcy = p0.w;
+            float dr = p1.y - p1.x;
+            float qa = (dcx * dcx + dcy * dcy) - dr * dr;
+            float qb = (dpx * dcx + dpy * dcy) + p1.x * dr;
+            float qc = (dpx * dpx + dpy * dpy) - p1.x * p1.x;
+            float discrim = qb * qb - qa * qc;
+            c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (discrim != 0.0f) {
+                float sq = sqrtf(discrim);
+                float ta = (sq + qb) / qa, tb = (-sq + qb) / qa;
+                if (ta > tb) {
+                    float w = ta;
+                    ta = tb;
+                    tb = w;
+                }
+                float t = ta >= 0.0f ? ta : tb;
+                c0 = color_texture_sample(a, p1.z + t, p1.w + 0.0f);
+            }
+        } else if (filter == 3) {
+            // filterBlur (tile_fragment.inc.glsl:313-351)
+            float ox = p0.x / (float)a.color_width, oy = p0.y / (float)a.color_height;
+            int support = (int)p0.z;
+            float gx = p1.x, gy = p1.y, gz = p1.z;
+            float sum = gx;
+            c0 = color_texture_sample(a, u, v);
+            c0.x *= gx; c0.y *= gx; c0.z *= gx; c0.w *= gx;
+            {
+                float nx = gx * gy, ny = gy * gz;
+                gx = nx;
+                gy = ny;
+            }
+            for (int i = 1; i <= support; i += 2) {
+                float partial = gx;
+                {
+                    float nx = gx * gy, ny = gy * gz;
+                    gx = nx;
+                    gy = ny;
+                }
+                partial += gx;
+                float k = (float)i + gx / partial;
+                float sx = ox * k, sy = oy * k;
+                float4 lo = color_texture_sample(a, u - sx, v - sy);
+                float4 hi = color_texture_sample(a, u + sx, v + sy);
+                c0.x += (lo.x + hi.x) * partial;
+                c0.y += (lo.y + hi.y) * partial;
+                c0.z += (lo.z + hi.z) * partial;
+                c0.w += (lo.w + hi.w) * partial;
+                sum += 2.0f * partial;
+                {
+                    float nx = gx * gy, ny = gy * gz;
+                    gx = nx;
+                    gy = ny;
+                }
+            }
+            c0.x /= sum; c0.y /= sum; c0.z /= sum; c0.w /= sum;
+        } else if (filter == 4) {
+            // filterColorMatrix (tile_fragment.inc.glsl:353-363)
+            float4 src = color_texture_sample(a, u, v);
+            float4 p2 = m[5], p3 = m[6], p4 = m[7];
+            c0.x = (((p0.x * src.x + p1.x * src.y) + p2.x * src.z) + p3.x * src.w) + p4.x;
+            c0.y = (((p0.y * src.x + p1.y * src.y) + p2.y * src.z) + p3.y * src.w) + p4.y;
+            c0.z = (((p0.z * src.x + p1.z * src.y) + p2.z * src.z) + p3.z * src.w) + p4.z;
+            c0.w = (((p0.w * src.x + p1.w * src.y) + p2.w * src.z) + p3.w * src.w) + p4.w;
+        } else {
+            c0 = color_texture_sample(a, u, v);
+        }
+        // combineColor0 (tile_fragment.inc.glsl:81-89)
+        if (combine == 1) {
+            float al = c0.w * col[3];
+            col[0] = c0.x; col[1] = c0.y; col[2] = c0.z; col[3] = al;
+        } else if (combine == 2) {
+            col[3] = c0.w * col[3];
+        }
+    }
+    col[3] *= mask_alpha;
+    int op = (ctrl >> 10) & 0xf;
+    if (op != 0) {
+        // composite (tile_fragment.inc.glsl:508-523): the destination comes
+        // from colour texture 0 at fragCoord / framebufferSize, the texture
+        // d3d11/tile.cs.glsl:135-152 binds there
+        float4 dv = color_texture_sample(a, fx / a.fb_width, fy / a.fb_height);
+        float dest[4] = {dv.x, dv.y, dv.z, dv.w}, blended[3];
+        composite_rgb(dest, col, op, blended);
+        float sa = col[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            col[k] = ((sa * (1.0f - dest[3])) * col[k] + (sa * dest[3]) * blended[k]) +
+                     (1.0f - sa) * dest[k];
+        col[3] = 1.0f;
+    }
+    return make_float4(col[0] * col[3], col[1] * col[3], col[2] * col[3], col[3]);
+}
+
 // ------------------------------------------------------------------ tile
 
 // tile.cs.glsl:71-163 with calculateColor/sampleMask (tile_fragment.inc.glsl:
@@ -2310,7 +2558,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
             backdrop = (float)((int32_t)w.w >> 24);
             ctrl &= ~0x3;
         }
-        float4 base = color < a.paint_count ? a.paints[color] : make_float4(0, 0, 0, 0);
+        // the paint's metadata: base colour (texel 2) and control word (texel 8)
+        const float4 *md = color < a.paint_count ? a.metadata + (size_t)color * 10 : nullptr;
+        float4 base = md ? md[2] : make_float4(0, 0, 0, 0);
+        bool plain = !md || md[8].x == 0.0f;  // no colour texture, Normal composite
         int32_t mask_ctrl = ctrl & 0x3;
         uint32_t texel = 0;
         if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
@@ -2328,12 +2579,23 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
                 }
                 mask_alpha = min_glsl(mask_alpha, cov);
             }
-            float al = base.w * mask_alpha;
-            float ia = 1.0f - al;
-            d[sub][0] = d[sub][0] * ia + base.x * al;
-            d[sub][1] = d[sub][1] * ia + base.y * al;
-            d[sub][2] = d[sub][2] * ia + base.z * al;
-            d[sub][3] = d[sub][3] * ia + al;
+            if (plain) {
+                float al = base.w * mask_alpha;
+                float ia = 1.0f - al;
+                d[sub][0] = d[sub][0] * ia + base.x * al;
+                d[sub][1] = d[sub][1] * ia + base.y * al;
+                d[sub][2] = d[sub][2] * ia + base.z * al;
+                d[sub][3] = d[sub][3] * ia + al;
+            } else {
+                float fx = (float)(gx * 16 + lx) + 0.5f;
+                float fy = (float)(gy * 16 + ly * 4 + (uint32_t)sub) + 0.5f;
+                float4 c = paint_color(a, md, fx, fy, mask_alpha);
+                float ia = 1.0f - c.w;
+                d[sub][0] = d[sub][0] * ia + c.x;
+                d[sub][1] = d[sub][1] * ia + c.y;
+                d[sub][2] = d[sub][2] * ia + c.z;
+                d[sub][3] = d[sub][3] * ia + c.w;
+            }
         }
         w = wn;
         w_next = wn_next;
@@ -2419,7 +2681,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
             backdrop = (float)((int32_t)w.w >> 24);
             ctrl &= ~0x3;
         }
-        base = color < a.paint_count ? a.paints[color] : make_float4(0, 0, 0, 0);
+        base = color < a.paint_count ? a.metadata[(size_t)color * 10 + 2] : make_float4(0, 0, 0, 0);
         int32_t mask_ctrl = ctrl & 0x3;
         uint32_t texel = 0;
         if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
@@ -2719,15 +2981,12 @@ __global__ void k_lut_expand(const uint8_t *rgba, float4 *lut) {
 }
 
 // ColorU::to_f32 (color/src/lib.rs:70-73) then f16::from_f32 (gpu/renderer.rs:724-727).
-__global__ void k_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count) {
+__global__ void k_metadata_expand(const uint16_t *halves, float4 *metadata, uint32_t texels) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const float inv255 = 1.0f / 255.0f;
-    float c[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        c[k] = __half2float(__float2half_rn((float)rgba[i * 4 + k] * inv255));
-    paints[i] = make_float4(c[0], c[1], c[2], c[3]);
+    if (i >= texels) return;
+    const __half *h = reinterpret_cast<const __half *>(halves) + (size_t)i * 4;
+    metadata[i] = make_float4(__half2float(h[0]), __half2float(h[1]), __half2float(h[2]),
+                              __half2float(h[3]));
 }
 
 }  // namespace
@@ -2918,10 +3177,11 @@ void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s) {
                        rgba, lut);
 }
 
-void launch_paints_expand(const uint8_t *rgba, float4 *paints, uint32_t count, hipStream_t s) {
-    if (count == 0) return;
-    hipLaunchKernelGGL(k_paints_expand, dim3((count + 255) / 256), dim3(256), 0, s, rgba, paints,
-                       count);
+void launch_metadata_expand(const uint16_t *halves, float4 *metadata, uint32_t texels,
+                            hipStream_t s) {
+    if (texels == 0) return;
+    hipLaunchKernelGGL(k_metadata_expand, dim3((texels + 255) / 256), dim3(256), 0, s, halves,
+                       metadata, texels);
 }
 
 }  // namespace pfh

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,87 @@
 extern "C" const uint8_t pfh_area_lut_rgba[];
 
 namespace {
+
+// IEEE binary16 of a float, rounded to nearest even (f16::from_f32 of the
+// half crate, gpu/renderer.rs:712-763): subnormals, overflow to infinity and
+// NaN kept.
+uint16_t f32_to_f16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t exp = (x >> 23) & 0xffu, man = x & 0x7fffffu;
+    if (exp == 0xffu) return (uint16_t)(sign | 0x7c00u | (man ? 0x200u | (man >> 13) : 0u));
+    int32_t e = (int32_t)exp - 127 + 15;
+    if (e >= 0x1f) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        man |= 0x800000u;
+        uint32_t shift = (uint32_t)(14 - e);
+        uint32_t half = man >> shift, rem = man & ((1u << shift) - 1u), mid = 1u << (shift - 1);
+        if (rem > mid || (rem == mid && (half & 1u))) half++;
+        return (uint16_t)(sign | half);
+    }
+    uint32_t half = ((uint32_t)e << 10) | (man >> 13), rem = man & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (half & 1u))) half++;  // may carry into the exponent
+    return (uint16_t)(sign | half);
+}
+
+// BlendMode::to_composite_ctrl (gpu/blend.rs:172-202) for PFH_BLEND_MODE_*.
+int composite_ctrl(int32_t blend_mode) {
+    static const int kTable[PFH_BLEND_MODE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                                     4, 5, 1, 2, 8, 3, 6, 7, 9, 0xa, 0xb, 0xc,
+                                                     0xd, 0xe, 0xf};
+    return blend_mode >= 0 && blend_mode < PFH_BLEND_MODE_COUNT ? kTable[blend_mode] : 0;
+}
+
+// The ten RGBA texels of one TextureMetadataEntry (gpu/renderer.rs:702-775)
+// with compute_filter_params (:967-1049), before binary16 rounding. False
+// for a filter this path has no shader for.
+bool encode_metadata(const pfh_texture_metadata_entry &e, float t[40]) {
+    for (int k = 0; k < 40; k++) t[k] = 0.0f;
+    const pfh_transform2f &m = e.color_0_transform;
+    t[0] = m.m00; t[1] = m.m10; t[2] = m.m01; t[3] = m.m11;  // m11 m21 m12 m22
+    t[4] = m.tx; t[5] = m.ty;                                 // m13 m23
+    const float inv255 = 1.0f / 255.0f;                       // ColorU::to_f32
+    t[8] = (float)e.base_color.r * inv255;
+    t[9] = (float)e.base_color.g * inv255;
+    t[10] = (float)e.base_color.b * inv255;
+    t[11] = (float)e.base_color.a * inv255;
+    int ctrl = (composite_ctrl(e.blend_mode) << 10) | ((e.color_0_combine_mode & 3) << 8);
+    const float *p = e.filter.params;
+    switch (e.filter.kind) {
+    case PFH_FILTER_NONE:
+        break;
+    case PFH_FILTER_RADIAL_GRADIENT:
+        // p0 = (line.from, line.vector), p1 = (radii, uv_origin)
+        t[12] = p[0]; t[13] = p[1]; t[14] = p[2] - p[0]; t[15] = p[3] - p[1];
+        t[16] = p[4]; t[17] = p[5]; t[18] = p[6]; t[19] = p[7];
+        ctrl |= 1 << 4;
+        break;
+    case PFH_FILTER_BLUR: {
+        const float sqrt_2_pi_inv = 0.3989422804014327f;
+        float sigma_inv = 1.0f / p[1];
+        float gy = expf(-0.5f * sigma_inv * sigma_inv);
+        bool y_dir = p[0] != 0.0f;
+        t[12] = y_dir ? 0.0f : 1.0f;
+        t[13] = y_dir ? 1.0f : 0.0f;
+        t[14] = ceilf(1.5f * p[1]) * 2.0f;  // support
+        t[16] = sqrt_2_pi_inv * sigma_inv;
+        t[17] = gy;
+        t[18] = gy * gy;
+        ctrl |= 3 << 4;
+        break;
+    }
+    case PFH_FILTER_COLOR_MATRIX:
+        for (int k = 0; k < 20; k++) t[12 + k] = p[k];
+        ctrl |= 4 << 4;
+        break;
+    default:
+        return false;
+    }
+    t[32] = (float)ctrl;
+    return true;
+}
 
 uint32_t next_pow2(uint64_t v) {
     uint64_t p = 1;
@@ -180,6 +262,8 @@ struct Batch {
     hipEvent_t fill_done = nullptr;   // its masks are written (fill on its side stream)
     int32_t load = 1;
     float clear[4] = {0, 0, 0, 0};
+    int32_t color_page = -1;       // colour texture 0 of a draw batch
+    uint32_t color_flags = 0;
     // device inputs
     pfh_dice_metadata *d_dice = nullptr;
     pfh_tile_path_info *d_tpi = nullptr;
@@ -235,9 +319,15 @@ struct pfh_context {
     EventProfiler prof;
     // tables
     float4 *d_lut = nullptr;
-    float4 *d_paints = nullptr;
+    float4 *d_metadata = nullptr;  // 10 float4 per paint (TextureMetadataEntry texels)
     uint32_t paint_count = 0;
-    std::vector<pfh_color_u> paints_host;
+    bool paints_plain = true;      // every paint's composite control word is zero
+    // colour texture pages (AllocateTexturePage): RGBA8 words, row by row
+    struct Page {
+        int32_t width = 0, height = 0;
+        uint32_t *texels = nullptr;
+    };
+    std::map<uint32_t, Page> pages;
     // scene sources
     std::vector<pfh_vector2f> draw_points_host, clip_points_host;
     std::vector<pfh_segment_indices> draw_idx_host, clip_idx_host;
@@ -502,8 +592,19 @@ pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
     t.mask_capacity = ctx->cap_a;
     t.first_tile_map = b.d_ftm;
     t.mask = ctx->d_mask;
-    t.paints = ctx->d_paints;
+    t.metadata = ctx->d_metadata;
     t.paint_count = ctx->paint_count;
+    t.color_texture = nullptr;
+    t.color_width = t.color_height = 0;
+    t.color_flags = b.color_flags;
+    auto page = b.color_page >= 0 ? ctx->pages.find((uint32_t)b.color_page) : ctx->pages.end();
+    if (page != ctx->pages.end()) {
+        t.color_texture = page->second.texels;
+        t.color_width = page->second.width;
+        t.color_height = page->second.height;
+    }
+    t.fb_width = (float)ctx->width;
+    t.fb_height = (float)ctx->height;
     t.framebuffer = ctx->d_fb;
     t.fb_width_tiles = ctx->fbw;
     t.fb_height_tiles = ctx->fbh;
@@ -731,6 +832,14 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     h.tile_path_info.assign(in->tile_path_info, in->tile_path_info + in->path_count);
     h.propagate_metadata.assign(in->propagate_metadata, in->propagate_metadata + in->path_count);
     h.backdrops.assign(in->backdrops, in->backdrops + in->backdrop_count);
+    h.color_texture_page = in->color_texture_page;
+    h.color_texture_sampling_flags = in->color_texture_sampling_flags;
+    if (draw && in->color_texture_page >= 0 &&
+        !ctx->pages.count((uint32_t)in->color_texture_page))
+        return fail(ctx, PFH_ERROR_STATE, "batch %u: colour texture page %d is not allocated",
+                    in->batch_id, in->color_texture_page);
+    b.color_page = draw ? in->color_texture_page : -1;
+    b.color_flags = in->color_texture_sampling_flags;
     b.draw = draw;
     if (draw) {
         // clear_color_for_draw_operation (gpu/renderer.rs:1190-1211)
@@ -818,6 +927,10 @@ pfh_status backward_check(pfh_context *ctx, const float *grad_image, const float
     if (!draw) return fail(ctx, PFH_ERROR_STATE, "no draw batch in the frame");
     if (draw->load)
         return fail(ctx, PFH_ERROR_STATE, "backward needs a frame drawn over a clear colour");
+    if (!ctx->paints_plain || draw->color_page >= 0)
+        return fail(ctx, PFH_ERROR_STATE,
+                    "backward differentiates solid paints only (no filter, combine, blend "
+                    "mode or colour texture)");
     if (point_count < ctx->draw_points_host.size())
         return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "grad_points holds %u points, the scene %zu",
                     point_count, ctx->draw_points_host.size());
@@ -1044,22 +1157,91 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
     return PFH_OK;
 }
 
+pfh_status pfh_upload_texture_metadata_entries(pfh_context *ctx,
+                                               const pfh_texture_metadata_entry *entries,
+                                               uint32_t count) {
+    if (!ctx || (count && !entries)) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (count > 65536) return fail(ctx, PFH_ERROR_LIMIT, "more than 65536 paints (u16 PaintId)");
+    std::vector<uint16_t> halves((size_t)std::max<uint32_t>(count, 1) * 40, 0);
+    bool plain = true;
+    for (uint32_t i = 0; i < count; i++) {
+        float t[40];
+        if (!encode_metadata(entries[i], t))
+            return fail(ctx, PFH_ERROR_INVALID_ARGUMENT,
+                        "paint %u: unknown filter %d (the text filter is not on this path)", i,
+                        entries[i].filter.kind);
+        for (int k = 0; k < 40; k++) halves[(size_t)i * 40 + k] = f32_to_f16(t[k]);
+        plain = plain && t[32] == 0.0f;
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, ctx->d_metadata);
+    pfh_status s = dalloc(ctx, &ctx->d_metadata, (size_t)std::max<uint32_t>(count, 1) * 10);
+    if (s != PFH_OK) return s;
+    uint16_t *d_halves = nullptr;
+    if ((s = upload(ctx, &d_halves, halves)) != PFH_OK) return s;
+    pfh::launch_metadata_expand(d_halves, ctx->d_metadata, count * 10, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d_halves);
+    ctx->paint_count = count;
+    ctx->paints_plain = plain;
+    drop_graph(ctx);
+    return PFH_OK;
+}
+
 pfh_status pfh_upload_texture_metadata(pfh_context *ctx, const pfh_color_u *colors,
                                        uint32_t count) {
     if (!ctx || (count && !colors)) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
-    if (count > 65536) return fail(ctx, PFH_ERROR_LIMIT, "more than 65536 paints (u16 PaintId)");
-    ctx->paints_host.assign(colors, colors + count);
-    dfree(ctx, ctx->d_paints);
-    pfh_status s = dalloc(ctx, &ctx->d_paints, count);
-    if (s != PFH_OK) return s;
-    uint8_t *d_bytes = nullptr;
-    std::vector<uint8_t> bytes(reinterpret_cast<const uint8_t *>(colors),
-                               reinterpret_cast<const uint8_t *>(colors) + 4 * (size_t)count);
-    if ((s = upload(ctx, &d_bytes, bytes)) != PFH_OK) return s;
-    pfh::launch_paints_expand(d_bytes, ctx->d_paints, count, ctx->stream);
+    // a solid paint's TextureMetadataEntry (paint.rs:641-659): identity colour
+    // transform, no combine, no filter, SrcOver
+    std::vector<pfh_texture_metadata_entry> e(count);
+    for (uint32_t i = 0; i < count; i++) {
+        std::memset(&e[i], 0, sizeof(e[i]));
+        e[i].color_0_transform = pfh_transform2f{1, 0, 0, 1, 0, 0};
+        e[i].base_color = colors[i];
+    }
+    return pfh_upload_texture_metadata_entries(ctx, e.data(), count);
+}
+
+pfh_status pfh_allocate_texture_page(pfh_context *ctx, uint32_t page_id, int32_t width,
+                                     int32_t height) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad texture page size %dx%d", width, height);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    dfree(ctx, d_bytes);
-    ctx->paint_count = count;
+    for (hipStream_t side : ctx->side) HIP_TRY(ctx, hipStreamSynchronize(side));
+    pfh_context::Page &pg = ctx->pages[page_id];
+    dfree(ctx, pg.texels);
+    pg.width = width;
+    pg.height = height;
+    pfh_status s = dalloc(ctx, &pg.texels, (size_t)width * height);
+    if (s != PFH_OK) return s;
+    HIP_TRY(ctx, hipMemsetAsync(pg.texels, 0, (size_t)width * height * 4, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    return PFH_OK;
+}
+
+pfh_status pfh_upload_texel_data(pfh_context *ctx, uint32_t page_id, const pfh_rect_i *rect,
+                                 const pfh_color_u *texels) {
+    if (!ctx || !rect || !texels) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    auto it = ctx->pages.find(page_id);
+    if (it == ctx->pages.end())
+        return fail(ctx, PFH_ERROR_STATE, "texture page %u is not allocated", page_id);
+    const pfh_context::Page &pg = it->second;
+    int32_t w = rect->max_x - rect->min_x, h = rect->max_y - rect->min_y;
+    if (rect->min_x < 0 || rect->min_y < 0 || w < 0 || h < 0 || rect->max_x > pg.width ||
+        rect->max_y > pg.height)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "texel rect outside page %u", page_id);
+    if (!w || !h) return PFH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // on the frame's stream, after the work that may still read the page
+    HIP_TRY(ctx, hipMemcpy2DAsync(pg.texels + (size_t)rect->min_y * pg.width + rect->min_x,
+                                  (size_t)pg.width * 4, texels, (size_t)w * 4, (size_t)w * 4,
+                                  (size_t)h, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return PFH_OK;
 }
 

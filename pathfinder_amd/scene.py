@@ -269,6 +269,15 @@ class PathArrays:
                           cat(bm, np.int32))
 
 
+# TextureMetadataEntry (gpu/renderer.rs:702-775) as pfh_texture_metadata_entry
+# lays it out: colour transform (m00 m01 m10 m11 tx ty), combine mode, base
+# colour, filter kind and 20 parameters, blend mode.
+METADATA_DTYPE = np.dtype([("transform", np.float32, 6), ("combine", np.int32),
+                           ("base_color", np.uint8, 4), ("filter_kind", np.int32),
+                           ("filter_params", np.float32, 20), ("blend_mode", np.int32)])
+assert METADATA_DTYPE.itemsize == C.sizeof(L.TextureMetadataEntry)
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
 
@@ -458,6 +467,8 @@ def batch_to_numpy(tb):
                                     np.int32).reshape(-1, 12),
         "backdrops": _copy(tb.backdrops, tb.backdrop_count * 12, np.int32).reshape(-1, 3),
         "clip_batch_id": tb.clip_batch_id,
+        "color_texture_page": tb.color_texture_page,
+        "color_texture_sampling_flags": tb.color_texture_sampling_flags,
     }
 
 
@@ -478,6 +489,8 @@ def numpy_to_batch(b):
     tb.backdrops = _ptr(keep["backdrops"])
     tb.backdrop_count = len(keep["backdrops"])
     tb.clip_batch_id = b["clip_batch_id"]
+    tb.color_texture_page = int(b.get("color_texture_page", -1))
+    tb.color_texture_sampling_flags = int(b.get("color_texture_sampling_flags", 0))
     return tb, keep
 
 
@@ -557,6 +570,21 @@ class Renderer:
     def upload_texture_metadata(self, rgba):
         a = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 4)
         self._check(self._lib.pfh_upload_texture_metadata(self._h, _ptr(a), len(a)))
+
+    def upload_texture_metadata_entries(self, entries):
+        """entries: a numpy array of dtype METADATA_DTYPE (one per paint)."""
+        a = np.ascontiguousarray(entries, METADATA_DTYPE)
+        self._check(self._lib.pfh_upload_texture_metadata_entries(
+            self._h, C.cast(_ptr(a), C.POINTER(L.TextureMetadataEntry)), len(a)))
+
+    def allocate_texture_page(self, page_id, width, height):
+        self._check(self._lib.pfh_allocate_texture_page(self._h, page_id, width, height))
+
+    def upload_texel_data(self, page_id, x, y, rgba):
+        """rgba: uint8 array of shape (h, w, 4) written at (x, y) of the page."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        r = L.RectI(x, y, x + a.shape[1], y + a.shape[0])
+        self._check(self._lib.pfh_upload_texel_data(self._h, page_id, C.byref(r), _ptr(a)))
 
     def upload_scene(self, draw, clip):
         segs = []

@@ -67,3 +67,24 @@ def assert_bit_exact(report, nonempty=("microlines", "tiles", "framebuffer")):
     for name in nonempty:
         sizes = [v[1] for k, v in report.items() if k == name or k.startswith(name + "[")]
         assert sizes and sum(sizes) > 0, "nothing compared for %s: %r" % (name, report)
+
+
+def product_seam(renderer, cmds):
+    """Drives the renderer seam with a packed command dict (packing.build plus
+    optional "metadata", "pages" and "texels" for colour-texture paints)."""
+    renderer.begin_scene()
+    for page, (w, h) in cmds.get("pages", {}).items():
+        renderer.allocate_texture_page(page, w, h)
+    for page, x, y, rgba in cmds.get("texels", ()):
+        renderer.upload_texel_data(page, x, y, rgba)
+    if "metadata" in cmds:
+        renderer.upload_texture_metadata_entries(cmds["metadata"])
+    else:
+        renderer.upload_texture_metadata(cmds["paints"])
+    renderer.upload_scene(cmds["draw"], cmds["clip"])
+    for kind, b in cmds["batches"]:
+        if kind == 4:
+            renderer.prepare_and_draw_tiles(b)
+        else:
+            renderer.prepare_tiles(b)
+    renderer.end_scene()
