@@ -469,6 +469,58 @@ void pfh_scene_destroy(pfh_scene *scene);
 void pfh_scene_set_view_box(pfh_scene *scene, const pfh_rect_f *view_box);
 /* Scene::push_paint (scene.rs:191, palette dedup paint.rs:381-390). Returns the PaintId. */
 uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color);
+/* Gradient and pattern paints (Paint::from_gradient / from_pattern,
+ * paint.rs:128-146; content/src/gradient.rs, content/src/pattern.rs). The
+ * build renders each gradient into a row of a 256x256 colour texture page and
+ * gives each image its own page (paint.rs:476-560, 789-846), and the paint's
+ * TextureMetadataEntry maps device pixels to those texels (:603-638). */
+typedef struct pfh_color_stop {
+    float offset;
+    pfh_color_u color;
+} pfh_color_stop;
+
+#define PFH_GRADIENT_LINEAR 0
+#define PFH_GRADIENT_RADIAL 1
+#define PFH_GRADIENT_WRAP_CLAMP 0
+#define PFH_GRADIENT_WRAP_REPEAT 1
+
+typedef struct pfh_gradient {
+    int32_t geometry;             /* PFH_GRADIENT_LINEAR or PFH_GRADIENT_RADIAL */
+    pfh_vector2f from, to;        /* the gradient line (radial: between the centres) */
+    float radius0, radius1;       /* radial only */
+    pfh_transform2f transform;    /* radial only: GradientGeometry::Radial.transform */
+    const pfh_color_stop *stops;  /* in Gradient::add order; kept stably sorted by offset */
+    uint32_t stop_count;
+    int32_t wrap;                 /* PFH_GRADIENT_WRAP_* */
+} pfh_gradient;
+
+#define PFH_PATTERN_REPEAT_X 0x1u
+#define PFH_PATTERN_REPEAT_Y 0x2u
+#define PFH_PATTERN_NO_SMOOTHING 0x4u
+
+typedef struct pfh_pattern {
+    int32_t width, height;        /* Image size */
+    const pfh_color_u *pixels;    /* width * height, row by row */
+    pfh_transform2f transform;    /* Pattern::transform */
+    uint32_t flags;               /* PFH_PATTERN_* */
+    pfh_filter filter;            /* NONE, BLUR (params: direction != 0 for y, sigma)
+                                     or COLOR_MATRIX */
+} pfh_pattern;
+
+/* PaintCompositeOp (paint.rs:87-93): how the overlay meets the base colour. */
+#define PFH_PAINT_COMPOSITE_SRC_IN 0
+#define PFH_PAINT_COMPOSITE_DEST_IN 1
+
+/* Push a gradient or pattern paint with a base colour (Paint::from_gradient
+ * and from_pattern use opaque white and SrcIn). Returns the PaintId, deduped
+ * like pfh_scene_push_paint, or 0xFFFFFFFF for an invalid description
+ * (unknown geometry, a pattern filter other than blur or colour matrix, an
+ * empty or oversized image). */
+uint32_t pfh_scene_push_gradient_paint(pfh_scene *scene, const pfh_color_u *base_color,
+                                       const pfh_gradient *gradient, int32_t composite_op);
+uint32_t pfh_scene_push_pattern_paint(pfh_scene *scene, const pfh_color_u *base_color,
+                                      const pfh_pattern *pattern, int32_t composite_op);
+
 /* Scene::push_clip_path (scene.rs:98). clip_path = -1 for none. Returns ClipPathId. */
 uint32_t pfh_scene_push_clip_path(pfh_scene *scene, const pfh_outline *outline, int32_t fill_rule,
                                   int32_t clip_path);
@@ -567,7 +619,9 @@ typedef enum pfh_command_kind {
     PFH_COMMAND_UPLOAD_SCENE_D3D11 = 2,
     PFH_COMMAND_PREPARE_CLIP_TILES_D3D11 = 3,
     PFH_COMMAND_DRAW_TILES_D3D11 = 4,
-    PFH_COMMAND_FINISH = 5
+    PFH_COMMAND_FINISH = 5,
+    PFH_COMMAND_ALLOCATE_TEXTURE_PAGE = 6,
+    PFH_COMMAND_UPLOAD_TEXEL_DATA = 7
 } pfh_command_kind;
 uint32_t pfh_command_list_len(const pfh_command_list *list);
 int32_t pfh_command_kind_at(const pfh_command_list *list, uint32_t index);
@@ -580,6 +634,19 @@ pfh_status pfh_command_segments(const pfh_command_list *list, uint32_t index,
 /* For UPLOAD_TEXTURE_METADATA: base colours, one per paint. */
 pfh_status pfh_command_paints(const pfh_command_list *list, uint32_t index,
                               const pfh_color_u **colors, uint32_t *count);
+/* For UPLOAD_TEXTURE_METADATA: the TextureMetadataEntry of every paint
+ * (create_texture_metadata, paint.rs:640-659). */
+pfh_status pfh_command_texture_metadata(const pfh_command_list *list, uint32_t index,
+                                        const pfh_texture_metadata_entry **entries,
+                                        uint32_t *count);
+/* For ALLOCATE_TEXTURE_PAGE (gpu_data.rs:45-49). */
+pfh_status pfh_command_texture_page(const pfh_command_list *list, uint32_t index,
+                                    uint32_t *page_id, int32_t *width, int32_t *height);
+/* For UPLOAD_TEXEL_DATA (gpu_data.rs:50-54): the page, the rect and its
+ * texels, row by row. */
+pfh_status pfh_command_texel_data(const pfh_command_list *list, uint32_t index,
+                                  uint32_t *page_id, pfh_rect_i *rect,
+                                  const pfh_color_u **texels);
 
 /* Renderer::render_command for every command of the list, between
  * pfh_begin_scene and pfh_end_scene (the caller brackets it). */

@@ -17,6 +17,8 @@ renderer/src/tiles.rs:64-66, all in binary32 like the reference's F32x4 code.
 """
 import numpy as np
 
+from . import palette
+
 F = np.float32
 CONTROL = 0x03
 CUBIC = 0x40000000
@@ -100,6 +102,7 @@ class _Batch:
         self.path_count = self.tile_count = self.segment_count = 0
         self.prop, self.dice, self.tpi, self.backdrops = [], [], [], []
         self.clip_batch_id = -1
+        self.color_texture = None   # (page, sampling flags, composite op)
 
     def push(self, rect, paint, ctrl, global_id, seg_range, clip, z_write):
         """TileBatchDataD3D11::push."""
@@ -132,8 +135,10 @@ class _Batch:
             "propagate_metadata": np.asarray(self.prop, np.int64).astype(np.int32).reshape(-1, 12),
             "backdrops": np.asarray(self.backdrops, np.int32).reshape(-1, 3),
             "clip_batch_id": self.clip_batch_id,
-            # DrawTileBatchD3D11.color_texture: none for solid-colour paints
-            "color_texture_page": -1, "color_texture_sampling_flags": 0,
+            # DrawTileBatchD3D11.color_texture (page -1: none)
+            "color_texture_page": -1 if self.color_texture is None else self.color_texture[0],
+            "color_texture_sampling_flags":
+                0 if self.color_texture is None else self.color_texture[1],
         }
 
 
@@ -263,7 +268,14 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
         return (level, idx)
 
     paints = np.asarray([[c.r, c.g, c.b, c.a] for c in scene.paints], np.uint8).reshape(-1, 4)
+    objects = getattr(scene, "paint_objects", None)
+    if objects is None or len(objects) != len(paints):
+        objects = [_SolidPaint(c) for c in paints]
+    pal = palette.build(objects, t)
     draw_batch = None
+    draw_batches = []
+    batch_tex = None
+    next_batch_id = MAX_CLIP_BATCHES
     for i in range(len(draw)):
         bounds = _xrect(t, draw_bounds[i])
         clipped = _intersect(bounds, vb)
@@ -272,13 +284,26 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
         mode = int(draw.blend_modes[i])
         rule = int(draw.fill_rules[i])
         ctrl = 0x2 if rule == 1 else 0x1
-        if draw_batch is None:
-            draw_batch = _Batch(MAX_CLIP_BATCHES, 0, t)
-        cl = add_clip(int(draw.clip_paths[i]), 0)
         paint = int(draw.paints[i])
+        tex = pal["textures"][paint] if paint < len(paints) else None
+        # fixup_batch_for_new_path_if_possible (builder.rs:914-942, 1227-1243):
+        # a batch adopts the first colour texture; another one breaks it
+        if draw_batch is not None and tex is not None:
+            if batch_tex is None:
+                batch_tex = tex
+                draw_batch.color_texture = tex
+            elif tex != batch_tex:
+                draw_batches.append(draw_batch)
+                draw_batch = None
+        if draw_batch is None:
+            draw_batch = _Batch(next_batch_id, 0, t)
+            next_batch_id += 1
+            batch_tex = tex
+            draw_batch.color_texture = tex
+        cl = add_clip(int(draw.clip_paths[i]), 0)
         # builder.rs:83 occludes = opaque && occludes_backdrop; builder.rs:430-436
         # destructive modes tile the view box
-        opaque = paint < len(paints) and paints[paint, 3] == 255 and mode in OCCLUDING
+        opaque = paint < len(paints) and pal["opaque"][paint] and mode in OCCLUDING
         rect = _tile_bounds(vb if mode in DESTRUCTIVE else clipped)
         draw_batch.push(rect, paint, ctrl, i, draw_ranges[i], cl, opaque)
 
@@ -287,5 +312,15 @@ def build(scene, transform=(1, 0, 0, 1, 0, 0)):
         if b.path_count > 0:
             batches.append((PREPARE_CLIP, b.to_dict()))
     if draw_batch is not None:
-        batches.append((DRAW, draw_batch.to_dict()))
-    return {"paints": paints, "draw": draw_segs, "clip": clip_segs, "batches": batches}
+        draw_batches.append(draw_batch)
+    for b in draw_batches:
+        batches.append((DRAW, b.to_dict()))
+    return {"paints": paints, "metadata": pal["metadata"], "pages": pal["pages"],
+            "texels": pal["texels"], "draw": draw_segs, "clip": clip_segs, "batches": batches}
+
+
+class _SolidPaint:
+    def __init__(self, rgba):
+        self.color = tuple(int(v) for v in rgba)
+        self.overlay = None
+        self.composite_op = 0

@@ -38,6 +38,8 @@ COMMAND_UPLOAD_SCENE_D3D11 = 2
 COMMAND_PREPARE_CLIP_TILES_D3D11 = 3
 COMMAND_DRAW_TILES_D3D11 = 4
 COMMAND_FINISH = 5
+COMMAND_ALLOCATE_TEXTURE_PAGE = 6
+COMMAND_UPLOAD_TEXEL_DATA = 7
 
 DEBUG_MICROLINES = 0
 DEBUG_FILLS = 1
@@ -52,6 +54,11 @@ DEBUG_COVERAGE = 8
 SAMPLING_REPEAT_U, SAMPLING_REPEAT_V, SAMPLING_NEAREST_MIN, SAMPLING_NEAREST_MAG = 1, 2, 4, 8
 COMBINE_NONE, COMBINE_SRC_IN, COMBINE_DEST_IN = 0, 1, 2
 FILTER_NONE, FILTER_RADIAL_GRADIENT, FILTER_BLUR, FILTER_COLOR_MATRIX = 0, 1, 2, 3
+GRADIENT_LINEAR, GRADIENT_RADIAL = 0, 1
+GRADIENT_WRAP_CLAMP, GRADIENT_WRAP_REPEAT = 0, 1
+PATTERN_REPEAT_X, PATTERN_REPEAT_Y, PATTERN_NO_SMOOTHING = 1, 2, 4
+PAINT_COMPOSITE_SRC_IN, PAINT_COMPOSITE_DEST_IN = 0, 1
+INVALID_PAINT = 0xFFFFFFFF
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # PFH_NOALLOC_SCRATCH_BYTES
 
@@ -118,6 +125,21 @@ class Filter(C.Structure):
 class TextureMetadataEntry(C.Structure):
     _fields_ = [("color_0_transform", Transform2F), ("color_0_combine_mode", C.c_int32),
                 ("base_color", ColorU), ("filter", Filter), ("blend_mode", C.c_int32)]
+
+
+class ColorStop(C.Structure):
+    _fields_ = [("offset", C.c_float), ("color", ColorU)]
+
+
+class Gradient(C.Structure):
+    _fields_ = [("geometry", C.c_int32), ("from_", Vector2F), ("to", Vector2F),
+                ("radius0", C.c_float), ("radius1", C.c_float), ("transform", Transform2F),
+                ("stops", C.POINTER(ColorStop)), ("stop_count", C.c_uint32), ("wrap", C.c_int32)]
+
+
+class Pattern(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pixels", C.c_void_p),
+                ("transform", Transform2F), ("flags", C.c_uint32), ("filter", Filter)]
 
 
 class RectI(C.Structure):
@@ -213,6 +235,8 @@ SIGNATURES = {
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),
     "pfh_scene_push_paint": (U32, [P, C.POINTER(ColorU)]),
+    "pfh_scene_push_gradient_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Gradient), I32]),
+    "pfh_scene_push_pattern_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Pattern), I32]),
     "pfh_scene_push_clip_path": (U32, [P, C.POINTER(Outline), I32, I32]),
     "pfh_scene_push_draw_path": (U32, [P, C.POINTER(Outline), U32, I32, I32, I32]),
     "pfh_scene_push_draw_paths": (U32, [P, P, P, U32, P, U32, P, U32, P, P, P, P]),
@@ -232,6 +256,10 @@ SIGNATURES = {
     "pfh_command_batch": (I32, [P, U32, C.POINTER(TileBatch)]),
     "pfh_command_segments": (I32, [P, U32, C.POINTER(Segments), C.POINTER(Segments)]),
     "pfh_command_paints": (I32, [P, U32, C.POINTER(P), C.POINTER(U32)]),
+    "pfh_command_texture_metadata": (I32, [P, U32, C.POINTER(P), C.POINTER(U32)]),
+    "pfh_command_texture_page": (I32, [P, U32, C.POINTER(U32), C.POINTER(I32),
+                                       C.POINTER(I32)]),
+    "pfh_command_texel_data": (I32, [P, U32, C.POINTER(U32), C.POINTER(RectI), C.POINTER(P)]),
     "pfh_render_commands": (I32, [P, P]),
     "pfh_scene_build_and_render": (I32, [P, P, C.POINTER(FrameOptions),
                                          C.POINTER(BuildOptions)]),

@@ -1,8 +1,9 @@
 // pfh_scene.cpp - Scene / SceneBuilder front-end for the D3D11-level path.
 //
-// Mirrors, for 2D transforms and solid paints:
+// Mirrors, for 2D transforms:
 //   Scene::{push_paint, push_draw_path, push_clip_path, set_view_box}
 //       renderer/src/scene.rs:77-226, palette dedup renderer/src/paint.rs:381-390
+//   Palette::build_paint_info (gradients, patterns)   paint.rs:398-437 (pfh_palette.cpp)
 //   SceneBuilder::build                    renderer/src/builder.rs:148-222
 //   BuiltSegments::from_scene / SegmentsD3D11::add_path   builder.rs:777-840
 //   build_tile_batches_for_draw_path_display_item         builder.rs:885-1056
@@ -13,6 +14,7 @@
 //   round_rect_out_to_tile_bounds                         renderer/src/tiles.rs:64-66
 // and emits the same RenderCommand stream (gpu_data.rs:37-105).
 #include "pfh_internal.h"
+#include "pfh_palette.h"
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +22,7 @@
 #include <map>
 #include <memory>
 #include <new>
+#include <string>
 #include <vector>
 
 namespace {
@@ -140,8 +143,9 @@ RectI tile_bounds(const RectF &r) {
 
 struct pfh_scene {
     RectF view_box;
-    std::vector<pfh_color_u> paints;
-    std::map<uint32_t, uint32_t> paint_cache;  // packed rgba -> PaintId
+    std::vector<pfh_color_u> paints;         // base colours, by PaintId
+    std::vector<pfh::PaintDesc> paint_descs;  // the paints, by PaintId
+    std::map<std::string, uint32_t> paint_cache;  // pfh::paint_key -> PaintId
     PathSet draw, clip;
     // pfh_scene_path_arrays: the per-path columns of the last export
     std::vector<uint32_t> ex_path_contour_ends, ex_paints;
@@ -150,12 +154,15 @@ struct pfh_scene {
 
 struct pfh_command {
     int32_t kind;
-    uint32_t batch_slot;  // index into batches for batch commands
+    uint32_t batch_slot;  // index into batches, pages or uploads
 };
 
 struct pfh_command_list {
     std::vector<pfh_command> commands;
     std::vector<pfh_color_u> paints;
+    std::vector<pfh_texture_metadata_entry> metadata;
+    std::vector<pfh::PageSize> pages;
+    std::vector<pfh::TexelUpload> uploads;
     std::vector<pfh_vector2f> draw_points, clip_points;
     std::vector<pfh_segment_indices> draw_indices, clip_indices;
     std::vector<pfh::BatchData> batches;
@@ -346,16 +353,45 @@ void pfh_scene_set_view_box(pfh_scene *scene, const pfh_rect_f *vb) {
     scene->view_box = RectF{vb->origin.x, vb->origin.y, vb->lower_right.x, vb->lower_right.y};
 }
 
-uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color) {
-    if (!scene || !color) return 0;
-    uint32_t key = (uint32_t)color->r | ((uint32_t)color->g << 8) | ((uint32_t)color->b << 16) |
-                   ((uint32_t)color->a << 24);
+namespace {
+// Palette::push_paint (paint.rs:381-390): the id of an equal paint, else the next id.
+uint32_t push_paint_desc(pfh_scene *scene, pfh::PaintDesc &&desc) {
+    std::string key = pfh::paint_key(desc);
     auto it = scene->paint_cache.find(key);
     if (it != scene->paint_cache.end()) return it->second;
     uint32_t id = (uint32_t)scene->paints.size();
-    scene->paints.push_back(*color);
-    scene->paint_cache[key] = id;
+    scene->paints.push_back(desc.base);
+    scene->paint_descs.push_back(std::move(desc));
+    scene->paint_cache.emplace(std::move(key), id);
     return id;
+}
+}  // namespace
+
+uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color) {
+    if (!scene || !color) return 0;
+    pfh::PaintDesc d;
+    d.base = *color;
+    return push_paint_desc(scene, std::move(d));
+}
+
+uint32_t pfh_scene_push_gradient_paint(pfh_scene *scene, const pfh_color_u *base_color,
+                                       const pfh_gradient *gradient, int32_t composite_op) {
+    pfh::PaintDesc d;
+    if (!scene || !base_color || !gradient ||
+        !pfh::paint_from_gradient(*base_color, *gradient, composite_op, &d) ||
+        scene->paints.size() >= 65536)
+        return 0xffffffffu;
+    return push_paint_desc(scene, std::move(d));
+}
+
+uint32_t pfh_scene_push_pattern_paint(pfh_scene *scene, const pfh_color_u *base_color,
+                                      const pfh_pattern *pattern, int32_t composite_op) {
+    pfh::PaintDesc d;
+    if (!scene || !base_color || !pattern ||
+        !pfh::paint_from_pattern(*base_color, *pattern, composite_op, &d) ||
+        scene->paints.size() >= 65536)
+        return 0xffffffffu;
+    return push_paint_desc(scene, std::move(d));
 }
 
 uint32_t pfh_scene_push_clip_path(pfh_scene *scene, const pfh_outline *o, int32_t fill_rule,
@@ -462,8 +498,19 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
 
     list->path_count = (uint32_t)(scene->clip.paths.size() + scene->draw.paths.size());
     list->commands.push_back({PFH_COMMAND_START, 0});
+    // Palette::build_paint_info (paint.rs:398-437): metadata, then a page
+    // for every new texture page, then the texel uploads
+    pfh::PaletteBuild palette;
+    pfh::build_palette(scene->paint_descs, st.transform, &palette);
     list->paints = scene->paints;
+    list->metadata = palette.metadata;
     list->commands.push_back({PFH_COMMAND_UPLOAD_TEXTURE_METADATA, 0});
+    list->pages = palette.pages;
+    for (size_t i = 0; i < list->pages.size(); i++)
+        list->commands.push_back({PFH_COMMAND_ALLOCATE_TEXTURE_PAGE, (uint32_t)i});
+    list->uploads = std::move(palette.uploads);
+    for (size_t i = 0; i < list->uploads.size(); i++)
+        list->commands.push_back({PFH_COMMAND_UPLOAD_TEXEL_DATA, (uint32_t)i});
 
     // BuiltSegments::from_scene: clip paths first, then draw paths (builder.rs:782-797).
     st.clip_seg_begin.resize(scene->clip.paths.size());
@@ -478,11 +525,16 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
                      &draw_seg_begin[i], &draw_seg_end[i]);
     list->commands.push_back({PFH_COMMAND_UPLOAD_SCENE_D3D11, 0});
 
-    // One DrawPaths display item holds every draw path (scene.rs:84-94); solid
-    // paints never break a batch (fixup_batch_for_new_path_if_possible, builder.rs:1221-1243).
+    // One DrawPaths display item holds every draw path (scene.rs:84-94). A
+    // path whose paint has a colour texture other than the batch's starts a
+    // new batch (fixup_batch_for_new_path_if_possible, builder.rs:914-942,
+    // 1227-1243); a batch without one adopts the first path's.
     const uint32_t MAX_CLIP_BATCHES = 32;
+    uint32_t next_batch_id = MAX_CLIP_BATCHES;
     bool have_draw_batch = false;
     pfh::BatchData draw_batch;
+    pfh::TileBatchTexture batch_texture;
+    std::vector<pfh::BatchData> draw_batches;
     for (size_t i = 0; i < scene->draw.paths.size(); i++) {
         const PathRecord &dp = scene->draw.paths[i];
         RectF bounds = xform_rect(st.transform, dp.bounds);
@@ -493,15 +545,30 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
         uint8_t ctrl = dp.fill_rule == PFH_FILL_RULE_EVEN_ODD ? 0x2 : 0x1;
         BuiltPathGPU built{tile_bounds(blend_is_destructive(dp.blend_mode) ? st.view_box : clipped),
                            dp.paint, ctrl};
+        pfh::TileBatchTexture path_texture;
+        if (dp.paint < palette.textures.size()) path_texture = palette.textures[dp.paint];
+        if (have_draw_batch && path_texture.page >= 0) {
+            if (batch_texture.page < 0) {
+                batch_texture = path_texture;
+                draw_batch.color_texture_page = path_texture.page;
+                draw_batch.color_texture_sampling_flags = path_texture.sampling_flags;
+            } else if (path_texture != batch_texture) {
+                draw_batches.push_back(std::move(draw_batch));
+                have_draw_batch = false;
+            }
+        }
         if (!have_draw_batch) {
-            draw_batch = new_batch(MAX_CLIP_BATCHES, PFH_PATH_SOURCE_DRAW, st.transform);
+            draw_batch = new_batch(next_batch_id++, PFH_PATH_SOURCE_DRAW, st.transform);
+            batch_texture = path_texture;
+            draw_batch.color_texture_page = path_texture.page;
+            draw_batch.color_texture_sampling_flags = path_texture.sampling_flags;
             have_draw_batch = true;
         }
         uint32_t cb = 0, ci = 0;
         bool has_clip = add_clip_path_to_batch(st, dp.clip_path, 0, &cb, &ci);
         // BuiltDrawPath::new (builder.rs:83): occludes = opaque paint and a
         // mode that occludes the backdrop
-        bool opaque = dp.paint < scene->paints.size() && scene->paints[dp.paint].a == 255 &&
+        bool opaque = dp.paint < palette.opaque.size() && palette.opaque[dp.paint] &&
                       blend_occludes_backdrop(dp.blend_mode);
         batch_push(draw_batch, built, (uint32_t)i, draw_seg_begin[i], draw_seg_end[i], has_clip,
                    cb, ci, opaque);
@@ -514,8 +581,9 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
         list->commands.push_back(
             {PFH_COMMAND_PREPARE_CLIP_TILES_D3D11, (uint32_t)list->batches.size() - 1});
     }
-    if (have_draw_batch) {
-        list->batches.push_back(std::move(draw_batch));
+    if (have_draw_batch) draw_batches.push_back(std::move(draw_batch));
+    for (pfh::BatchData &b : draw_batches) {
+        list->batches.push_back(std::move(b));
         list->commands.push_back({PFH_COMMAND_DRAW_TILES_D3D11, (uint32_t)list->batches.size() - 1});
     }
     list->commands.push_back({PFH_COMMAND_FINISH, 0});
@@ -573,6 +641,42 @@ pfh_status pfh_command_paints(const pfh_command_list *list, uint32_t index,
     return PFH_OK;
 }
 
+pfh_status pfh_command_texture_metadata(const pfh_command_list *list, uint32_t index,
+                                        const pfh_texture_metadata_entry **entries,
+                                        uint32_t *count) {
+    if (!list || !entries || !count || index >= list->commands.size() ||
+        list->commands[index].kind != PFH_COMMAND_UPLOAD_TEXTURE_METADATA)
+        return PFH_ERROR_INVALID_ARGUMENT;
+    *entries = list->metadata.data();
+    *count = (uint32_t)list->metadata.size();
+    return PFH_OK;
+}
+
+pfh_status pfh_command_texture_page(const pfh_command_list *list, uint32_t index,
+                                    uint32_t *page_id, int32_t *width, int32_t *height) {
+    if (!list || !page_id || !width || !height || index >= list->commands.size() ||
+        list->commands[index].kind != PFH_COMMAND_ALLOCATE_TEXTURE_PAGE)
+        return PFH_ERROR_INVALID_ARGUMENT;
+    uint32_t slot = list->commands[index].batch_slot;
+    *page_id = slot;
+    *width = list->pages[slot].width;
+    *height = list->pages[slot].height;
+    return PFH_OK;
+}
+
+pfh_status pfh_command_texel_data(const pfh_command_list *list, uint32_t index,
+                                  uint32_t *page_id, pfh_rect_i *rect,
+                                  const pfh_color_u **texels) {
+    if (!list || !page_id || !rect || !texels || index >= list->commands.size() ||
+        list->commands[index].kind != PFH_COMMAND_UPLOAD_TEXEL_DATA)
+        return PFH_ERROR_INVALID_ARGUMENT;
+    const pfh::TexelUpload &u = list->uploads[list->commands[index].batch_slot];
+    *page_id = u.page;
+    *rect = u.rect;
+    *texels = u.texels.data();
+    return PFH_OK;
+}
+
 pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list) {
     if (!ctx || !list) return PFH_ERROR_INVALID_ARGUMENT;
     for (size_t i = 0; i < list->commands.size(); i++) {
@@ -580,9 +684,19 @@ pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list) {
         pfh_status s = PFH_OK;
         switch (c.kind) {
         case PFH_COMMAND_UPLOAD_TEXTURE_METADATA:
-            s = pfh_upload_texture_metadata(ctx, list->paints.data(),
-                                            (uint32_t)list->paints.size());
+            s = pfh_upload_texture_metadata_entries(ctx, list->metadata.data(),
+                                                    (uint32_t)list->metadata.size());
             break;
+        case PFH_COMMAND_ALLOCATE_TEXTURE_PAGE: {
+            const pfh::PageSize &pg = list->pages[c.batch_slot];
+            s = pfh_allocate_texture_page(ctx, c.batch_slot, pg.width, pg.height);
+            break;
+        }
+        case PFH_COMMAND_UPLOAD_TEXEL_DATA: {
+            const pfh::TexelUpload &u = list->uploads[c.batch_slot];
+            s = pfh_upload_texel_data(ctx, u.page, &u.rect, u.texels.data());
+            break;
+        }
         case PFH_COMMAND_UPLOAD_SCENE_D3D11: {
             pfh_segments d, k;
             pfh_command_segments(list, (uint32_t)i, &d, &k);

@@ -81,15 +81,191 @@ class ColorF:
         self.r, self.g, self.b, self.a = float(r), float(g), float(b), float(a)
 
 
-class Paint:
-    """A solid paint (Paint::from_color, paint.rs)."""
+def _f32(v):
+    return np.float32(v)
 
-    def __init__(self, color):
+
+def _xf_mul(a, b):
+    """Transform2F * Transform2F in binary32 (transform2d.rs:115-130, 301-310);
+    a, b are (m00, m01, m10, m11, tx, ty)."""
+    a00, a01, a10, a11, atx, aty = [_f32(v) for v in a]
+    b00, b01, b10, b11, btx, bty = [_f32(v) for v in b]
+    return (a00 * b00 + a01 * b10, a00 * b01 + a01 * b11,
+            a10 * b00 + a11 * b10, a10 * b01 + a11 * b11,
+            (a00 * btx + a01 * bty) + atx, (a10 * btx + a11 * bty) + aty)
+
+
+def _xf_point(t, x, y):
+    m00, m01, m10, m11, tx, ty = [_f32(v) for v in t]
+    x, y = _f32(x), _f32(y)
+    return ((m00 * x + m01 * y) + tx, (m10 * x + m11 * y) + ty)
+
+
+class ColorStop:
+    """ColorStop (content/src/gradient.rs:36-42)."""
+
+    def __init__(self, color, offset):
+        self.color, self.offset = color, float(np.float32(offset))
+
+
+class Gradient:
+    """Gradient (content/src/gradient.rs:26-78): linear, or radial between two
+    circles, with colour stops and a wrap mode."""
+
+    CLAMP, REPEAT = L.GRADIENT_WRAP_CLAMP, L.GRADIENT_WRAP_REPEAT
+
+    def __init__(self, geometry, from_, to, radii=(0.0, 0.0)):
+        self.geometry = geometry
+        self.from_ = tuple(float(_f32(v)) for v in from_)
+        self.to = tuple(float(_f32(v)) for v in to)
+        self.radii = tuple(float(_f32(v)) for v in radii)
+        self.transform = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+        self.stops = []
+        self.wrap = Gradient.CLAMP
+
+    @staticmethod
+    def linear_from_points(from_, to):
+        return Gradient(L.GRADIENT_LINEAR, from_, to)
+
+    @staticmethod
+    def radial(center_or_line, radii):
+        """Gradient::radial: a point (both circles share the centre) or a
+        (from, to) pair of centres."""
+        c = center_or_line
+        from_, to = (c, c) if np.ndim(c) == 1 else (c[0], c[1])
+        return Gradient(L.GRADIENT_RADIAL, from_, to, radii)
+
+    def add_color_stop(self, color, offset):
+        """Gradient::add (gradient.rs:159-164): after every stop whose offset
+        is <= the new one."""
+        stop = ColorStop(color, offset)
+        i = 0
+        while i < len(self.stops) and self.stops[i].offset <= stop.offset:
+            i += 1
+        self.stops.insert(i, stop)
+
+    def apply_transform(self, t):
+        """Gradient::apply_transform (:230-241): moves a linear gradient's
+        line, composes a radial gradient's transform."""
+        t = t.m if isinstance(t, Transform2F) else tuple(t)
+        if t == (1.0, 0.0, 0.0, 1.0, 0.0, 0.0):
+            return
+        if self.geometry == L.GRADIENT_LINEAR:
+            self.from_ = tuple(float(v) for v in _xf_point(t, *self.from_))
+            self.to = tuple(float(v) for v in _xf_point(t, *self.to))
+        else:
+            self.transform = tuple(float(v) for v in _xf_mul(t, self.transform))
+
+    def to_c(self):
+        stops = (L.ColorStop * max(len(self.stops), 1))(
+            *[L.ColorStop(st.offset, L.ColorU(*st.color)) for st in self.stops])
+        g = L.Gradient(self.geometry, L.Vector2F(*self.from_), L.Vector2F(*self.to),
+                       self.radii[0], self.radii[1], L.Transform2F(*self.transform),
+                       C.cast(stops, C.POINTER(L.ColorStop)), len(self.stops), self.wrap)
+        return g, stops
+
+
+class Image:
+    """Image (content/src/pattern.rs:37-44): RGBA8 pixels of shape (h, w, 4)."""
+
+    def __init__(self, pixels):
+        self.pixels = np.ascontiguousarray(pixels, np.uint8)
+        assert self.pixels.ndim == 3 and self.pixels.shape[2] == 4
+        self.height, self.width = self.pixels.shape[:2]
+
+    def is_opaque(self):
+        return bool(np.all(self.pixels[..., 3] == 255))
+
+
+class Pattern:
+    """Pattern (content/src/pattern.rs:24-32) of an image: a transform, an
+    optional filter (blur or colour matrix), repeat flags and smoothing."""
+
+    def __init__(self, image):
+        self.image = image
+        self.transform = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+        self.filter = (L.FILTER_NONE, ())
+        self.repeat_x = self.repeat_y = False
+        self.smoothing_enabled = True
+
+    @staticmethod
+    def from_image(image):
+        return Pattern(image)
+
+    def apply_transform(self, t):
+        """Pattern::apply_transform (pattern.rs:119-121): t * transform."""
+        t = t.m if isinstance(t, Transform2F) else tuple(t)
+        self.transform = tuple(float(v) for v in _xf_mul(t, self.transform))
+
+    def set_blur(self, direction_y, sigma):
+        self.filter = (L.FILTER_BLUR, (1.0 if direction_y else 0.0, float(sigma)))
+
+    def set_color_matrix(self, columns):
+        """The five columns of ColorMatrix (20 floats), as feColorMatrix."""
+        self.filter = (L.FILTER_COLOR_MATRIX, tuple(float(v) for v in np.ravel(columns)))
+
+    def flags(self):
+        return ((L.PATTERN_REPEAT_X if self.repeat_x else 0) |
+                (L.PATTERN_REPEAT_Y if self.repeat_y else 0) |
+                (0 if self.smoothing_enabled else L.PATTERN_NO_SMOOTHING))
+
+    def to_c(self):
+        f = L.Filter()
+        f.kind = self.filter[0]
+        for i, v in enumerate(self.filter[1]):
+            f.params[i] = v
+        p = L.Pattern(self.image.width, self.image.height,
+                      self.image.pixels.ctypes.data_as(C.c_void_p),
+                      L.Transform2F(*self.transform), self.flags(), f)
+        return p, self.image.pixels
+
+
+class Paint:
+    """Paint (renderer/src/paint.rs:53-146): a base colour and, for gradients
+    and patterns, an overlay combined with it (PaintCompositeOp)."""
+
+    SRC_IN, DEST_IN = L.PAINT_COMPOSITE_SRC_IN, L.PAINT_COMPOSITE_DEST_IN
+
+    def __init__(self, color, overlay=None, composite_op=L.PAINT_COMPOSITE_SRC_IN):
         self.color = color
+        self.overlay = overlay
+        self.composite_op = composite_op
 
     @staticmethod
     def from_color(color):
         return Paint(color)
+
+    @staticmethod
+    def from_gradient(gradient):
+        return Paint(ColorU(255, 255, 255, 255), gradient)
+
+    @staticmethod
+    def from_pattern(pattern):
+        return Paint(ColorU(255, 255, 255, 255), pattern)
+
+    def is_opaque(self):
+        """Paint::is_opaque (paint.rs:163-176)."""
+        if not self.color.is_opaque():
+            return False
+        if isinstance(self.overlay, Gradient):
+            return all(st.color.is_opaque() for st in self.overlay.stops)
+        if isinstance(self.overlay, Pattern):
+            return self.overlay.image.is_opaque()
+        return True
+
+    def key(self):
+        """Palette::push_paint's equality (paint.rs:381-390)."""
+        c = tuple(self.color)
+        o = self.overlay
+        if o is None:
+            return ("color", c)
+        if isinstance(o, Gradient):
+            radial = o.geometry == L.GRADIENT_RADIAL
+            return ("gradient", c, self.composite_op, o.geometry, o.from_, o.to,
+                    o.radii if radial else None, o.transform if radial else None, o.wrap,
+                    tuple((st.offset, tuple(st.color)) for st in o.stops))
+        return ("pattern", c, self.composite_op, o.image.width, o.image.height,
+                o.transform, o.flags(), o.filter, o.image.pixels.tobytes())
 
 
 class RectF:
@@ -287,7 +463,8 @@ class Scene:
 
     def __init__(self):
         self.view_box = RectF(0, 0, 0, 0)
-        self.paints = []
+        self.paints = []          # base colours, by PaintId
+        self.paint_objects = []   # the Paint of every PaintId
         self._paint_cache = {}
         self._draw_parts = []
         self._clip_parts = []
@@ -298,14 +475,16 @@ class Scene:
 
     # Scene::push_paint + Palette dedup (paint.rs:381-390)
     def push_paint(self, paint):
-        c = paint.color if isinstance(paint, Paint) else paint
-        key = (c.r, c.g, c.b, c.a)
+        if not isinstance(paint, Paint):
+            paint = Paint(paint)
+        key = paint.key()
         if key in self._paint_cache:
             return self._paint_cache[key]
         pid = len(self.paints)
         if pid >= 65536:
             raise ValueError("PaintId is 16 bits (paint.rs:81)")
-        self.paints.append(ColorU(*key))
+        self.paints.append(ColorU(*paint.color))
+        self.paint_objects.append(paint)
         self._paint_cache[key] = pid
         return pid
 
@@ -358,8 +537,21 @@ class Scene:
         vb = L.RectF(L.Vector2F(self.view_box.x0, self.view_box.y0),
                      L.Vector2F(self.view_box.x1, self.view_box.y1))
         lib.pfh_scene_set_view_box(h, C.byref(vb))
-        for c in self.paints:
-            lib.pfh_scene_push_paint(h, C.byref(L.ColorU(c.r, c.g, c.b, c.a)))
+        for i, p in enumerate(self.paint_objects):
+            c = L.ColorU(*p.color)
+            if p.overlay is None:
+                got = lib.pfh_scene_push_paint(h, C.byref(c))
+            elif isinstance(p.overlay, Gradient):
+                g, keep = p.overlay.to_c()
+                got = lib.pfh_scene_push_gradient_paint(h, C.byref(c), C.byref(g),
+                                                        p.composite_op)
+            else:
+                q, keep = p.overlay.to_c()
+                got = lib.pfh_scene_push_pattern_paint(h, C.byref(c), C.byref(q),
+                                                       p.composite_op)
+            if got != i:
+                lib.pfh_scene_destroy(h)
+                raise ValueError("paint %d: the library gave id %d" % (i, got))
         for a in self._clip_parts:
             lib.pfh_scene_push_clip_paths(h, _ptr(a.points), _ptr(a.flags), len(a.points),
                                           _ptr(a.contour_ends), len(a.contour_ends),
@@ -441,6 +633,41 @@ class CommandList:
                 L.check(self._lib.pfh_command_paints(self._h, i, C.byref(ptr), C.byref(n)))
                 return _copy(ptr.value, n.value * 4, np.uint8).reshape(-1, 4)
         return np.zeros((0, 4), np.uint8)
+
+    def metadata(self):
+        """The TextureMetadataEntry of every paint, as METADATA_DTYPE records."""
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_UPLOAD_TEXTURE_METADATA:
+                ptr, n = C.c_void_p(), C.c_uint32()
+                L.check(self._lib.pfh_command_texture_metadata(self._h, i, C.byref(ptr),
+                                                               C.byref(n)))
+                return _copy(ptr.value, n.value * METADATA_DTYPE.itemsize, np.uint8).view(
+                    METADATA_DTYPE)
+        return np.zeros(0, METADATA_DTYPE)
+
+    def pages(self):
+        """{page id: (width, height)} of the ALLOCATE_TEXTURE_PAGE commands."""
+        out = {}
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_ALLOCATE_TEXTURE_PAGE:
+                pid, w, h = C.c_uint32(), C.c_int32(), C.c_int32()
+                L.check(self._lib.pfh_command_texture_page(self._h, i, C.byref(pid),
+                                                           C.byref(w), C.byref(h)))
+                out[pid.value] = (w.value, h.value)
+        return out
+
+    def texel_uploads(self):
+        """[(page, x, y, (h, w, 4) uint8)] of the UPLOAD_TEXEL_DATA commands."""
+        out = []
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_UPLOAD_TEXEL_DATA:
+                pid, r, ptr = C.c_uint32(), L.RectI(), C.c_void_p()
+                L.check(self._lib.pfh_command_texel_data(self._h, i, C.byref(pid), C.byref(r),
+                                                         C.byref(ptr)))
+                w, h = r.max_x - r.min_x, r.max_y - r.min_y
+                out.append((pid.value, r.min_x, r.min_y,
+                            _copy(ptr.value, w * h * 4, np.uint8).reshape(h, w, 4)))
+        return out
 
 
 def _copy(addr, nbytes, dtype):

@@ -502,3 +502,124 @@ def blend_scene(seed=SEED_BASE + 5, width=160, height=120):
                                 [int(rng.integers(2))], [clip if mode % 5 == 4 else -1], [mode]))
     s.push_draw_paths(PathArrays.concat(parts))
     return s, (width, height), ColorF(1, 1, 1, 1)
+
+
+def _rotation_scale(theta, s, tx, ty):
+    c, sn = math.cos(theta) * s, math.sin(theta) * s
+    return (c, -sn, sn, c, tx, ty)
+
+
+def paint_scene(seed=SEED_BASE + 6, size=512, paths=96):
+    """Gradient and pattern paints (paint.rs, gradient.rs, pattern.rs) on
+    random closed cubic paths: linear gradients (clamp and repeat, stops
+    added out of order, some translucent), radial gradients (concentric and
+    two-point, transformed), and image patterns (repeat and clamp, smoothing
+    on and off, blur and colour-matrix filters, DestIn, one image shared by two
+    paints), interleaved with solid colours so that batches break and adopt
+    colour textures. Returns (scene, (size, size), background)."""
+    from .scene import ColorU, Gradient, Image, Paint, Pattern
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    scene.set_view_box(RectF(0, 0, size, size))
+
+    def color(opaque=True):
+        r, g, b = (int(v) for v in rng.integers(0, 256, 3))
+        return ColorU(r, g, b, 255 if opaque else int(rng.integers(40, 255)))
+
+    def stops(g, n, opaque=True):
+        offs = rng.uniform(0, 1, n)
+        offs[rng.random(n) < 0.2] = 0.0
+        for o in offs:  # out of order: Gradient::add keeps them sorted
+            g.add_color_stop(color(opaque or rng.random() < 0.5), float(o))
+        return g
+
+    def image(w, h, opaque=True):
+        px = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        if opaque:
+            px[..., 3] = 255
+        return Image(px)
+
+    pool = []
+    for k in range(3):
+        g = stops(Gradient.linear_from_points(rng.uniform(0, size, 2), rng.uniform(0, size, 2)),
+                  int(rng.integers(2, 6)), opaque=k != 2)
+        pool.append(Paint.from_gradient(g))
+    g = stops(Gradient.linear_from_points((100.0, 100.0), (140.0, 130.0)), 3)
+    g.wrap = Gradient.REPEAT
+    pool.append(Paint.from_gradient(g))
+    g = stops(Gradient.radial((size / 2, size / 2), (10.0, size / 2.5)), 4)
+    pool.append(Paint.from_gradient(g))
+    g = stops(Gradient.radial(((150.0, 200.0), (330.0, 260.0)), (20.0, 140.0)), 3)
+    g.apply_transform(_rotation_scale(0.3, 1.1, -20.0, 15.0))
+    pool.append(Paint.from_gradient(g))
+    g = stops(Gradient.radial((260.0, 250.0), (0.0, 60.0)), 2)
+    g.wrap = Gradient.REPEAT
+    pool.append(Paint.from_gradient(g))
+    shared = image(13, 9)
+    p = Pattern.from_image(shared)
+    p.repeat_x = p.repeat_y = True
+    p.apply_transform(_rotation_scale(0.4, 7.0, 31.0, -12.0))
+    pool.append(Paint.from_pattern(p))
+    p = Pattern.from_image(shared)
+    p.repeat_x = p.repeat_y = True
+    p.apply_transform(_rotation_scale(-1.1, 3.5, 0.0, 0.0))
+    pool.append(Paint.from_pattern(p))
+    p = Pattern.from_image(image(32, 24, opaque=False))
+    p.smoothing_enabled = False
+    p.apply_transform(_rotation_scale(0.0, 9.0, 90.0, 120.0))
+    pool.append(Paint.from_pattern(p))
+    p = Pattern.from_image(image(40, 30))
+    p.repeat_x = True
+    p.set_blur(False, 2.0)
+    p.apply_transform(_rotation_scale(0.2, 4.0, 10.0, 10.0))
+    pool.append(Paint.from_pattern(p))
+    p = Pattern.from_image(image(20, 20))
+    p.repeat_x = p.repeat_y = True
+    p.set_color_matrix(np.concatenate([rng.uniform(-0.5, 1.0, 16), rng.uniform(-0.2, 0.3, 4)]))
+    p.apply_transform(_rotation_scale(0.7, 6.0, 0.0, 0.0))
+    pool.append(Paint.from_pattern(p))
+    p = Pattern.from_image(image(16, 16))
+    p.repeat_x = p.repeat_y = True
+    p.apply_transform(_rotation_scale(0.0, 5.0, 3.0, 3.0))
+    dest_in = Paint.from_pattern(p)
+    dest_in.color = ColorU(220, 60, 30, 255)
+    dest_in.composite_op = Paint.DEST_IN
+    pool.append(dest_in)
+    arrays = _cubic_paths(rng, paths, size)
+    ids = []
+    for i in range(paths):
+        if rng.random() < 0.25:
+            ids.append(scene.push_paint(Paint.from_color(color(rng.random() < 0.6))))
+        else:
+            ids.append(scene.push_paint(pool[int(rng.integers(0, len(pool)))]))
+    arrays.paints = np.asarray(ids, np.uint32)
+    scene.push_draw_paths(arrays)
+    return scene, (size, size), ColorF(1.0, 1.0, 1.0, 1.0)
+
+
+def gradient_pages_scene(seed=SEED_BASE + 7, count=300, size=480):
+    """`count` squares, each with its own two-stop linear gradient: more
+    gradients than the 256 rows of one gradient page, so the palette opens a
+    second page and the draw batch breaks there."""
+    from .scene import ColorU, Gradient, Paint
+    rng = np.random.default_rng(seed)
+    scene = Scene()
+    scene.set_view_box(RectF(0, 0, size, size))
+    side = int(math.ceil(math.sqrt(count)))
+    cell = size / side
+    pts, ids = [], []
+    for i in range(count):
+        x0, y0 = (i % side) * cell, (i // side) * cell
+        pts.append([(x0 + 1, y0 + 1), (x0 + cell - 1, y0 + 1), (x0 + cell - 1, y0 + cell - 1),
+                    (x0 + 1, y0 + cell - 1)])
+        g = Gradient.linear_from_points((x0, y0), (x0 + cell, y0 + cell * rng.uniform(0, 1)))
+        for o in (0.0, 1.0):
+            r, gg, b = (int(v) for v in rng.integers(0, 256, 3))
+            g.add_color_stop(ColorU(r, gg, b, 255), o)
+        ids.append(scene.push_paint(Paint.from_gradient(g)))
+    pts = np.asarray(pts, np.float32).reshape(-1, 2)
+    arrays = PathArrays(pts, np.zeros(len(pts), np.uint8),
+                        np.arange(4, len(pts) + 1, 4, dtype=np.uint32),
+                        np.arange(1, count + 1, dtype=np.uint32), np.asarray(ids, np.uint32))
+    scene.push_draw_paths(arrays)
+    return scene, (size, size), ColorF(0.0, 0.0, 0.0, 1.0)

@@ -73,3 +73,32 @@ def test_backward_refuses_textured_paints(renderer_factory):
     with pytest.raises(L.PfhError, match="solid paints"):
         r.backward(g_img.data_ptr(), g_pts.data_ptr(), len(case["cmds"]["draw"]["points"]),
                    g_col.data_ptr(), 1)
+
+
+@pytest.mark.parametrize("xf", [None, (0.8, 0.3, -0.25, 0.9, 100.5, -40.25)])
+def test_paint_scene_bit_exact(xf, renderer_factory):
+    """A scene of gradient and pattern paints through the C++ scene builder
+    and the HIP path, against the packing restatement and the oracle: every
+    batch's buffers and the framebuffer."""
+    from helpers import oracle_frame, product_frame
+    from pathfinder_amd import synth
+    scene, size, bg = synth.paint_scene()
+    _say("case paint scene", size, xf)
+    cmds, o = oracle_frame(scene, size, bg, transform=xf or (1, 0, 0, 1, 0, 0))
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene, transform=xf)
+    report = compare_frame(r, o, cmds)
+    _say({k: v for k, v in report.items() if v[0]})
+    assert_bit_exact(report)
+
+
+def test_gradient_pages_bit_exact(renderer_factory):
+    from helpers import oracle_frame, product_frame
+    from pathfinder_amd import synth
+    scene, size, bg = synth.gradient_pages_scene()
+    _say("case gradient pages", size)
+    cmds, o = oracle_frame(scene, size, bg)
+    r = renderer_factory(size[0], size[1], bg)
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    assert_bit_exact(report)

@@ -24,6 +24,12 @@ def _compare(scene, transform=None):
         assert np.array_equal(got["points"], exp["points"])
         assert np.array_equal(got["indices"], exp["indices"])
     assert np.array_equal(cl.paints(), want["paints"])
+    assert np.array_equal(cl.metadata().view(np.uint8), want["metadata"].view(np.uint8))
+    assert cl.pages() == want["pages"]
+    got_t = cl.texel_uploads()
+    assert len(got_t) == len(want["texels"])
+    for g, w in zip(got_t, want["texels"]):
+        assert g[:3] == tuple(w[:3]) and np.array_equal(g[3], w[3])
     got_b = cl.batches()
     assert [k for k, _ in got_b] == [k for k, _ in want["batches"]]
     for (_, gb), (_, wb) in zip(got_b, want["batches"]):
@@ -172,3 +178,22 @@ def test_null_arguments_are_rejected():
     assert lib.pfh_end_scene(None) == 1
     assert lib.pfh_scene_build(None, None, None) == 1
     assert lib.pfh_render_commands(None, None) == 1
+
+
+@pytest.mark.parametrize("xf", [None, (0.8, 0.3, -0.25, 0.9, 100.5, -40.25)])
+def test_packing_paint_scene(xf):
+    """Gradients and patterns: TextureMetadataEntry bytes, texture pages,
+    texel uploads (gradient ramps and images) and the colour-texture batch
+    breaks equal the restatement."""
+    scene, _, _ = synth.paint_scene()
+    cmds = _compare(scene, xf)
+    draws = [b for k, b in cmds["batches"] if k == packing.DRAW]
+    assert len(draws) > 10
+    assert {b["color_texture_page"] for b in draws} == set(cmds["pages"])
+
+
+def test_packing_gradient_pages():
+    scene, _, _ = synth.gradient_pages_scene()
+    cmds = _compare(scene)
+    assert cmds["pages"] == {0: (256, 256), 1: (256, 256)}
+    assert [b["color_texture_page"] for _, b in cmds["batches"]] == [0, 1]
