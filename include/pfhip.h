@@ -195,12 +195,14 @@ typedef struct pfh_texture_metadata_entry {
 
 typedef struct pfh_context pfh_context;
 
-/* RendererLevel (gpu/options.rs:41-46); only D3D11 is implemented. */
+/* RendererLevel (gpu/options.rs:41-46). A context serves the seams of both
+ * levels; the level names the one its owner builds scenes for. */
+#define PFH_RENDERER_LEVEL_D3D9 1
 #define PFH_RENDERER_LEVEL_D3D11 2
 
 typedef struct pfh_context_options {
     int32_t device;               /* HIP device ordinal */
-    int32_t level;                /* PFH_RENDERER_LEVEL_D3D11 */
+    int32_t level;                /* PFH_RENDERER_LEVEL_D3D9 or _D3D11 */
     const uint8_t *area_lut_rgba; /* 256x256 RGBA8 area LUT; NULL = built-in copy */
     uint32_t initial_microline_capacity; /* 0 = 16384 (d3d11/renderer.rs:43) */
     uint32_t initial_fill_capacity;      /* 0 = 16384 (d3d11/renderer.rs:44) */
@@ -469,6 +471,60 @@ void pfh_scene_destroy(pfh_scene *scene);
 void pfh_scene_set_view_box(pfh_scene *scene, const pfh_rect_f *view_box);
 /* Scene::push_paint (scene.rs:191, palette dedup paint.rs:381-390). Returns the PaintId. */
 uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color);
+/* ------------------------------------------------------------ D3D9 level
+ * RendererD3D9 (gpu/d3d9/renderer.rs:150-472): fills from the CPU tiler are
+ * rasterized into a binary16 RGBA mask (each fragment added to its texel in
+ * fill order, the sum rounded to binary16), clip tiles combine masks, and the
+ * batch's tiles composite src-over into the RGBA8 target one tile at a time
+ * (the target rounded to 8 bits after every tile). Solid paints, SrcOver. */
+
+/* Fill (gpu_data.rs:356-363) as the raster level uses it: the line in 8.8
+ * fixed point within the tile, and the alpha tile it belongs to. */
+typedef struct pfh_fill_d3d9 {
+    uint16_t from_x, from_y, to_x, to_y;
+    uint32_t alpha_tile_index;
+} pfh_fill_d3d9;
+
+/* TileObjectPrimitive (gpu_data.rs:266-275). alpha_tile_id 0xFFFFFFFF: a
+ * solid tile. ctrl: 1 nonzero, 2 even-odd (the mask's fill rule). */
+typedef struct pfh_tile_object_d3d9 {
+    int16_t tile_x, tile_y;
+    uint32_t alpha_tile_id;
+    uint32_t path_id;
+    uint16_t color;
+    uint8_t ctrl;
+    int8_t backdrop;
+} pfh_tile_object_d3d9;
+
+/* Clip (gpu_data.rs:378-383): dest = min(|dest + dest_backdrop|, |src + src_backdrop|). */
+typedef struct pfh_clip_d3d9 {
+    uint32_t dest_tile_id;
+    int32_t dest_backdrop;
+    uint32_t src_tile_id;
+    int32_t src_backdrop;
+} pfh_clip_d3d9;
+
+/* DrawTileBatchD3D9 (gpu_data.rs:227-237) for solid paints and SrcOver: the
+ * tiles in draw order, the clip tiles, and z_buffer_data (DenseTileMap<i32>:
+ * the rect in tiles and its values row by row; a tile whose path id is below
+ * its z value is not drawn). */
+typedef struct pfh_draw_tile_batch_d3d9 {
+    const pfh_tile_object_d3d9 *tiles;
+    uint32_t tile_count;
+    const pfh_clip_d3d9 *clips;
+    uint32_t clip_count;
+    const int32_t *z_buffer;
+    pfh_rect_i z_buffer_rect;
+} pfh_draw_tile_batch_d3d9;
+
+/* RendererD3D9::add_fills (:150-174): queues fills; alpha tiles 0 ..
+ * max(alpha_tile_index) get mask storage. */
+pfh_status pfh_d3d9_add_fills(pfh_context *ctx, const pfh_fill_d3d9 *fills, uint32_t count);
+/* draw_buffered_fills (:176-184): rasterizes the queued fills into the mask. */
+pfh_status pfh_d3d9_flush_fills(pfh_context *ctx);
+/* clip_tiles + draw_tiles (:254-440) for one batch, inside a scene. */
+pfh_status pfh_d3d9_draw_tiles(pfh_context *ctx, const pfh_draw_tile_batch_d3d9 *batch);
+
 /* Gradient and pattern paints (Paint::from_gradient / from_pattern,
  * paint.rs:128-146; content/src/gradient.rs, content/src/pattern.rs). The
  * build renders each gradient into a row of a 256x256 colour texture page and
@@ -621,7 +677,10 @@ typedef enum pfh_command_kind {
     PFH_COMMAND_DRAW_TILES_D3D11 = 4,
     PFH_COMMAND_FINISH = 5,
     PFH_COMMAND_ALLOCATE_TEXTURE_PAGE = 6,
-    PFH_COMMAND_UPLOAD_TEXEL_DATA = 7
+    PFH_COMMAND_UPLOAD_TEXEL_DATA = 7,
+    PFH_COMMAND_ADD_FILLS_D3D9 = 8,
+    PFH_COMMAND_FLUSH_FILLS_D3D9 = 9,
+    PFH_COMMAND_DRAW_TILES_D3D9 = 10
 } pfh_command_kind;
 uint32_t pfh_command_list_len(const pfh_command_list *list);
 int32_t pfh_command_kind_at(const pfh_command_list *list, uint32_t index);
@@ -648,6 +707,25 @@ pfh_status pfh_command_texel_data(const pfh_command_list *list, uint32_t index,
                                   uint32_t *page_id, pfh_rect_i *rect,
                                   const pfh_color_u **texels);
 
+/* For ADD_FILLS_D3D9: one path's fills. */
+pfh_status pfh_command_fills_d3d9(const pfh_command_list *list, uint32_t index,
+                                  const pfh_fill_d3d9 **fills, uint32_t *count);
+/* For DRAW_TILES_D3D9 (pointers stay valid while the list lives). */
+pfh_status pfh_command_batch_d3d9(const pfh_command_list *list, uint32_t index,
+                                  pfh_draw_tile_batch_d3d9 *out);
+
+/* SceneBuilder::build for the D3D9 level (builder.rs:148-222 with
+ * build_paths_on_cpu :224-318 and the D3D9 arm of
+ * build_tile_batches_for_draw_path_display_item :998-1041): the tiler runs
+ * here, on the calling thread and up to 15 more, and the list holds
+ * ADD_FILLS_D3D9 per path with fills (clip paths, then draw paths, alpha
+ * tiles numbered path by path), FLUSH_FILLS_D3D9 and one DRAW_TILES_D3D9.
+ * Fails with PFH_ERROR_INVALID_ARGUMENT for a paint with a colour texture or
+ * a blend mode other than SrcOver. A clip path's own clip path is ignored,
+ * as the reference's tiler gets no built clip paths for clip paths. */
+pfh_status pfh_scene_build_d3d9(pfh_scene *scene, const pfh_build_options *options,
+                                pfh_command_list **out);
+
 /* Renderer::render_command for every command of the list, between
  * pfh_begin_scene and pfh_end_scene (the caller brackets it). */
 pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list);
@@ -656,6 +734,10 @@ pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list);
 pfh_status pfh_scene_build_and_render(pfh_scene *scene, pfh_context *ctx,
                                       const pfh_frame_options *frame,
                                       const pfh_build_options *options);
+/* The same at the D3D9 level (pfh_scene_build_d3d9). */
+pfh_status pfh_scene_build_and_render_d3d9(pfh_scene *scene, pfh_context *ctx,
+                                           const pfh_frame_options *frame,
+                                           const pfh_build_options *options);
 
 #ifdef __cplusplus
 }

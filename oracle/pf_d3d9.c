@@ -87,8 +87,9 @@ static void d9_add_fill(d9_path *p, d9v from, d9v to, int32_t tx, int32_t ty) {
                   (to.y - oy) * 256.0f};
     int32_t q[4];
     for (int i = 0; i < 4; i++) {
-        float c = v[i] < 0.0f ? 0.0f : v[i];
-        c = c > (float)(D9_TILE * 256 - 1) ? (float)(D9_TILE * 256 - 1) : c;
+        /* F32x4::clamp: max then min, as SSE orders them (NaN -> the bound) */
+        float c = v[i] > 0.0f ? v[i] : 0.0f;
+        c = c < (float)(D9_TILE * 256 - 1) ? c : (float)(D9_TILE * 256 - 1);
         q[i] = (int32_t)rintf(c); /* _mm_cvtps_epi32: round to nearest even */
     }
     if (q[0] == q[2]) return;
@@ -243,10 +244,14 @@ static void d9_tile_path(d9_path *p, const orc_d3d9_paths *src, uint32_t i, cons
             bx1 = q.x > bx1 ? q.x : bx1; by1 = q.y > by1 ? q.y : by1;
         }
     }
-    /* RectF::intersection with the view box; empty -> RectF::default() */
+    /* RectF::intersection with the view box (geometry/src/rect.rs:121-137:
+     * strict overlap); none -> RectF::default(). An empty outline's bounds
+     * are RectF::default() too. */
+    if (s0 == s1) bx0 = by0 = bx1 = by1 = 0.0f;
     float ix0 = bx0 > vb[0] ? bx0 : vb[0], iy0 = by0 > vb[1] ? by0 : vb[1];
     float ix1 = bx1 < vb[2] ? bx1 : vb[2], iy1 = by1 < vb[3] ? by1 : vb[3];
-    if (!(ix0 <= ix1 && iy0 <= iy1) || s0 == s1) ix0 = iy0 = ix1 = iy1 = 0.0f;
+    if (!(bx0 < vb[2] && by0 < vb[3] && vb[0] < bx1 && vb[1] < by1))
+        ix0 = iy0 = ix1 = iy1 = 0.0f;
     /* tiles.rs:64-66 round_rect_out_to_tile_bounds */
     p->rx0 = (int32_t)floorf(ix0 / (float)D9_TILE);
     p->ry0 = (int32_t)floorf(iy0 / (float)D9_TILE);
@@ -378,14 +383,15 @@ static inline uint32_t d9_unorm8(float x) {
     return (uint32_t)floorf(c * 255.0f + 0.5f);
 }
 
-int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, int threads) {
-    int nthreads = threads > 0 ? threads : 1;
+/* Tiles every clip path, then every draw path, numbers alpha tiles path by
+ * path and prepares the tiles (backdrops, clip jobs). */
+static int d9_build_paths(const orc_d3d9_scene *s, int nthreads, d9_path **clips_out,
+                          d9_path **draws_out, int32_t *n_alpha_out) {
     const float *xf = s->transform, *vb = s->view_box;
     uint32_t nc = s->clip.n_paths, nd = s->draw.n_paths;
     d9_path *clips = (d9_path *)calloc(nc ? nc : 1, sizeof(d9_path));
     d9_path *draws = (d9_path *)calloc(nd ? nd : 1, sizeof(d9_path));
     if (!clips || !draws) return -1;
-    double t_start = orc_now();
 
     /* builder.rs build_clip_paths then build_paths: tile every path */
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
@@ -422,7 +428,104 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
         const d9_path *c = (p->clip >= 0 && (uint32_t)p->clip < nc) ? &clips[p->clip] : NULL;
         d9_prepare(p, c);
     }
+    *clips_out = clips;
+    *draws_out = draws;
+    *n_alpha_out = n_alpha;
+    return 0;
+}
 
+static void d9_free_paths(d9_path *clips, uint32_t nc, d9_path *draws, uint32_t nd) {
+    for (uint32_t i = 0; i < nc; i++) d9_free_path(&clips[i]);
+    for (uint32_t i = 0; i < nd; i++) d9_free_path(&draws[i]);
+    free(clips);
+    free(draws);
+}
+
+int orc_d3d9_build(const orc_d3d9_scene *s, int threads, orc_d3d9_built *out) {
+    memset(out, 0, sizeof(*out));
+    uint32_t nc = s->clip.n_paths, nd = s->draw.n_paths;
+    d9_path *clips, *draws;
+    int32_t n_alpha;
+    if (d9_build_paths(s, threads > 0 ? threads : 1, &clips, &draws, &n_alpha) != 0) return -1;
+    /* fills: AddFillsD3D9 of every clip path, then every draw path */
+    uint64_t nf = 0, ntiles = 0, njobs = 0;
+    for (uint32_t i = 0; i < nc + nd; i++) nf += (i < nc ? &clips[i] : &draws[i - nc])->n_fills;
+    for (uint32_t i = 0; i < nd; i++) {
+        for (int32_t t = 0; t < draws[i].w * draws[i].h; t++)
+            if (draws[i].alpha[t] != D9_NO_TILE || draws[i].backdrop[t] != 0) ntiles++;
+        njobs += draws[i].n_jobs;
+    }
+    out->fills = (uint32_t *)malloc((nf ? nf : 1) * 12);
+    out->tiles = (uint32_t *)malloc((ntiles ? ntiles : 1) * 16);
+    out->clips = (int32_t *)malloc((njobs ? njobs : 1) * 16);
+    /* z-buffer: DenseTileMap over the view box's tile rect (builder.rs:976-987) */
+    int32_t zr[4] = {(int32_t)floorf(s->view_box[0] / (float)D9_TILE),
+                     (int32_t)floorf(s->view_box[1] / (float)D9_TILE),
+                     (int32_t)ceilf(s->view_box[2] / (float)D9_TILE),
+                     (int32_t)ceilf(s->view_box[3] / (float)D9_TILE)};
+    int64_t zw = zr[2] - zr[0] > 0 ? zr[2] - zr[0] : 0, zh = zr[3] - zr[1] > 0 ? zr[3] - zr[1] : 0;
+    out->z = (int32_t *)calloc((zw * zh) > 0 ? zw * zh : 1, sizeof(int32_t));
+    memcpy(out->z_rect, zr, sizeof(zr));
+    if (!out->fills || !out->tiles || !out->clips || !out->z) return -1;
+    for (uint32_t i = 0; i < nc + nd; i++) {
+        const d9_path *p = i < nc ? &clips[i] : &draws[i - nc];
+        for (uint32_t k = 0; k < p->n_fills; k++) {
+            const d9_fill *f = &p->fills[k];
+            uint32_t *w = out->fills + (size_t)out->n_fills++ * 3;
+            w[0] = (uint32_t)f->x0 | ((uint32_t)f->y0 << 16);
+            w[1] = (uint32_t)f->x1 | ((uint32_t)f->y1 << 16);
+            w[2] = (uint32_t)(p->alpha_base + f->local);
+        }
+    }
+    /* DrawTileBatchD3D9 (builder.rs:998-1041): every tile with an alpha tile
+     * or a backdrop, path by path; z = the last occluding path over a solid
+     * tile; clip tiles where both tiles exist */
+    for (uint32_t i = 0; i < nd; i++) {
+        const d9_path *p = &draws[i];
+        uint32_t pc = p->paint < s->n_paints ? p->paint : 0;
+        int occludes = s->n_paints && s->paints_rgba[pc * 4 + 3] == 255;
+        for (int32_t t = 0; t < p->w * p->h; t++) {
+            if (p->alpha[t] == D9_NO_TILE && p->backdrop[t] == 0) continue;
+            int32_t tx = p->rx0 + t % p->w, ty = p->ry0 + t / p->w;
+            uint32_t *w = out->tiles + (size_t)out->n_tiles++ * 4;
+            w[0] = (uint32_t)(uint16_t)tx | ((uint32_t)(uint16_t)ty << 16);
+            w[1] = p->alpha[t] == D9_NO_TILE ? 0xffffffffu : (uint32_t)p->alpha[t];
+            w[2] = i;
+            w[3] = (p->paint & 0xffffu) | ((uint32_t)(p->fill_rule ? 2 : 1) << 16) |
+                   ((uint32_t)(uint8_t)p->backdrop[t] << 24);
+            if (occludes && p->alpha[t] == D9_NO_TILE) {
+                int64_t zx = tx - zr[0], zy = ty - zr[1];
+                if (zx >= 0 && zy >= 0 && zx < zw && zy < zh) {
+                    int32_t *zv = &out->z[zy * zw + zx];
+                    if ((int32_t)i > *zv) *zv = (int32_t)i;
+                }
+            }
+        }
+        for (uint32_t j = 0; j < p->n_jobs; j++) {
+            int32_t *w = out->clips + (size_t)out->n_clips++ * 4;
+            w[0] = p->jobs[j].dest;
+            w[1] = p->jobs[j].dest_backdrop;
+            w[2] = p->jobs[j].src;
+            w[3] = p->jobs[j].src_backdrop;
+        }
+    }
+    out->n_alpha = (uint32_t)n_alpha;
+    d9_free_paths(clips, nc, draws, nd);
+    return 0;
+}
+
+void orc_d3d9_built_free(orc_d3d9_built *b) {
+    free(b->fills); free(b->tiles); free(b->clips); free(b->z);
+    memset(b, 0, sizeof(*b));
+}
+
+int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, int threads) {
+    int nthreads = threads > 0 ? threads : 1;
+    uint32_t nc = s->clip.n_paths, nd = s->draw.n_paths;
+    d9_path *clips, *draws;
+    int32_t n_alpha;
+    double t_start = orc_now();
+    if (d9_build_paths(s, nthreads, &clips, &draws, &n_alpha) != 0) return -1;
     double t_tiled = orc_now();
     orc_stage_seconds[ORC_STAGE_D3D9_TILER] += t_tiled - t_start;
     /* masks: binary16 RGBA, 64 texels per alpha tile, cleared */
@@ -527,8 +630,6 @@ int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *s, uint32_t *fb, i
     }
     orc_stage_seconds[ORC_STAGE_D3D9_COMPOSITE] += orc_now() - t_masked;
     free(base); free(lists); free(start); free(counts); free(z); free(mask);
-    for (uint32_t i = 0; i < nc; i++) d9_free_path(&clips[i]);
-    for (uint32_t i = 0; i < nd; i++) d9_free_path(&draws[i]);
-    free(clips); free(draws);
+    d9_free_paths(clips, nc, draws, nd);
     return 0;
 }

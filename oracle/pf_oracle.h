@@ -196,6 +196,25 @@ typedef struct orc_d3d9_scene {
  * scene rows) with the LUT of ctx. Returns 0 on success. */
 int orc_d3d9_render(const orc_ctx *ctx, const orc_d3d9_scene *scene, uint32_t *fb, int threads);
 
+/* The render commands of the D3D9 level before any raster: every fill
+ * (Fill: LineSegmentU16 from x, y, to x, y, then the alpha tile: 3 words),
+ * the draw batch's tiles (TileObjectPrimitive: 4 words), its clip tiles
+ * (Clip: 4 words) and its z-buffer (the view box's tile rect, row by row).
+ * Arrays are malloc'd; release them with orc_d3d9_built_free. */
+typedef struct orc_d3d9_built {
+    uint32_t *fills;
+    uint32_t n_fills;
+    uint32_t *tiles;
+    uint32_t n_tiles;
+    int32_t *clips;
+    uint32_t n_clips;
+    int32_t *z;
+    int32_t z_rect[4];
+    uint32_t n_alpha;
+} orc_d3d9_built;
+int orc_d3d9_build(const orc_d3d9_scene *scene, int threads, orc_d3d9_built *out);
+void orc_d3d9_built_free(orc_d3d9_built *built);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@ STATUS_NAMES = {
     6: "PFH_ERROR_LIMIT",
 }
 
+RENDERER_LEVEL_D3D9 = 1
 RENDERER_LEVEL_D3D11 = 2
 PATH_SOURCE_DRAW = 0
 PATH_SOURCE_CLIP = 1
@@ -40,6 +41,9 @@ COMMAND_DRAW_TILES_D3D11 = 4
 COMMAND_FINISH = 5
 COMMAND_ALLOCATE_TEXTURE_PAGE = 6
 COMMAND_UPLOAD_TEXEL_DATA = 7
+COMMAND_ADD_FILLS_D3D9 = 8
+COMMAND_FLUSH_FILLS_D3D9 = 9
+COMMAND_DRAW_TILES_D3D9 = 10
 
 DEBUG_MICROLINES = 0
 DEBUG_FILLS = 1
@@ -127,6 +131,22 @@ class TextureMetadataEntry(C.Structure):
                 ("base_color", ColorU), ("filter", Filter), ("blend_mode", C.c_int32)]
 
 
+class FillD3D9(C.Structure):
+    _fields_ = [("from_x", C.c_uint16), ("from_y", C.c_uint16), ("to_x", C.c_uint16),
+                ("to_y", C.c_uint16), ("alpha_tile_index", C.c_uint32)]
+
+
+class TileObjectD3D9(C.Structure):
+    _fields_ = [("tile_x", C.c_int16), ("tile_y", C.c_int16), ("alpha_tile_id", C.c_uint32),
+                ("path_id", C.c_uint32), ("color", C.c_uint16), ("ctrl", C.c_uint8),
+                ("backdrop", C.c_int8)]
+
+
+class ClipD3D9(C.Structure):
+    _fields_ = [("dest_tile_id", C.c_uint32), ("dest_backdrop", C.c_int32),
+                ("src_tile_id", C.c_uint32), ("src_backdrop", C.c_int32)]
+
+
 class ColorStop(C.Structure):
     _fields_ = [("offset", C.c_float), ("color", ColorU)]
 
@@ -145,6 +165,11 @@ class Pattern(C.Structure):
 class RectI(C.Structure):
     _fields_ = [("min_x", C.c_int32), ("min_y", C.c_int32), ("max_x", C.c_int32),
                 ("max_y", C.c_int32)]
+
+
+class DrawTileBatchD3D9(C.Structure):
+    _fields_ = [("tiles", C.c_void_p), ("tile_count", C.c_uint32), ("clips", C.c_void_p),
+                ("clip_count", C.c_uint32), ("z_buffer", C.c_void_p), ("z_buffer_rect", RectI)]
 
 
 class ContextOptions(C.Structure):
@@ -260,8 +285,16 @@ SIGNATURES = {
     "pfh_command_texture_page": (I32, [P, U32, C.POINTER(U32), C.POINTER(I32),
                                        C.POINTER(I32)]),
     "pfh_command_texel_data": (I32, [P, U32, C.POINTER(U32), C.POINTER(RectI), C.POINTER(P)]),
+    "pfh_command_fills_d3d9": (I32, [P, U32, C.POINTER(P), C.POINTER(U32)]),
+    "pfh_command_batch_d3d9": (I32, [P, U32, C.POINTER(DrawTileBatchD3D9)]),
+    "pfh_scene_build_d3d9": (I32, [P, C.POINTER(BuildOptions), C.POINTER(P)]),
+    "pfh_d3d9_add_fills": (I32, [P, P, U32]),
+    "pfh_d3d9_flush_fills": (I32, [P]),
+    "pfh_d3d9_draw_tiles": (I32, [P, C.POINTER(DrawTileBatchD3D9)]),
     "pfh_render_commands": (I32, [P, P]),
     "pfh_scene_build_and_render": (I32, [P, P, C.POINTER(FrameOptions),
+                                         C.POINTER(BuildOptions)]),
+    "pfh_scene_build_and_render_d3d9": (I32, [P, P, C.POINTER(FrameOptions),
                                          C.POINTER(BuildOptions)]),
 }
 

@@ -60,6 +60,8 @@ def build(verbose=False):
           "-o", obj("pfh_scene")], verbose)
     _run([cxx, *COMMON, "-ffp-contract=off", "-c", os.path.join(CSRC, "pfh_palette.cpp"),
           "-o", obj("pfh_palette")], verbose)
+    _run([cxx, *COMMON, "-ffp-contract=off", "-pthread", "-c", os.path.join(CSRC, "pfh_d3d9.cpp"),
+          "-o", obj("pfh_d3d9")], verbose)
     _run([cxx, *COMMON, "-c", os.path.join(CSRC, "pfh_svg.cpp"), "-o", obj("pfh_svg")],
          verbose)
     _run([cxx, *COMMON, "-c", os.path.join(CSRC, "pfh_capi.cpp"), "-o", obj("pfh_capi")],

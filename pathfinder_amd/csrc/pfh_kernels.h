@@ -209,6 +209,39 @@ void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s);
 // binary16 metadata texels to floats (exact)
 void launch_metadata_expand(const uint16_t *halves, float4 *metadata, uint32_t texels, hipStream_t s);
 
+// The D3D9 level (gpu/d3d9/renderer.rs): mask raster, clip combine and
+// tile composite. The mask holds 256 binary16 values per alpha tile.
+struct D3D9FillArgs {
+    const uint2 *fills;        // Fill words 0-1, grouped by alpha tile in fill order
+    const uint4 *jobs;         // per job: alpha tile, first fill, end fill, 0
+    uint32_t job_count;
+    uint16_t *mask;
+    uint32_t alpha_capacity;   // alpha tiles the mask holds
+    const float4 *lut;
+};
+struct D3D9ClipArgs {
+    const int4 *clips;         // Clip: dest, dest backdrop, src, src backdrop
+    uint32_t clip_count;
+    uint16_t *mask;
+    uint32_t alpha_capacity;
+};
+struct D3D9TileArgs {
+    const uint4 *tiles;        // TileObjectPrimitive words
+    const uint32_t *list_offsets;  // per framebuffer tile, plus the end
+    const uint32_t *list;      // tile indices per framebuffer tile, in draw order
+    const uint16_t *mask;
+    uint32_t alpha_capacity;
+    const float4 *metadata;    // 10 texels per paint; texel 2 is the base colour
+    uint32_t paint_count;
+    uint32_t *framebuffer;
+    int32_t fb_width_tiles, fb_height_tiles;
+    int32_t load;
+    float clear[4];
+};
+void launch_d3d9_fill(const D3D9FillArgs &a, hipStream_t s);
+void launch_d3d9_clip(const D3D9ClipArgs &a, hipStream_t s);
+void launch_d3d9_tile(const D3D9TileArgs &a, hipStream_t s);
+
 // Scratch words needed by the scans of a batch.
 size_t scan_partials_words(uint32_t max_elements);
 

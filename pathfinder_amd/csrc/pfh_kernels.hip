@@ -2989,6 +2989,140 @@ __global__ void k_metadata_expand(const uint16_t *halves, float4 *metadata, uint
                               __half2float(h[3]));
 }
 
+// ------------------------------------------------------------------ D3D9 level
+//
+// The raster half of RendererD3D9 (gpu/d3d9/renderer.rs:150-472) on compute:
+// d3d9/fill.{vs,fs}.glsl draws each fill as a quad over pixel columns
+// floor(min x) .. ceil(max x) - 1 and mask texel rows floor(floor(min y) / 4)
+// .. 3 of its alpha tile, and additive blending into the RGBA16F mask adds
+// each fragment's computeCoverage to the texel, rounding the sum to binary16
+// in fill order. One wavefront per alpha tile: lane (x, r) owns mask texel
+// r * 16 + x (pixel column x, pixel rows 4r .. 4r + 3 in its four channels)
+// and walks the tile's fills in order.
+
+__device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ uint16_t f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+
+__global__ void __launch_bounds__(kBlock) k_d3d9_fill(D3D9FillArgs a) {
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= a.job_count) return;
+    uint32_t lane = lane_id(), x = lane & 15u, r = lane >> 4;
+    uint4 job = a.jobs[wave];  // alpha tile, first fill, end
+    if (job.x >= a.alpha_capacity) return;
+    LutRef lut = lut_ref(a.lut);
+    uint2 *texel = reinterpret_cast<uint2 *>(a.mask + (size_t)job.x * 256u) + lane;
+    uint2 w = *texel;
+    uint16_t m[4] = {(uint16_t)(w.x & 0xffffu), (uint16_t)(w.x >> 16), (uint16_t)(w.y & 0xffffu),
+                     (uint16_t)(w.y >> 16)};
+    float fragx = (float)x + 0.5f, fragy = (float)(4u * r) + 0.5f;
+    for (uint32_t k = job.y; k < job.z; k++) {
+        uint2 fw = a.fills[k];
+        FillRecord f;
+        FillColumns c = make_record(fw, f);
+        float miny = fminf((float)(fw.x >> 16), (float)(fw.y >> 16)) * (1.0f / 256.0f);
+        int32_t r0 = (int32_t)floorf(floorf(miny) * 0.25f);
+        if (x < c.c0 || x > c.c1 || (int32_t)r < r0) continue;
+        float4 cov = coverage_record(lut, f, fragx, fragy);
+        m[0] = f2h(h2f(m[0]) + cov.x);
+        m[1] = f2h(h2f(m[1]) + cov.y);
+        m[2] = f2h(h2f(m[2]) + cov.z);
+        m[3] = f2h(h2f(m[3]) + cov.w);
+    }
+    *texel = make_uint2((uint32_t)m[0] | ((uint32_t)m[1] << 16),
+                        (uint32_t)m[2] | ((uint32_t)m[3] << 16));
+}
+
+// tile_clip_copy + tile_clip_combine (d3d9/tile_clip_*.glsl): dest texel =
+// min(|dest + dest backdrop|, |src + src backdrop|). A draw path's alpha tile
+// is the dest of at most one clip and no clip path's tile is a dest, so the
+// sources read here are the values before any combine. One thread per
+// (clip, texel).
+__global__ void __launch_bounds__(kBlock) k_d3d9_clip(D3D9ClipArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.clip_count * 64u) return;
+    int4 c = a.clips[i >> 6];
+    uint32_t texel = i & 63u;
+    uint32_t dest = (uint32_t)c.x, src = (uint32_t)c.z;
+    if (dest >= a.alpha_capacity || src >= a.alpha_capacity) return;
+    uint2 *d = reinterpret_cast<uint2 *>(a.mask + (size_t)dest * 256u) + texel;
+    const uint2 *sp = reinterpret_cast<const uint2 *>(a.mask + (size_t)src * 256u) + texel;
+    uint2 dw = *d, sw = *sp;
+    uint16_t dh[4] = {(uint16_t)(dw.x & 0xffffu), (uint16_t)(dw.x >> 16),
+                      (uint16_t)(dw.y & 0xffffu), (uint16_t)(dw.y >> 16)};
+    uint16_t sh[4] = {(uint16_t)(sw.x & 0xffffu), (uint16_t)(sw.x >> 16),
+                      (uint16_t)(sw.y & 0xffffu), (uint16_t)(sw.y >> 16)};
+    uint16_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float dv = fabsf(h2f(dh[k]) + (float)c.y);
+        float sv = fabsf(h2f(sh[k]) + (float)c.w);
+        o[k] = f2h(sv < dv ? sv : dv);
+    }
+    *d = make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+}
+
+// d3d9/tile.{vs,fs}.glsl with calculateColor for solid paints: the batch's
+// tiles over one framebuffer tile, in draw order (z-culled tiles already
+// left out of the list), each blended src-over into the RGBA8 target and
+// rounded to 8 bits before the next. One wavefront per framebuffer tile,
+// lane (x, r) the pixels of column x, rows 4r .. 4r + 3 (its mask texel).
+__global__ void __launch_bounds__(kBlock) k_d3d9_tile(D3D9TileArgs a) {
+    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (wave >= fbn) return;
+    uint32_t begin = a.list_offsets[wave], end = a.list_offsets[wave + 1];
+    if (a.load && begin == end) return;
+    uint32_t lane = lane_id(), x = lane & 15u, r = lane >> 4;
+    uint32_t gx = wave % (uint32_t)a.fb_width_tiles, gy = wave / (uint32_t)a.fb_width_tiles;
+    size_t pitch = (size_t)a.fb_width_tiles * 16u;
+    uint32_t *px = a.framebuffer + (size_t)(gy * 16u + r * 4u) * pitch + gx * 16u + x;
+    uint32_t d8[4];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+        if (a.load) {
+            d8[sub] = px[sub * pitch];
+        } else {
+            d8[sub] = unorm8(a.clear[0]) | (unorm8(a.clear[1]) << 8) | (unorm8(a.clear[2]) << 16) |
+                      (unorm8(a.clear[3]) << 24);
+        }
+    }
+    for (uint32_t e = begin; e < end; e++) {
+        uint4 t = a.tiles[a.list[e]];  // TileObjectPrimitive words
+        uint32_t alpha = t.y;
+        uint32_t color = t.w & 0xffffu;
+        bool even_odd = ((t.w >> 16) & 0x2u) != 0;
+        float backdrop = (float)((int32_t)t.w >> 24);
+        uint32_t paint = color < a.paint_count ? color : 0u;
+        float4 base = a.paint_count ? a.metadata[(size_t)paint * 10u + 2u]
+                                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint2 mw = make_uint2(0u, 0u);
+        if (alpha < a.alpha_capacity)
+            mw = reinterpret_cast<const uint2 *>(a.mask + (size_t)alpha * 256u)[lane];
+        uint16_t mh[4] = {(uint16_t)(mw.x & 0xffffu), (uint16_t)(mw.x >> 16),
+                          (uint16_t)(mw.y & 0xffffu), (uint16_t)(mw.y >> 16)};
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            float cov = h2f(mh[sub]) + backdrop;
+            if (!even_odd) {
+                cov = fabsf(cov);
+            } else {
+                float m = cov - 2.0f * floorf(cov / 2.0f);
+                cov = 1.0f - fabsf(1.0f - m);
+            }
+            float mask_alpha = cov < 1.0f ? cov : 1.0f;
+            float al = base.w * mask_alpha;
+            float ia = 1.0f - al;
+            uint32_t v = d8[sub];
+            float d0 = (float)(v & 0xffu) / 255.0f, d1 = (float)((v >> 8) & 0xffu) / 255.0f;
+            float d2 = (float)((v >> 16) & 0xffu) / 255.0f, d3 = (float)(v >> 24) / 255.0f;
+            d8[sub] = unorm8(base.x * al + d0 * ia) | (unorm8(base.y * al + d1 * ia) << 8) |
+                      (unorm8(base.z * al + d2 * ia) << 16) | (unorm8(al + d3 * ia) << 24);
+        }
+    }
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) px[sub * pitch] = d8[sub];
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -3175,6 +3309,23 @@ void launch_framebuffer_to_float(const uint32_t *fb, uint32_t pitch_words, uint3
 void launch_lut_expand(const uint8_t *rgba, float4 *lut, hipStream_t s) {
     hipLaunchKernelGGL(k_lut_expand, dim3((kLutRows * kLutStride + 255) / 256), dim3(256), 0, s,
                        rgba, lut);
+}
+
+void launch_d3d9_fill(const D3D9FillArgs &a, hipStream_t s) {
+    if (!a.job_count) return;
+    hipLaunchKernelGGL(k_d3d9_fill, dim3((a.job_count + 3) / 4), dim3(kBlock), 0, s, a);
+}
+
+void launch_d3d9_clip(const D3D9ClipArgs &a, hipStream_t s) {
+    if (!a.clip_count) return;
+    hipLaunchKernelGGL(k_d3d9_clip, dim3((a.clip_count * 64u + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, a);
+}
+
+void launch_d3d9_tile(const D3D9TileArgs &a, hipStream_t s) {
+    uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
+    if (!fbn) return;
+    hipLaunchKernelGGL(k_d3d9_tile, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
 }
 
 void launch_metadata_expand(const uint16_t *halves, float4 *metadata, uint32_t texels,

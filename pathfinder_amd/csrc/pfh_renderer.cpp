@@ -339,6 +339,14 @@ struct pfh_context {
     bool has_background = false;
     float background[4] = {0, 0, 0, 0};
     bool dest_dirty = false;
+    // the D3D9 level: queued fills, the frame's alpha tiles and the binary16
+    // mask (256 halves per alpha tile; the leading d9_mask_zeroed tiles are
+    // cleared for this frame)
+    std::vector<pfh_fill_d3d9> d9_pending;
+    uint32_t d9_alpha_count = 0;
+    uint32_t d9_mask_zeroed = 0;
+    uint16_t *d_d9_mask = nullptr;
+    uint32_t d9_mask_cap = 0;
     uint32_t *d_fb = nullptr, *d_fb_saved = nullptr;
     size_t fb_words = 0;
     uint32_t *d_mask = nullptr;
@@ -1048,7 +1056,8 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     pfh_context_options opt = {};
     opt.level = PFH_RENDERER_LEVEL_D3D11;
     if (options) opt = *options;
-    if (opt.level != PFH_RENDERER_LEVEL_D3D11) return PFH_ERROR_INVALID_ARGUMENT;
+    if (opt.level != PFH_RENDERER_LEVEL_D3D11 && opt.level != PFH_RENDERER_LEVEL_D3D9)
+        return PFH_ERROR_INVALID_ARGUMENT;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= opt.device || opt.device < 0)
         return PFH_ERROR_NO_DEVICE;
@@ -1136,6 +1145,9 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
     ctx->background[2] = frame->background_color.b;
     ctx->background[3] = frame->background_color.a;
     ctx->dest_dirty = false;
+    ctx->d9_pending.clear();
+    ctx->d9_alpha_count = 0;
+    ctx->d9_mask_zeroed = 0;
     // A frame that starts by loading the destination must start from the same
     // pixels when it is rendered again.
     if (!ctx->has_background) {
@@ -1154,6 +1166,174 @@ pfh_status pfh_begin_scene(pfh_context *ctx, const pfh_frame_options *frame) {
     if ((s = start_frame_on_side_streams(ctx)) != PFH_OK) return s;
     std::memset(&ctx->stats, 0, sizeof(ctx->stats));
     ctx->in_scene = true;
+    return PFH_OK;
+}
+
+pfh_status pfh_d3d9_add_fills(pfh_context *ctx, const pfh_fill_d3d9 *fills, uint32_t count) {
+    if (!ctx || (count && !fills)) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "add_fills outside a scene");
+    for (uint32_t i = 0; i < count; i++) {
+        if (fills[i].alpha_tile_index >= (1u << 24))
+            return fail(ctx, PFH_ERROR_LIMIT, "alpha tile %u beyond 2^24",
+                        fills[i].alpha_tile_index);
+        ctx->d9_alpha_count = std::max(ctx->d9_alpha_count, fills[i].alpha_tile_index + 1);
+    }
+    ctx->d9_pending.insert(ctx->d9_pending.end(), fills, fills + count);
+    return PFH_OK;
+}
+
+namespace {
+// Mask storage for the frame's alpha tiles: grown keeping what earlier
+// flushes drew, the tiles not yet cleared this frame zeroed.
+pfh_status d9_ensure_mask(pfh_context *ctx, uint32_t tiles) {
+    if (tiles > ctx->d9_mask_cap) {
+        uint32_t cap = std::max<uint32_t>(ctx->d9_mask_cap ? ctx->d9_mask_cap : 4096, 4096);
+        while (cap < tiles) cap *= 2;
+        uint16_t *m = nullptr;
+        pfh_status s = dalloc(ctx, &m, (size_t)cap * 256);
+        if (s != PFH_OK) return s;
+        if (ctx->d9_mask_zeroed)
+            HIP_TRY(ctx, hipMemcpyAsync(m, ctx->d_d9_mask, (size_t)ctx->d9_mask_zeroed * 512,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        dfree(ctx, ctx->d_d9_mask);
+        ctx->d_d9_mask = m;
+        ctx->d9_mask_cap = cap;
+    }
+    if (tiles > ctx->d9_mask_zeroed) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_d9_mask + (size_t)ctx->d9_mask_zeroed * 256, 0,
+                                    (size_t)(tiles - ctx->d9_mask_zeroed) * 512, ctx->stream));
+        ctx->d9_mask_zeroed = tiles;
+    }
+    return PFH_OK;
+}
+}  // namespace
+
+pfh_status pfh_d3d9_flush_fills(pfh_context *ctx) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (!ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "flush_fills outside a scene");
+    if (ctx->d9_pending.empty()) return PFH_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    pfh_status s = d9_ensure_mask(ctx, ctx->d9_alpha_count);
+    if (s != PFH_OK) return s;
+    // group the fills by alpha tile, keeping their order (a stable counting
+    // sort): one job per alpha tile that has fills
+    const std::vector<pfh_fill_d3d9> &f = ctx->d9_pending;
+    std::vector<uint32_t> count(ctx->d9_alpha_count + 1, 0);
+    for (const pfh_fill_d3d9 &x : f) count[x.alpha_tile_index + 1]++;
+    std::vector<uint4> jobs;
+    for (uint32_t t = 0; t < ctx->d9_alpha_count; t++) {
+        if (count[t + 1]) jobs.push_back(make_uint4(t, count[t], count[t] + count[t + 1], 0));
+        count[t + 1] += count[t];
+    }
+    std::vector<uint2> grouped(f.size());
+    for (const pfh_fill_d3d9 &x : f)
+        grouped[count[x.alpha_tile_index]++] =
+            make_uint2((uint32_t)x.from_x | ((uint32_t)x.from_y << 16),
+                       (uint32_t)x.to_x | ((uint32_t)x.to_y << 16));
+    uint2 *d_fills = nullptr;
+    uint4 *d_jobs = nullptr;
+    if ((s = upload(ctx, &d_fills, grouped)) != PFH_OK) return s;
+    if ((s = upload(ctx, &d_jobs, jobs)) != PFH_OK) return s;
+    pfh::D3D9FillArgs a;
+    a.fills = d_fills;
+    a.jobs = d_jobs;
+    a.job_count = (uint32_t)jobs.size();
+    a.mask = ctx->d_d9_mask;
+    a.alpha_capacity = ctx->d9_mask_cap;
+    a.lut = ctx->d_lut;
+    pfh::launch_d3d9_fill(a, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d_fills);
+    dfree(ctx, d_jobs);
+    ctx->stats.fill_count += (uint32_t)f.size();
+    ctx->d9_pending.clear();
+    return PFH_OK;
+}
+
+pfh_status pfh_d3d9_draw_tiles(pfh_context *ctx, const pfh_draw_tile_batch_d3d9 *batch) {
+    if (!ctx || !batch || (batch->tile_count && !batch->tiles) ||
+        (batch->clip_count && !batch->clips))
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "draw_tiles outside a scene");
+    const pfh_rect_i &zr = batch->z_buffer_rect;
+    int64_t zw = std::max(zr.max_x - zr.min_x, 0), zh = std::max(zr.max_y - zr.min_y, 0);
+    if (zw * zh && !batch->z_buffer) return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "no z-buffer");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    pfh_status s = pfh_d3d9_flush_fills(ctx);
+    if (s != PFH_OK) return s;
+    if ((s = d9_ensure_mask(ctx, ctx->d9_alpha_count)) != PFH_OK) return s;
+    // clip_tiles
+    if (batch->clip_count) {
+        std::vector<int4> clips(batch->clip_count);
+        std::memcpy(clips.data(), batch->clips, clips.size() * sizeof(int4));
+        int4 *d_clips = nullptr;
+        if ((s = upload(ctx, &d_clips, clips)) != PFH_OK) return s;
+        pfh::D3D9ClipArgs c;
+        c.clips = d_clips;
+        c.clip_count = batch->clip_count;
+        c.mask = ctx->d_d9_mask;
+        c.alpha_capacity = std::min(ctx->d9_mask_cap, ctx->d9_mask_zeroed);
+        pfh::launch_d3d9_clip(c, ctx->stream);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        dfree(ctx, d_clips);
+    }
+    // draw_tiles: the tiles over each framebuffer tile in draw order, a tile
+    // below its z value left out (tile.vs.glsl's z test)
+    uint32_t fbw = (uint32_t)ctx->fbw, fbh = (uint32_t)ctx->fbh;
+    size_t fbn = (size_t)fbw * fbh;
+    std::vector<uint32_t> offsets(fbn + 1, 0), keep;
+    keep.reserve(batch->tile_count);
+    auto fb_index = [&](const pfh_tile_object_d3d9 &t, size_t *g) {
+        if (t.tile_x < 0 || t.tile_y < 0 || (uint32_t)t.tile_x >= fbw || (uint32_t)t.tile_y >= fbh)
+            return false;
+        int64_t zx = t.tile_x - zr.min_x, zy = t.tile_y - zr.min_y;
+        int32_t z = (zx >= 0 && zy >= 0 && zx < zw && zy < zh) ? batch->z_buffer[zy * zw + zx] : 0;
+        if ((int64_t)t.path_id < (int64_t)z) return false;
+        *g = (size_t)t.tile_y * fbw + (size_t)t.tile_x;
+        return true;
+    };
+    for (uint32_t i = 0; i < batch->tile_count; i++) {
+        size_t g;
+        if (fb_index(batch->tiles[i], &g)) offsets[g + 1]++;
+    }
+    for (size_t g = 0; g < fbn; g++) offsets[g + 1] += offsets[g];
+    std::vector<uint32_t> list(offsets[fbn]), fill_pos(offsets.begin(), offsets.end() - 1);
+    for (uint32_t i = 0; i < batch->tile_count; i++) {
+        size_t g;
+        if (fb_index(batch->tiles[i], &g)) list[fill_pos[g]++] = i;
+    }
+    std::vector<uint4> tiles(batch->tile_count);
+    std::memcpy(tiles.data(), batch->tiles, tiles.size() * sizeof(uint4));
+    uint4 *d_tiles = nullptr;
+    uint32_t *d_offsets = nullptr, *d_list = nullptr;
+    if ((s = upload(ctx, &d_tiles, tiles)) != PFH_OK) return s;
+    if ((s = upload(ctx, &d_offsets, offsets)) != PFH_OK) return s;
+    if ((s = upload(ctx, &d_list, list)) != PFH_OK) return s;
+    pfh::D3D9TileArgs t;
+    t.tiles = d_tiles;
+    t.list_offsets = d_offsets;
+    t.list = d_list;
+    t.mask = ctx->d_d9_mask;
+    t.alpha_capacity = std::min(ctx->d9_mask_cap, ctx->d9_mask_zeroed);
+    t.metadata = ctx->d_metadata;
+    t.paint_count = ctx->paint_count;
+    t.framebuffer = ctx->d_fb;
+    t.fb_width_tiles = (int32_t)fbw;
+    t.fb_height_tiles = (int32_t)fbh;
+    t.load = (ctx->dest_dirty || !ctx->has_background) ? 1 : 0;
+    for (int i = 0; i < 4; i++) t.clear[i] = t.load ? 0.0f : ctx->background[i];
+    ctx->dest_dirty = true;
+    pfh::launch_d3d9_tile(t, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, d_tiles);
+    dfree(ctx, d_offsets);
+    dfree(ctx, d_list);
+    ctx->stats.total_tile_count += batch->tile_count;
+    ctx->stats.alpha_tile_count = ctx->d9_alpha_count;
     return PFH_OK;
 }
 

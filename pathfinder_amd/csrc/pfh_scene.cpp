@@ -15,6 +15,7 @@
 // and emits the same RenderCommand stream (gpu_data.rs:37-105).
 #include "pfh_internal.h"
 #include "pfh_palette.h"
+#include "pfh_d3d9.h"
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +24,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -167,6 +169,9 @@ struct pfh_command_list {
     std::vector<pfh_segment_indices> draw_indices, clip_indices;
     std::vector<pfh::BatchData> batches;
     uint32_t path_count = 0;
+    // the D3D9 level: fills per path (ADD_FILLS_D3D9 slots) and the batch
+    std::vector<std::vector<pfh_fill_d3d9>> fills_d3d9;
+    pfh9::Built batch_d3d9;
 };
 
 namespace {
@@ -591,6 +596,88 @@ pfh_status pfh_scene_build(pfh_scene *scene, const pfh_build_options *options,
     return PFH_OK;
 }
 
+pfh_status pfh_scene_build_d3d9(pfh_scene *scene, const pfh_build_options *options,
+                                pfh_command_list **out) {
+    if (!scene || !out) return PFH_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    for (const pfh::PaintDesc &d : scene->paint_descs)
+        if (d.overlay != pfh::OVERLAY_NONE) return PFH_ERROR_INVALID_ARGUMENT;
+    for (const PathRecord &r : scene->draw.paths)
+        if (r.blend_mode != PFH_BLEND_MODE_SRC_OVER) return PFH_ERROR_INVALID_ARGUMENT;
+    std::unique_ptr<pfh_command_list> list(new (std::nothrow) pfh_command_list());
+    if (!list) return PFH_ERROR_OUT_OF_MEMORY;
+    pfh_transform2f transform = options ? options->transform : pfh_transform2f{1, 0, 0, 1, 0, 0};
+    list->path_count = (uint32_t)(scene->clip.paths.size() + scene->draw.paths.size());
+    list->commands.push_back({PFH_COMMAND_START, 0});
+    pfh::PaletteBuild palette;
+    pfh::build_palette(scene->paint_descs, transform, &palette);
+    list->paints = scene->paints;
+    list->metadata = palette.metadata;
+    list->commands.push_back({PFH_COMMAND_UPLOAD_TEXTURE_METADATA, 0});
+    // the outlines packed as SegmentsD3D11 are the segments the tiler walks
+    // (each contour closed back to its first point)
+    std::vector<pfh9::PathInput> clips(scene->clip.paths.size()), draws(scene->draw.paths.size());
+    for (size_t i = 0; i < clips.size(); i++) {
+        add_segments(scene->clip, scene->clip.paths[i], list->clip_points, list->clip_indices,
+                     &clips[i].seg_begin, &clips[i].seg_end);
+        clips[i].fill_rule = scene->clip.paths[i].fill_rule;
+        clips[i].clip_path = -1;  // a clip path's clip is not applied at this level
+        clips[i].paint = 0;
+        clips[i].occludes = false;
+    }
+    for (size_t i = 0; i < draws.size(); i++) {
+        const PathRecord &r = scene->draw.paths[i];
+        add_segments(scene->draw, r, list->draw_points, list->draw_indices, &draws[i].seg_begin,
+                     &draws[i].seg_end);
+        draws[i].fill_rule = r.fill_rule;
+        draws[i].clip_path = r.clip_path;
+        draws[i].paint = r.paint;
+        draws[i].occludes = r.paint < palette.opaque.size() && palette.opaque[r.paint];
+    }
+    const float vb[4] = {scene->view_box.x0, scene->view_box.y0, scene->view_box.x1,
+                         scene->view_box.y1};
+    unsigned hw = std::thread::hardware_concurrency();
+    int threads = (int)std::min(16u, std::max(1u, hw));
+    pfh9::build(list->clip_points.data(), list->clip_indices.data(), clips,
+                list->draw_points.data(), list->draw_indices.data(), draws, transform, vb,
+                threads, &list->batch_d3d9);
+    list->fills_d3d9 = std::move(list->batch_d3d9.fills);
+    for (size_t i = 0; i < list->fills_d3d9.size(); i++)
+        if (!list->fills_d3d9[i].empty())
+            list->commands.push_back({PFH_COMMAND_ADD_FILLS_D3D9, (uint32_t)i});
+    list->commands.push_back({PFH_COMMAND_FLUSH_FILLS_D3D9, 0});
+    if (!scene->draw.paths.empty()) list->commands.push_back({PFH_COMMAND_DRAW_TILES_D3D9, 0});
+    list->commands.push_back({PFH_COMMAND_FINISH, 0});
+    *out = list.release();
+    return PFH_OK;
+}
+
+pfh_status pfh_command_fills_d3d9(const pfh_command_list *list, uint32_t index,
+                                  const pfh_fill_d3d9 **fills, uint32_t *count) {
+    if (!list || !fills || !count || index >= list->commands.size() ||
+        list->commands[index].kind != PFH_COMMAND_ADD_FILLS_D3D9)
+        return PFH_ERROR_INVALID_ARGUMENT;
+    const std::vector<pfh_fill_d3d9> &f = list->fills_d3d9[list->commands[index].batch_slot];
+    *fills = f.data();
+    *count = (uint32_t)f.size();
+    return PFH_OK;
+}
+
+pfh_status pfh_command_batch_d3d9(const pfh_command_list *list, uint32_t index,
+                                  pfh_draw_tile_batch_d3d9 *out) {
+    if (!list || !out || index >= list->commands.size() ||
+        list->commands[index].kind != PFH_COMMAND_DRAW_TILES_D3D9)
+        return PFH_ERROR_INVALID_ARGUMENT;
+    const pfh9::Built &b = list->batch_d3d9;
+    out->tiles = b.tiles.data();
+    out->tile_count = (uint32_t)b.tiles.size();
+    out->clips = b.clips.data();
+    out->clip_count = (uint32_t)b.clips.size();
+    out->z_buffer = b.z.data();
+    out->z_buffer_rect = b.z_rect;
+    return PFH_OK;
+}
+
 void pfh_command_list_destroy(pfh_command_list *list) { delete list; }
 
 uint32_t pfh_command_list_len(const pfh_command_list *list) {
@@ -697,6 +784,20 @@ pfh_status pfh_render_commands(pfh_context *ctx, const pfh_command_list *list) {
             s = pfh_upload_texel_data(ctx, u.page, &u.rect, u.texels.data());
             break;
         }
+        case PFH_COMMAND_ADD_FILLS_D3D9: {
+            const std::vector<pfh_fill_d3d9> &f = list->fills_d3d9[c.batch_slot];
+            s = pfh_d3d9_add_fills(ctx, f.data(), (uint32_t)f.size());
+            break;
+        }
+        case PFH_COMMAND_FLUSH_FILLS_D3D9:
+            s = pfh_d3d9_flush_fills(ctx);
+            break;
+        case PFH_COMMAND_DRAW_TILES_D3D9: {
+            pfh_draw_tile_batch_d3d9 b;
+            pfh_command_batch_d3d9(list, (uint32_t)i, &b);
+            s = pfh_d3d9_draw_tiles(ctx, &b);
+            break;
+        }
         case PFH_COMMAND_UPLOAD_SCENE_D3D11: {
             pfh_segments d, k;
             pfh_command_segments(list, (uint32_t)i, &d, &k);
@@ -729,6 +830,20 @@ pfh_status pfh_scene_build_and_render(pfh_scene *scene, pfh_context *ctx,
     if (!scene || !ctx || !frame) return PFH_ERROR_INVALID_ARGUMENT;
     pfh_command_list *list = nullptr;
     pfh_status s = pfh_scene_build(scene, options, &list);
+    if (s != PFH_OK) return s;
+    s = pfh_begin_scene(ctx, frame);
+    if (s == PFH_OK) s = pfh_render_commands(ctx, list);
+    if (s == PFH_OK) s = pfh_end_scene(ctx);
+    pfh_command_list_destroy(list);
+    return s;
+}
+
+pfh_status pfh_scene_build_and_render_d3d9(pfh_scene *scene, pfh_context *ctx,
+                                           const pfh_frame_options *frame,
+                                           const pfh_build_options *options) {
+    if (!scene || !ctx || !frame) return PFH_ERROR_INVALID_ARGUMENT;
+    pfh_command_list *list = nullptr;
+    pfh_status s = pfh_scene_build_d3d9(scene, options, &list);
     if (s != PFH_OK) return s;
     s = pfh_begin_scene(ctx, frame);
     if (s == PFH_OK) s = pfh_render_commands(ctx, list);

@@ -55,6 +55,7 @@ class BlendMode(enum.IntEnum):
 
 
 class RendererLevel(enum.IntEnum):
+    D3D9 = L.RENDERER_LEVEL_D3D9
     D3D11 = L.RENDERER_LEVEL_D3D11
 
 
@@ -565,21 +566,23 @@ class Scene:
                                           _ptr(a.blend_modes))
         return h
 
-    def build(self, options=None):
-        """SceneBuilder::build (builder.rs:148-222) -> CommandList."""
+    def build(self, options=None, level=RendererLevel.D3D11):
+        """SceneBuilder::build (builder.rs:148-222) -> CommandList, for the
+        D3D11 level (GPU binning) or the D3D9 level (the CPU tiler)."""
         lib = L.load()
         h = self._native()
         try:
             bo = L.BuildOptions((options or BuildOptions()).transform.to_c())
             out = C.c_void_p()
-            L.check(lib.pfh_scene_build(h, C.byref(bo), C.byref(out)))
+            fn = lib.pfh_scene_build_d3d9 if level == RendererLevel.D3D9 else lib.pfh_scene_build
+            L.check(fn(h, C.byref(bo), C.byref(out)))
             return CommandList(out)
         finally:
             lib.pfh_scene_destroy(h)
 
     def build_and_render(self, renderer, options=None):
-        """Scene::build_and_render (scene.rs:369-378)."""
-        commands = self.build(options)
+        """Scene::build_and_render (scene.rs:369-378) at the renderer's level."""
+        commands = self.build(options, renderer.mode.level)
         renderer.begin_scene()
         renderer.render_commands(commands)
         renderer.end_scene()
@@ -668,6 +671,33 @@ class CommandList:
                 out.append((pid.value, r.min_x, r.min_y,
                             _copy(ptr.value, w * h * 4, np.uint8).reshape(h, w, 4)))
         return out
+
+
+    def fills_d3d9(self):
+        """Every ADD_FILLS_D3D9 fill in command order: (N, 3) words (from x |
+        from y << 16, to x | to y << 16, alpha tile)."""
+        out = []
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_ADD_FILLS_D3D9:
+                ptr, n = C.c_void_p(), C.c_uint32()
+                L.check(self._lib.pfh_command_fills_d3d9(self._h, i, C.byref(ptr), C.byref(n)))
+                out.append(_copy(ptr.value, n.value * 12, np.uint32).reshape(-1, 3))
+        return np.concatenate(out) if out else np.zeros((0, 3), np.uint32)
+
+    def batch_d3d9(self):
+        """The DRAW_TILES_D3D9 batch: tiles (N, 4) words, clips (N, 4),
+        z-buffer and its rect (min x, min y, max x, max y), or None."""
+        for i, k in enumerate(self.kinds()):
+            if k == L.COMMAND_DRAW_TILES_D3D9:
+                b = L.DrawTileBatchD3D9()
+                L.check(self._lib.pfh_command_batch_d3d9(self._h, i, C.byref(b)))
+                r = b.z_buffer_rect
+                zw, zh = max(r.max_x - r.min_x, 0), max(r.max_y - r.min_y, 0)
+                return {"tiles": _copy(b.tiles, b.tile_count * 16, np.uint32).reshape(-1, 4),
+                        "clips": _copy(b.clips, b.clip_count * 16, np.int32).reshape(-1, 4),
+                        "z": _copy(b.z_buffer, zw * zh * 4, np.int32),
+                        "z_rect": (r.min_x, r.min_y, r.max_x, r.max_y)}
+        return None
 
 
 def _copy(addr, nbytes, dtype):
