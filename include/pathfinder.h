@@ -138,8 +138,10 @@ void PFHIPDeviceDestroy(PFHIPDeviceRef device);
 PFHIPDestFramebufferRef PFHIPDestFramebufferCreateFullWindow(const PFVector2I *window_size);
 void PFHIPDestFramebufferDestroy(PFHIPDestFramebufferRef dest_framebuffer);
 /* Takes ownership of device and options->dest, not of resources
- * (c/src/lib.rs:601-615). NULL if the device cannot be used or the level is
- * not PF_RENDERER_LEVEL_D3D11. */
+ * (c/src/lib.rs:601-615). Either level: PF_RENDERER_LEVEL_D3D11 bins on the
+ * GPU; PF_RENDERER_LEVEL_D3D9 tiles on the CPU when the scene is built and
+ * rasterizes masks on the GPU (solid paints and SrcOver only). NULL if the
+ * device cannot be used. */
 PFHIPRendererRef PFHIPRendererCreate(PFHIPDeviceRef device, PFResourceLoaderRef resources,
                                      const PFRendererMode *mode,
                                      const PFRendererOptions *options);
@@ -165,8 +167,9 @@ void PFBuildOptionsSetSubpixelAAEnabled(PFBuildOptionsRef options, int subpixel_
 
 /* ------------------------------------------------------------ scenes */
 void PFSceneDestroy(PFSceneRef scene);
-/* Consumes the scene (c/src/lib.rs:790-796); the build runs on the calling
- * thread at render time (there is no executor). */
+/* Consumes the scene (c/src/lib.rs:790-796); the build runs at render time
+ * (the D3D9 level's tiler on the calling thread and worker threads). The
+ * level must be the renderer's. */
 PFSceneProxyRef PFSceneProxyCreateFromSceneAndRayonExecutor(PFSceneRef scene,
                                                             PFRendererLevel level);
 void PFSceneProxyDestroy(PFSceneProxyRef scene_proxy);

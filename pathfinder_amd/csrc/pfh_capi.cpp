@@ -72,6 +72,7 @@ struct PFHIPDestFramebuffer {
 };
 
 struct PFHIPRenderer {
+    PFRendererLevel level = PF_RENDERER_LEVEL_D3D11;
     PFHIPDevice *device = nullptr;
     PFHIPDestFramebuffer *dest = nullptr;
     pfh_context *ctx = nullptr;
@@ -512,7 +513,8 @@ PFHIPRendererRef PFHIPRendererCreate(PFHIPDeviceRef device, PFResourceLoaderRef 
     PFHIPDestFramebuffer *dest =
         options ? static_cast<PFHIPDestFramebuffer *>(options->dest) : nullptr;
     if (!device || !resources || !mode || !options || !dest ||
-        mode->level != PF_RENDERER_LEVEL_D3D11 || resources->area_lut.size() != 256 * 256 * 4) {
+        (mode->level != PF_RENDERER_LEVEL_D3D11 && mode->level != PF_RENDERER_LEVEL_D3D9) ||
+        resources->area_lut.size() != 256 * 256 * 4) {
         delete device;
         delete dest;
         return nullptr;
@@ -523,12 +525,14 @@ PFHIPRendererRef PFHIPRendererCreate(PFHIPDeviceRef device, PFResourceLoaderRef 
         delete dest;
         return nullptr;
     }
+    r->level = mode->level;
     r->device = device;
     r->dest = dest;
     pfh_context_options co;
     std::memset(&co, 0, sizeof(co));
     co.device = device->ordinal;
-    co.level = PFH_RENDERER_LEVEL_D3D11;
+    co.level = mode->level == PF_RENDERER_LEVEL_D3D9 ? PFH_RENDERER_LEVEL_D3D9
+                                                     : PFH_RENDERER_LEVEL_D3D11;
     co.area_lut_rgba = resources->area_lut.data();
     if (pfh_context_create(&co, &r->ctx) != PFH_OK) {
         PFHIPRendererDestroy(r);
@@ -628,17 +632,23 @@ void PFSceneProxyBuildAndRenderHIP(PFSceneProxyRef proxy, PFHIPRendererRef rende
     PFBuildOptions defaults;
     const PFBuildOptions &o = options ? *options : defaults;
     if (o.dilation[0] != 0 || o.dilation[1] != 0 || o.subpixel_aa) {
-        renderer->error = "dilation and subpixel AA are not on the D3D11-level path";
+        renderer->error = "dilation and subpixel AA are not supported";
         return;
     }
-    if (proxy->level != PF_RENDERER_LEVEL_D3D11) {
-        renderer->error = "the scene proxy's level is not PF_RENDERER_LEVEL_D3D11";
+    if (proxy->level != renderer->level) {
+        renderer->error = "the scene proxy's level is not the renderer's";
         return;
     }
     pfh_build_options bo;
     bo.transform = o.t;
-    pfh_status s = pfh_scene_build_and_render(proxy->scene, renderer->ctx, &renderer->frame, &bo);
-    renderer->error = s == PFH_OK ? "" : pfh_last_error(renderer->ctx);
+    pfh_status s = proxy->level == PF_RENDERER_LEVEL_D3D9
+                       ? pfh_scene_build_and_render_d3d9(proxy->scene, renderer->ctx,
+                                                         &renderer->frame, &bo)
+                       : pfh_scene_build_and_render(proxy->scene, renderer->ctx,
+                                                    &renderer->frame, &bo);
+    if (s == PFH_OK) renderer->error.clear();
+    else if (*pfh_last_error(renderer->ctx)) renderer->error = pfh_last_error(renderer->ctx);
+    else renderer->error = "the scene has a paint or blend mode this level does not draw";
 }
 
 // ------------------------------------------------------------ SVG

@@ -123,18 +123,19 @@ def _canvas_scene(lib):
     return lib.PFCanvasCreateScene(canvas)
 
 
-def _render(lib, scene_h, size, transform=None, loader=None):
-    """PFHIPRendererCreate + PFSceneProxyBuildAndRenderHIP; consumes scene_h."""
+def _render(lib, scene_h, size, transform=None, loader=None, level=2):
+    """PFHIPRendererCreate + PFSceneProxyBuildAndRenderHIP at `level`
+    (PF_RENDERER_LEVEL_D3D11 2, D3D9 1); consumes scene_h."""
     device = lib.PFHIPDeviceCreate(0)
     own_loader = loader is None
     loader = loader or lib.PFEmbeddedResourceLoaderCreate()
     dest = lib.PFHIPDestFramebufferCreateFullWindow(C.byref(V2I(*size)))
     opts = Options(dest, ColorFC(1, 1, 1, 1), 1)
-    r = lib.PFHIPRendererCreate(device, loader, C.byref(Mode(2)), C.byref(opts))
+    r = lib.PFHIPRendererCreate(device, loader, C.byref(Mode(level)), C.byref(opts))
     if own_loader:
         lib.PFResourceLoaderDestroy(loader)
     assert r, "PFHIPRendererCreate failed"
-    proxy = lib.PFSceneProxyCreateFromSceneAndRayonExecutor(scene_h, 2)
+    proxy = lib.PFSceneProxyCreateFromSceneAndRayonExecutor(scene_h, level)
     build = lib.PFBuildOptionsCreate()
     if transform:
         t = lib.PFRenderTransformCreate2D(C.byref(Xf(transform[:4], V2(*transform[4:]))))
@@ -207,3 +208,23 @@ def test_filesystem_resource_loader_draws_like_the_embedded_lut():
         lib.PFResourceLoaderDestroy(loader)
     b = _render(lib, _canvas_scene(lib), (160, 120))
     assert np.array_equal(a, b)
+
+
+def test_d3d9_level_through_the_c_api_bit_exact():
+    """PF_RENDERER_LEVEL_D3D9: the canvas and the tiger, tiled on the CPU and
+    rasterized on the GPU, against the D3D9 oracle."""
+    from oracle import d3d9 as od3d9
+    from pathfinder_amd import RectF
+    lib = _lib()
+    _say("case capi d3d9 canvas")
+    h = _canvas_scene(lib)
+    want = od3d9.render(scene_from_native(h), (160, 120), ColorF(1, 1, 1, 1))
+    got = _render(lib, h, (160, 120), level=1)
+    assert np.array_equal(got, want), int(np.count_nonzero(got != want))
+    _say("case capi d3d9 tiger")
+    h = lib.PFSVGSceneIntoScene(lib.PFSVGSceneCreateWithPath(TIGER.encode()))
+    lib.pfh_scene_set_view_box(h, C.byref(L.RectF(L.Vector2F(0, 0), L.Vector2F(600, 600))))
+    t = (3, 0, 0, 3, 0, 0)
+    want = od3d9.render(scene_from_native(h), (600, 600), ColorF(1, 1, 1, 1), transform=t)
+    got = _render(lib, h, (600, 600), transform=t, level=1)
+    assert np.array_equal(got, want), int(np.count_nonzero(got != want))
