@@ -24,7 +24,9 @@ GPU: forward = one frame with the coverage checkpoint (re-rendered from
 device memory), backward = pfh_backward (composite and coverage adjoint),
 ratio = (forward + backward) / forward, all device time with the scene
 resident. step_ms is one whole optimisation step through torch.autograd
-(host packing, uploads, forward, loss, backward).
+(the points copied to the host, the C++ repack of a scene kept alive with its
+points and colours replaced in place, uploads, forward, loss, backward,
+Adam).
 
 batch_glyphs / batch_canvases: many items of uneven sizes packed into frames
 (pathfinder_amd.batch): 50,000 glyphs at 8-72 px on 4096 x 4096 atlas pages,
@@ -216,6 +218,7 @@ def diff_leg(steps, warmup):
             "fwd_ms": round(fwd_ms, 4), "bwd_ms": round(bwd_ms, 4),
             "fwd_bwd_ratio": round((fwd_ms + bwd_ms) / fwd_ms, 3),
             "step_ms": round(step_ms, 3),
+            "step_over_fwd_bwd": round(step_ms / (fwd_ms + bwd_ms), 3),
             "optimizer": "adam, lr 0.2 px (points) and 0.005 (colours), mean squared error",
             "loss_first_last": [round(losses[0], 6), round(losses[-1], 6)],
             "loss_curve": [round(v, 6) for v in losses],

@@ -577,6 +577,17 @@ uint32_t pfh_scene_push_gradient_paint(pfh_scene *scene, const pfh_color_u *base
 uint32_t pfh_scene_push_pattern_paint(pfh_scene *scene, const pfh_color_u *base_color,
                                       const pfh_pattern *pattern, int32_t composite_op);
 
+/* In-place updates for callers that re-render one structure with moving
+ * points and changing colours (inverse rendering); no counterpart in the
+ * reference. set_draw_points replaces the coordinates of every draw path's
+ * points (count must equal the points pushed) and recomputes the bounds;
+ * set_paint_colors replaces the paint table with these solid colours, paint
+ * i being colours[i] even where two are equal. */
+pfh_status pfh_scene_set_draw_points(pfh_scene *scene, const pfh_vector2f *points,
+                                     uint32_t count);
+pfh_status pfh_scene_set_paint_colors(pfh_scene *scene, const pfh_color_u *colors,
+                                      uint32_t count);
+
 /* Scene::push_clip_path (scene.rs:98). clip_path = -1 for none. Returns ClipPathId. */
 uint32_t pfh_scene_push_clip_path(pfh_scene *scene, const pfh_outline *outline, int32_t fill_rule,
                                   int32_t clip_path);

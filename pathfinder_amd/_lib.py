@@ -260,6 +260,8 @@ SIGNATURES = {
     "pfh_scene_destroy": (None, [P]),
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),
     "pfh_scene_push_paint": (U32, [P, C.POINTER(ColorU)]),
+    "pfh_scene_set_draw_points": (I32, [P, P, U32]),
+    "pfh_scene_set_paint_colors": (I32, [P, P, U32]),
     "pfh_scene_push_gradient_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Gradient), I32]),
     "pfh_scene_push_pattern_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Pattern), I32]),
     "pfh_scene_push_clip_path": (U32, [P, C.POINTER(Outline), I32, I32]),

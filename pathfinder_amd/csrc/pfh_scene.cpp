@@ -379,6 +379,32 @@ uint32_t pfh_scene_push_paint(pfh_scene *scene, const pfh_color_u *color) {
     return push_paint_desc(scene, std::move(d));
 }
 
+pfh_status pfh_scene_set_draw_points(pfh_scene *scene, const pfh_vector2f *points,
+                                     uint32_t count) {
+    if (!scene || (count && !points)) return PFH_ERROR_INVALID_ARGUMENT;
+    PathSet &set = scene->draw;
+    if (count != set.points.size()) return PFH_ERROR_INVALID_ARGUMENT;
+    std::copy(points, points + count, set.points.begin());
+    for (PathRecord &r : set.paths)
+        r.bounds = outline_bounds(set.points.data(), set.contour_ends.data(), r.point_begin,
+                                  r.contour_begin, r.contour_end);
+    return PFH_OK;
+}
+
+pfh_status pfh_scene_set_paint_colors(pfh_scene *scene, const pfh_color_u *colors,
+                                      uint32_t count) {
+    if (!scene || (count && !colors)) return PFH_ERROR_INVALID_ARGUMENT;
+    if (count > 65536) return PFH_ERROR_LIMIT;
+    scene->paints.assign(colors, colors + count);
+    scene->paint_descs.assign(count, pfh::PaintDesc());
+    scene->paint_cache.clear();
+    for (uint32_t i = 0; i < count; i++) {
+        scene->paint_descs[i].base = colors[i];
+        scene->paint_cache.emplace(pfh::paint_key(scene->paint_descs[i]), i);  // first id wins
+    }
+    return PFH_OK;
+}
+
 uint32_t pfh_scene_push_gradient_paint(pfh_scene *scene, const pfh_color_u *base_color,
                                        const pfh_gradient *gradient, int32_t composite_op) {
     pfh::PaintDesc d;

@@ -12,10 +12,13 @@ clamp(|C|, 0, 1) and the src-over composite; the 8.8 fixed point, unorm8
 masks and pixels and the paints' u8/f16 rounding pass gradients through
 unchanged. This has no counterpart in the reference, which renders only.
 """
+import ctypes as C
+
 import numpy as np
 import torch
 
-from .scene import (ColorF, ColorU, DestFramebuffer, PathArrays, RectF, Renderer,
+from . import _lib as L
+from .scene import (ColorF, ColorU, CommandList, DestFramebuffer, PathArrays, RectF, Renderer,
                     RendererOptions, Scene)
 
 
@@ -66,8 +69,12 @@ class DiffRenderer:
         self.renderer.set_gradients(True)
         self.packed = torch.as_tensor(shapes.packed, device=self.device)
         self.frame = 0
+        self._scene = None  # native scene of the structure, updated in place
 
     def close(self):
+        if self._scene:
+            L.load().pfh_scene_destroy(self._scene)
+            self._scene = None
         self.renderer.close()
 
     def scene(self, points, colors):
@@ -86,14 +93,45 @@ class DiffRenderer:
                                      self.shapes.fill_rules))
         return s
 
+    def commands(self, points, colors):
+        """The command list of these leaves (host arrays): the structure is
+        pushed into a native scene once, and later frames replace its points
+        and paint colours in place (paint i is path i's colour) before the
+        C++ build."""
+        lib = L.load()
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 2)
+        col8 = np.ascontiguousarray(colors_to_u8(colors).reshape(-1, 4))
+        if len(pts) != self.shapes.point_count or len(col8) != self.shapes.path_count:
+            raise ValueError("%d points, %d colours for shapes of %d points, %d paths" %
+                             (len(pts), len(col8), self.shapes.point_count,
+                              self.shapes.path_count))
+        first = self._scene is None
+        if first:
+            s = Scene()
+            s.set_view_box(RectF(0, 0, self.width, self.height))
+            s.push_draw_paths(PathArrays(pts, self.shapes.flags, self.shapes.contour_ends,
+                                         self.shapes.path_contour_ends,
+                                         np.arange(self.shapes.path_count, dtype=np.uint32),
+                                         self.shapes.fill_rules))
+            self._scene = s._native()
+        L.check(lib.pfh_scene_set_draw_points(self._scene, pts.ctypes.data_as(C.c_void_p),
+                                              len(pts)))
+        L.check(lib.pfh_scene_set_paint_colors(self._scene, col8.ctypes.data_as(C.c_void_p),
+                                               len(col8)))
+        bo = L.BuildOptions(L.Transform2F(1, 0, 0, 1, 0, 0))
+        out = C.c_void_p()
+        L.check(lib.pfh_scene_build(self._scene, C.byref(bo), C.byref(out)))
+        cmds = CommandList(out)
+        if first:
+            draw, _ = cmds.segments()
+            if len(draw["points"]) != len(self.shapes.packed):
+                raise RuntimeError("packed points do not follow the contour layout")
+        return cmds
+
     def render(self, points, colors):
         """Renders and returns the image (H, W, 4) float32 on the device."""
-        scene = self.scene(points.detach().to("cpu", torch.float32).numpy(),
-                           colors.detach().to("cpu", torch.float32).numpy())
-        cmds = scene.build()
-        draw, _ = cmds.segments()
-        if len(draw["points"]) != len(self.shapes.packed):
-            raise RuntimeError("packed points do not follow the contour layout")
+        cmds = self.commands(points.detach().to("cpu", torch.float32).numpy(),
+                             colors.detach().to("cpu", torch.float32).numpy())
         r = self.renderer
         r.begin_scene()
         r.render_commands(cmds)

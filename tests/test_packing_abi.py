@@ -197,3 +197,51 @@ def test_packing_gradient_pages():
     cmds = _compare(scene)
     assert cmds["pages"] == {0: (256, 256), 1: (256, 256)}
     assert [b["color_texture_page"] for _, b in cmds["batches"]] == [0, 1]
+
+
+def test_in_place_point_and_paint_updates():
+    """pfh_scene_set_draw_points / pfh_scene_set_paint_colors: a scene built
+    once and updated in place packs as a scene built from scratch with the
+    new points and one paint per path."""
+    import ctypes
+    from pathfinder_amd import diff
+    from pathfinder_amd.scene import ColorU, CommandList, Paint, PathArrays, RectF, Scene
+    d = synth.diff_paths(paths=40, size=128)
+    sh = diff.Shapes(d["flags"], d["contour_ends"], d["path_contour_ends"])
+    lib = L.load()
+    pts0 = np.ascontiguousarray(d["points"], np.float32)
+    pts1 = np.ascontiguousarray(pts0 * np.float32(0.9) + np.float32(7.3))
+    cols = np.ascontiguousarray(np.tile([[10, 20, 30, 255]], (sh.path_count, 1)), np.uint8)
+
+    def fresh(points):
+        s = Scene()
+        s.set_view_box(RectF(0, 0, 128, 128))
+        for c in cols:  # one paint per path, equal colours kept apart
+            s.paints.append(ColorU(*c))
+            s.paint_objects.append(Paint(ColorU(*c)))
+        s.push_draw_paths(PathArrays(points, sh.flags, sh.contour_ends, sh.path_contour_ends,
+                                     np.arange(sh.path_count, dtype=np.uint32)))
+        return packing.build(s)
+
+    s = Scene()
+    s.set_view_box(RectF(0, 0, 128, 128))
+    s.push_draw_paths(PathArrays(pts0, sh.flags, sh.contour_ends, sh.path_contour_ends,
+                                       np.arange(sh.path_count, dtype=np.uint32)))
+    h = s._native()
+    try:
+        L.check(lib.pfh_scene_set_draw_points(h, pts1.ctypes.data_as(ctypes.c_void_p), len(pts1)))
+        L.check(lib.pfh_scene_set_paint_colors(h, cols.ctypes.data_as(ctypes.c_void_p),
+                                               len(cols)))
+        assert lib.pfh_scene_set_draw_points(h, pts1.ctypes.data_as(ctypes.c_void_p), 3) != 0
+        out = ctypes.c_void_p()
+        bo = L.BuildOptions(L.Transform2F(1, 0, 0, 1, 0, 0))
+        L.check(lib.pfh_scene_build(h, ctypes.byref(bo), ctypes.byref(out)))
+        cl = CommandList(out)
+    finally:
+        lib.pfh_scene_destroy(h)
+    want = fresh(pts1)
+    assert np.array_equal(cl.paints(), want["paints"])
+    assert np.array_equal(cl.segments()[0]["points"], want["draw"]["points"])
+    for (_, gb), (_, wb) in zip(cl.batches(), want["batches"]):
+        for key in ("dice_metadata", "tile_path_info", "propagate_metadata", "backdrops"):
+            assert np.array_equal(gb[key], np.asarray(wb[key]).astype(gb[key].dtype)), key
