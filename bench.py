@@ -211,6 +211,38 @@ def diff_leg(steps, warmup):
                    shapes.path_count)
     bwd_ms = (time.perf_counter() - t0) / steps * 1e3
     st = r.stats()
+    # what the forward keeps for the backward, and its cost: the float
+    # checkpoint of C, 2-bit states written by the forward fill (the
+    # library's default, timed above), or nothing with the fill run again in
+    # the backward
+    from pathfinder_amd import _lib as L
+    modes = {}
+    for name, mode, per_tile in (("coverage", L.CHECKPOINT_COVERAGE, 1024),
+                                 ("states", L.CHECKPOINT_STATES, 64),
+                                 ("recompute", L.CHECKPOINT_RECOMPUTE, 64)):
+        r.set_gradient_checkpoint(mode)
+        diff.render(dr, p, c)
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            r.rerender()
+        r.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.rerender()
+        r.synchronize()
+        f_ms = (time.perf_counter() - t0) / steps * 1e3
+        for _ in range(warmup):
+            r.backward(g.data_ptr(), gp.data_ptr(), len(shapes.packed), gc.data_ptr(),
+                       shapes.path_count)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.backward(g.data_ptr(), gp.data_ptr(), len(shapes.packed), gc.data_ptr(),
+                       shapes.path_count)
+        b_ms = (time.perf_counter() - t0) / steps * 1e3
+        modes[name] = {"fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4),
+                       "fwd_bwd_over_fwd": round((f_ms + b_ms) / f_ms, 3),
+                       "kept_bytes": st["alpha_tile_count"] * per_tile if mode != 2 else 0,
+                       "backward_scratch_bytes": st["alpha_tile_count"] * 64 if mode == 2 else 0}
     dr.close()
     return {"workload": "diff-1024-cubic-paths-512", "width": w, "height": h,
             "paths": shapes.path_count, "points": shapes.point_count,
@@ -222,7 +254,8 @@ def diff_leg(steps, warmup):
             "optimizer": "adam, lr 0.2 px (points) and 0.005 (colours), mean squared error",
             "loss_first_last": [round(losses[0], 6), round(losses[-1], 6)],
             "loss_curve": [round(v, 6) for v in losses],
-            "loss_fell": bool(losses[-1] < losses[0])}
+            "loss_fell": bool(losses[-1] < losses[0]),
+            "checkpoint_modes": modes}
 
 
 def batch_leg(kind, steps, warmup, sample=32):

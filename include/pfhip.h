@@ -307,9 +307,13 @@ typedef enum pfh_debug_kind {
     PFH_DEBUG_ALPHA_TILES = 5,    /* pfh_alpha_tile[batch alpha tile count] */
     PFH_DEBUG_BACKDROPS = 6,      /* pfh_backdrop_info[backdrop count], after bin */
     PFH_DEBUG_ALPHA_RANGE = 7,    /* uint32[2]: first and end alpha tile id */
-    PFH_DEBUG_COVERAGE = 8        /* float[256 per alpha tile of the batch]: the coverage
+    PFH_DEBUG_COVERAGE = 8,       /* float[256 per alpha tile of the batch]: the coverage
                                      checkpoint (pfh_set_gradients), pixel (x, y) of a
                                      tile at [(y & 3) * 64 + (y >> 2) * 16 + x] */
+    PFH_DEBUG_COVERAGE_STATES = 9 /* uint8[64 per alpha tile of the batch]: the adjoint's
+                                     states (PFH_CHECKPOINT_STATES / _RECOMPUTE), byte
+                                     (y >> 2) * 16 + x, bits 2 (y & 3) (gate) and
+                                     2 (y & 3) + 1 (C < 0) */
 } pfh_debug_kind;
 pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind kind, void *dst,
                           size_t capacity, size_t *out_len);
@@ -371,6 +375,24 @@ pfh_status pfh_profile_read(pfh_context *ctx, pfh_kernel_time *out, uint32_t cap
  * pfh_set_gradients: before a frame, makes the fill kernel checkpoint C for
  * every alpha-tile pixel (256 floats per alpha tile, sized with the mask). */
 pfh_status pfh_set_gradients(pfh_context *ctx, int32_t enable);
+
+/* What the forward keeps for pfh_backward (no counterpart in the reference),
+ * set outside a scene:
+ *   PFH_CHECKPOINT_COVERAGE  the signed coverage sum C of every alpha-tile
+ *                            pixel, 1 KiB per alpha tile (the debug read of
+ *                            C needs it);
+ *   PFH_CHECKPOINT_STATES    what the adjoint uses of C, 2 bits per pixel
+ *                            (0 < |C| < 1 with no limiting clip; C < 0),
+ *                            64 B per alpha tile, written by the same fill
+ *                            (the default: smallest and fastest);
+ *   PFH_CHECKPOINT_RECOMPUTE nothing: pfh_backward runs the draw batch's fill
+ *                            again to write those states (64 B per alpha
+ *                            tile of scratch) before the adjoint.
+ * The three give the same gradients bit for bit. */
+#define PFH_CHECKPOINT_COVERAGE 0
+#define PFH_CHECKPOINT_STATES 1
+#define PFH_CHECKPOINT_RECOMPUTE 2
+pfh_status pfh_set_gradient_checkpoint(pfh_context *ctx, int32_t mode);
 
 /* The last frame's image as floats (byte / 255), width x height x 4, rows
  * top down, into device memory `dst`. */

@@ -54,6 +54,7 @@ DEBUG_ALPHA_TILES = 5
 DEBUG_BACKDROPS = 6
 DEBUG_ALPHA_RANGE = 7
 DEBUG_COVERAGE = 8
+DEBUG_COVERAGE_STATES = 9
 
 SAMPLING_REPEAT_U, SAMPLING_REPEAT_V, SAMPLING_NEAREST_MIN, SAMPLING_NEAREST_MAG = 1, 2, 4, 8
 COMBINE_NONE, COMBINE_SRC_IN, COMBINE_DEST_IN = 0, 1, 2
@@ -63,6 +64,7 @@ GRADIENT_WRAP_CLAMP, GRADIENT_WRAP_REPEAT = 0, 1
 PATTERN_REPEAT_X, PATTERN_REPEAT_Y, PATTERN_NO_SMOOTHING = 1, 2, 4
 PAINT_COMPOSITE_SRC_IN, PAINT_COMPOSITE_DEST_IN = 0, 1
 INVALID_PAINT = 0xFFFFFFFF
+CHECKPOINT_COVERAGE, CHECKPOINT_STATES, CHECKPOINT_RECOMPUTE = 0, 1, 2
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # PFH_NOALLOC_SCRATCH_BYTES
 
@@ -261,6 +263,7 @@ SIGNATURES = {
     "pfh_scene_set_view_box": (None, [P, C.POINTER(RectF)]),
     "pfh_scene_push_paint": (U32, [P, C.POINTER(ColorU)]),
     "pfh_scene_set_draw_points": (I32, [P, P, U32]),
+    "pfh_set_gradient_checkpoint": (I32, [P, I32]),
     "pfh_scene_set_paint_colors": (I32, [P, P, U32]),
     "pfh_scene_push_gradient_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Gradient), I32]),
     "pfh_scene_push_pattern_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Pattern), I32]),

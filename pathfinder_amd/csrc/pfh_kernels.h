@@ -103,6 +103,10 @@ struct BatchArgs {
     float *coverage;               // optional: 256 floats per alpha tile, the signed
                                    // coverage sum before abs/clamp (adjoint checkpoint)
     uint32_t *tile_paths;          // optional: the batch path of every tile (adjoint)
+    uint8_t *cov_state;            // optional: per pixel 2 bits (the adjoint's gate
+                                   // 0 < |C| < 1 and no limiting clip; C < 0), one byte
+                                   // per lane, 64 B per alpha tile
+    int32_t states_only;           // recompute pass: write cov_state only (no mask)
     int32_t fb_width_tiles, fb_height_tiles;
     // launch shape (pfh_launch_shape)
     uint32_t grid_cap;             // workgroups of the grid-stride kernels, at most
@@ -134,7 +138,8 @@ struct TileArgs {
 // colour: per framebuffer tile, the layers of the sorted list.
 struct TileGradArgs {
     TileArgs f;                    // the forward composite's inputs
-    const float *coverage;         // fill checkpoint, 256 per alpha tile
+    const float *coverage;         // fill checkpoint, 256 per alpha tile, or
+    const uint8_t *cov_state;      // the fill's 2-bit states (BatchArgs::cov_state)
     const uint2 *alpha_tiles;      // the batch's alpha tiles (tile, clip alpha id)
     uint32_t alpha_start;          // frame-global id of the batch's first alpha tile
     uint32_t alpha_count;

@@ -1926,6 +1926,24 @@ __device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_in
                                            int32_t backdrop, int32_t clip, uint32_t lane,
                                            float4 acc) {
     float bd = (float)backdrop;
+    if (a.cov_state) {
+        // what the adjoint reads of C (k_tile_backward): the gate 0 < |C| < 1
+        // with the clip texel not below min(|C|, 1), and the sign
+        uint32_t ct = 0;
+        if (clip >= 0 && (uint32_t)clip < a.mask_capacity)
+            ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+        const float C[4] = {bd + acc.x, bd + acc.y, bd + acc.z, bd + acc.w};
+        uint32_t st = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float aC = fabsf(C[k]);
+            bool limited = clip >= 0 && byte_to_unit((ct >> (8 * k)) & 0xffu) < min(aC, 1.0f);
+            bool open = aC > 0.0f && aC < 1.0f && !limited;
+            st |= ((open ? 1u : 0u) | (C[k] < 0.0f ? 2u : 0u)) << (2 * k);
+        }
+        a.cov_state[(size_t)alpha_index * 64 + lane] = (uint8_t)st;
+        if (a.states_only) return;
+    }
     float c0 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.x), 0.0f, 1.0f);
     float c1 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.y), 0.0f, 1.0f);
     float c2 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.z), 0.0f, 1.0f);
@@ -2756,6 +2774,8 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
         uint32_t ctexel = 0;
         if (clip >= 0 && (uint32_t)clip < a.mask_capacity)
             ctexel = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+        uint32_t cstate = (alpha_tile && g.cov_state)
+                              ? g.cov_state[(size_t)(uint32_t)alpha * 64 + lane] : 0u;
         float gr = 0.0f, gg = 0.0f, gb = 0.0f, ga = 0.0f;
 #pragma unroll
         for (int sub = 0; sub < 4; sub++) {
@@ -2769,16 +2789,21 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
             gb += q.z * Tk * al;
             ga += dal * ma[sub];
             if (alpha_tile) {
-                float C = g.coverage[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane];
-                float aC = fabsf(C);
-                bool limited = false;
-                if (clip >= 0) {
-                    float cv = (float)((ctexel >> (8 * sub)) & 0xffu) / 255.0f;
-                    limited = cv < min(aC, 1.0f);
-                }
                 float dC = 0.0f;
-                if (aC > 0.0f && aC < 1.0f && !limited)
-                    dC = dal * base.w * (C < 0.0f ? -1.0f : 1.0f);
+                if (g.cov_state) {
+                    uint32_t st = cstate >> (2 * sub);
+                    if (st & 1u) dC = dal * base.w * ((st & 2u) ? -1.0f : 1.0f);
+                } else {
+                    float C = g.coverage[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane];
+                    float aC = fabsf(C);
+                    bool limited = false;
+                    if (clip >= 0) {
+                        float cv = (float)((ctexel >> (8 * sub)) & 0xffu) / 255.0f;
+                        limited = cv < min(aC, 1.0f);
+                    }
+                    if (aC > 0.0f && aC < 1.0f && !limited)
+                        dC = dal * base.w * (C < 0.0f ? -1.0f : 1.0f);
+                }
                 g.grad_cov[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane] = dC;
             }
             float ia = 1.0f - al;

@@ -354,6 +354,8 @@ struct pfh_context {
     // adjoint: the fill's coverage checkpoint (256 floats per alpha tile,
     // sized with the mask) and the backward pass's scratch
     bool gradients = false;
+    int32_t checkpoint = PFH_CHECKPOINT_STATES;    // what the forward keeps for the adjoint
+    uint8_t *d_cov_state = nullptr;                // 64 B per alpha tile (states, recompute)
     // launch shape of the grid-stride kernels and of k_fill (pfh_autotune)
     pfh_launch_shape shape = {16384, 8192};
     // pfh_rerender_scene through a captured graph of the frame
@@ -473,8 +475,10 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
 }
 
 pfh_status ensure_coverage(pfh_context *ctx) {
-    if (!ctx->gradients || ctx->d_coverage) return PFH_OK;
-    return dalloc(ctx, &ctx->d_coverage, (size_t)ctx->cap_a * 256);
+    if (!ctx->gradients) return PFH_OK;
+    if (ctx->checkpoint == PFH_CHECKPOINT_COVERAGE)
+        return ctx->d_coverage ? PFH_OK : dalloc(ctx, &ctx->d_coverage, (size_t)ctx->cap_a * 256);
+    return ctx->d_cov_state ? PFH_OK : dalloc(ctx, &ctx->d_cov_state, (size_t)ctx->cap_a * 64);
 }
 
 pfh_status ensure_mask(pfh_context *ctx, uint32_t tiles) {
@@ -483,6 +487,8 @@ pfh_status ensure_mask(pfh_context *ctx, uint32_t tiles) {
     dfree(ctx, ctx->d_mask);
     dfree(ctx, ctx->d_coverage);
     ctx->d_coverage = nullptr;
+    dfree(ctx, ctx->d_cov_state);
+    ctx->d_cov_state = nullptr;
     ctx->cap_a = cap;
     pfh_status s = dalloc(ctx, &ctx->d_mask, (size_t)cap * 64);
     return s != PFH_OK ? s : ensure_coverage(ctx);
@@ -573,7 +579,11 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.mask = ctx->d_mask;
     a.mask_capacity = ctx->cap_a;
     a.lut = ctx->d_lut;
-    a.coverage = ctx->gradients ? ctx->d_coverage : nullptr;
+    a.coverage = ctx->gradients && ctx->checkpoint == PFH_CHECKPOINT_COVERAGE ? ctx->d_coverage
+                                                                             : nullptr;
+    a.cov_state = ctx->gradients && ctx->checkpoint == PFH_CHECKPOINT_STATES ? ctx->d_cov_state
+                                                                            : nullptr;
+    a.states_only = 0;
     a.tile_paths = ctx->gradients ? b.d_tile_paths : nullptr;
     a.fb_width_tiles = ctx->fbw;
     a.fb_height_tiles = ctx->fbh;
@@ -924,7 +934,9 @@ pfh_status backward_check(pfh_context *ctx, const float *grad_image, const float
     if (!ctx || !grad_image || !grad_points || !grad_colors)
         return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "null argument");
     if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "backward inside a scene");
-    if (!ctx->gradients || !ctx->d_coverage)
+    if (!ctx->gradients ||
+        !(ctx->checkpoint == PFH_CHECKPOINT_COVERAGE ? (void *)ctx->d_coverage
+                                                     : (void *)ctx->d_cov_state))
         return fail(ctx, PFH_ERROR_STATE, "the frame was rendered without pfh_set_gradients");
     Batch *draw = nullptr;
     for (auto &b : ctx->batches) {
@@ -971,9 +983,19 @@ pfh_status backward_launch(pfh_context *ctx, Batch &b, const float *grad_image, 
                            float *grad_colors, uint32_t path_count, const BackwardScratch &sc,
                            uint32_t slots, uint32_t alpha_end) {
     hipStream_t s = ctx->stream;
+    if (ctx->checkpoint == PFH_CHECKPOINT_RECOMPUTE) {
+        // the forward kept nothing: the fill runs again over the draw batch
+        // and writes the adjoint's states only
+        pfh::BatchArgs fa = batch_args(ctx, b);
+        fa.coverage = nullptr;
+        fa.cov_state = ctx->d_cov_state;
+        fa.states_only = 1;
+        pfh::launch_fill(fa, s, nullptr);
+    }
     pfh::TileGradArgs tg;
     tg.f = tile_args(ctx, b);
-    tg.coverage = ctx->d_coverage;
+    tg.coverage = ctx->checkpoint == PFH_CHECKPOINT_COVERAGE ? ctx->d_coverage : nullptr;
+    tg.cov_state = ctx->checkpoint == PFH_CHECKPOINT_COVERAGE ? nullptr : ctx->d_cov_state;
     tg.alpha_tiles = b.d_alpha;
     tg.alpha_start = b.counters.alpha_start;
     tg.alpha_count = b.counters.alpha;
@@ -1674,6 +1696,12 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
         src = ctx->d_coverage + (size_t)b->counters.alpha_start * 256;
         bytes = (size_t)std::min(b->counters.alpha, ctx->cap_a - std::min(b->counters.alpha_start, ctx->cap_a)) * 1024;
         break;
+    case PFH_DEBUG_COVERAGE_STATES:
+        if (!ctx->gradients || !ctx->d_cov_state)
+            return fail(ctx, PFH_ERROR_STATE, "no coverage states (pfh_set_gradient_checkpoint)");
+        src = ctx->d_cov_state + (size_t)b->counters.alpha_start * 64;
+        bytes = (size_t)std::min(b->counters.alpha, ctx->cap_a - std::min(b->counters.alpha_start, ctx->cap_a)) * 64;
+        break;
     default: return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "bad debug kind %d", (int)kind);
     }
     *out_len = bytes;
@@ -1743,6 +1771,26 @@ pfh_status pfh_set_gradients(pfh_context *ctx, int32_t enable) {
     if (!ctx->gradients) return PFH_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     return ensure_mask(ctx, 0);
+}
+
+pfh_status pfh_set_gradient_checkpoint(pfh_context *ctx, int32_t mode) {
+    if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
+    if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "set_gradient_checkpoint inside a scene");
+    if (mode != PFH_CHECKPOINT_COVERAGE && mode != PFH_CHECKPOINT_STATES &&
+        mode != PFH_CHECKPOINT_RECOMPUTE)
+        return fail(ctx, PFH_ERROR_INVALID_ARGUMENT, "unknown checkpoint mode %d", mode);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->checkpoint = mode;
+    if (mode != PFH_CHECKPOINT_COVERAGE) {
+        dfree(ctx, ctx->d_coverage);
+        ctx->d_coverage = nullptr;
+    } else {
+        dfree(ctx, ctx->d_cov_state);
+        ctx->d_cov_state = nullptr;
+    }
+    return ensure_coverage(ctx);
 }
 
 pfh_status pfh_framebuffer_to_float(pfh_context *ctx, float *dst) {

@@ -59,7 +59,8 @@ def colors_to_u8(colors):
 class DiffRenderer:
     """A renderer with gradients enabled for one Shapes and frame size."""
 
-    def __init__(self, shapes, width, height, background=(1.0, 1.0, 1.0, 1.0), device=None):
+    def __init__(self, shapes, width, height, background=(1.0, 1.0, 1.0, 1.0), device=None,
+                 checkpoint=None):
         self.shapes = shapes
         self.width, self.height = int(width), int(height)
         self.background = tuple(float(v) for v in background)
@@ -67,6 +68,8 @@ class DiffRenderer:
         self.renderer = Renderer(RendererOptions(DestFramebuffer.full_window(width, height),
                                                  ColorF(*self.background)))
         self.renderer.set_gradients(True)
+        if checkpoint is not None:
+            self.renderer.set_gradient_checkpoint(checkpoint)
         self.packed = torch.as_tensor(shapes.packed, device=self.device)
         self.frame = 0
         self._scene = None  # native scene of the structure, updated in place

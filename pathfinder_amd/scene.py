@@ -895,6 +895,12 @@ class Renderer:
         frames that follow."""
         self._check(self._lib.pfh_set_gradients(self._h, 1 if enable else 0))
 
+    def set_gradient_checkpoint(self, mode):
+        """What the forward keeps for the backward: L.CHECKPOINT_COVERAGE (C,
+        1 KiB per alpha tile), L.CHECKPOINT_STATES (2 bits per pixel) or
+        L.CHECKPOINT_RECOMPUTE (nothing; the backward runs the fill again)."""
+        self._check(self._lib.pfh_set_gradient_checkpoint(self._h, int(mode)))
+
     def framebuffer_to_float(self, dst_ptr):
         """The last frame as width x height x 4 floats into device memory."""
         self._check(self._lib.pfh_framebuffer_to_float(self._h, C.c_void_p(dst_ptr)))

@@ -95,6 +95,7 @@ def _gpu_layers(dr, scene, W, H, path_count):
 def test_gpu_gradient_matches_cpu_backward(seed, paths, size):
     import torch
 
+    from pathfinder_amd import _lib as L
     from pathfinder_amd import diff
     _say("case diff gpu vs cpu", seed, paths, size)
     rng = np.random.default_rng(seed)
@@ -103,7 +104,7 @@ def test_gpu_gradient_matches_cpu_backward(seed, paths, size):
                              rng.uniform(0.3, 1.0, (paths, 1))], axis=1).astype(np.float32)
     shapes = diff.Shapes(flags, ends, pends)
     bg = (1.0, 1.0, 1.0, 1.0)
-    dr = diff.DiffRenderer(shapes, size, size, bg)
+    dr = diff.DiffRenderer(shapes, size, size, bg, checkpoint=L.CHECKPOINT_COVERAGE)
     try:
         p_t = torch.tensor(pts, device="cuda", requires_grad=True)
         c_t = torch.tensor(colors, device="cuda", requires_grad=True)
@@ -188,3 +189,104 @@ def test_adam_fit_with_colours_lowers_loss_at_512():
         assert sum(b < a for a, b in zip(losses, losses[1:])) >= 8
     finally:
         dr.close()
+
+
+def _expected_states(C, clip_texels, clipped):
+    """The adjoint's 2-bit states from the coverage checkpoint C (per alpha
+    tile [k][lane]) and, for the clipped tiles, their clip texel bytes."""
+    aC = np.abs(C)
+    open_ = (aC > 0) & (aC < 1)
+    if clip_texels is not None:
+        cv = clip_texels.astype(np.float32) / np.float32(255.0)
+        limited = (cv < np.minimum(aC, np.float32(1.0))) & clipped[:, None, None]
+        open_ &= ~limited
+    neg = C < 0
+    bits = open_.astype(np.uint32) | (neg.astype(np.uint32) << 1)      # [tile][k][lane]
+    return (bits[:, 0] | (bits[:, 1] << 2) | (bits[:, 2] << 4) | (bits[:, 3] << 6)).astype(
+        np.uint8)
+
+
+@pytest.mark.parametrize("name", ["clip", "small"])
+def test_coverage_states_equal_the_checkpoint(name, renderer_factory):
+    """The states of PFH_CHECKPOINT_STATES (written by the forward fill) and
+    of PFH_CHECKPOINT_RECOMPUTE (written by the fill the backward reruns)
+    equal the ones computed from the float checkpoint C and the clip texels,
+    on a scene with clip paths and on 256 random paths."""
+    import ctypes as C_
+    import torch
+    from pathfinder_amd import _lib as L
+    from pathfinder_amd import synth
+    from helpers import product_frame
+    scene, size, bg = (synth.tiny_scenes()["clip"] if name == "clip"
+                       else synth.small_scene(paths=256, size=512))
+    got = {}
+    for mode in (L.CHECKPOINT_COVERAGE, L.CHECKPOINT_STATES, L.CHECKPOINT_RECOMPUTE):
+        _say("case states", name, mode)
+        r = renderer_factory(size[0], size[1], bg)
+        r.set_gradients(True)
+        r.set_gradient_checkpoint(mode)
+        cl = product_frame(r, scene)
+        (kind, b), = [kb for kb in cl.batches() if kb[0] == 4]
+        bid = b["batch_id"]
+        if mode == L.CHECKPOINT_COVERAGE:
+            cov = r.read_debug(bid, L.DEBUG_COVERAGE).view(np.float32).reshape(-1, 4, 64)
+            alpha = r.read_debug(bid, L.DEBUG_ALPHA_TILES).view(np.uint32).reshape(-1, 2)
+            mask = r.read_mask().view(np.uint8).reshape(-1, 64, 4)
+            clip = alpha[:, 1].astype(np.int64)
+            texels = None
+            if np.any(clip != 0xFFFFFFFF):
+                texels = np.zeros((len(clip), 4, 64), np.uint8)
+                for i in np.nonzero(clip != 0xFFFFFFFF)[0]:
+                    texels[i] = mask[clip[i]].T
+            want = _expected_states(cov, texels, clip != 0xFFFFFFFF)
+            assert np.any(want & 1) and (name != "clip" or texels is not None)
+            continue
+        if mode == L.CHECKPOINT_RECOMPUTE:
+            g = torch.zeros(size[0] * size[1] * 4, device="cuda")
+            n = len(cl.segments()[0]["points"])
+            gp = torch.zeros(2 * n, device="cuda")
+            gc = torch.zeros(4 * scene.draw_path_count(), device="cuda")
+            torch.cuda.synchronize()
+            r.backward(g.data_ptr(), gp.data_ptr(), n, gc.data_ptr(), scene.draw_path_count())
+        got[mode] = r.read_debug(bid, L.DEBUG_COVERAGE_STATES).view(np.uint8).reshape(-1, 64)
+    for mode, st in got.items():
+        bad = np.argwhere(st != want)
+        for t, lane in bad[:6]:
+            _say("mode", mode, "tile", t, "lane", lane, "got", st[t, lane], "want", want[t, lane],
+                 "C", cov[t, :, lane].tolist(), "clip", int(clip[t]),
+                 "texels", None if texels is None else texels[t, :, lane].tolist())
+        assert np.array_equal(st, want), (mode, int(np.count_nonzero(st != want)))
+
+
+def test_checkpoint_modes_give_the_same_gradients():
+    """PFH_CHECKPOINT_COVERAGE (C per pixel), _STATES (2 bits per pixel from
+    the forward fill) and _RECOMPUTE (the backward reruns the fill) give the
+    same image and the same point and colour gradients (the states they feed
+    the adjoint are compared bit for bit above)."""
+    import torch
+    from pathfinder_amd import _lib as L
+    from pathfinder_amd import diff, synth
+    d = synth.diff_paths(paths=96, size=128)
+    shapes = diff.Shapes(d["flags"], d["contour_ends"], d["path_contour_ends"])
+    out = {}
+    for mode in (L.CHECKPOINT_COVERAGE, L.CHECKPOINT_STATES, L.CHECKPOINT_RECOMPUTE):
+        _say("case checkpoint mode", mode)
+        dr = diff.DiffRenderer(shapes, 128, 128, checkpoint=mode)
+        try:
+            p_t = torch.tensor(d["points"], device="cuda", requires_grad=True)
+            c_t = torch.tensor(d["colors"], device="cuda", requires_grad=True)
+            img = diff.render(dr, p_t, c_t)
+            loss = ((img - 0.5) ** 2).sum()
+            loss.backward()
+            out[mode] = (img.detach().cpu().numpy(), p_t.grad.cpu().numpy(),
+                         c_t.grad.cpu().numpy())
+        finally:
+            dr.close()
+    ref = out[L.CHECKPOINT_COVERAGE]
+    assert np.abs(ref[1]).max() > 0
+    for mode, got in out.items():
+        # the image bit for bit; the gradients up to the order of the float
+        # atomics that sum them (which changes from run to run in any mode)
+        assert np.array_equal(got[0], ref[0]), mode
+        for a, b in zip(got[1:], ref[1:]):
+            assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), mode
