@@ -32,10 +32,13 @@ batch_glyphs / batch_canvases: many items of uneven sizes packed into frames
 (pathfinder_amd.batch): 50,000 glyphs at 8-72 px on 4096 x 4096 atlas pages,
 and 512 canvases of 64^2-1024^2 with 10-2000 paths each on 8192-wide frames
 (a tile budget per frame keeps the 24-bit alpha tile ids). One frame renders
-all of its items with the pipeline's fixed launch sequence; aggregate
+all of its items with the pipeline's fixed launch sequence; each frame is
+autotuned (launch shape, then eager launches or graph replay, whichever is
+faster; `tuned`, with the eager default's times beside). aggregate
 Msegments/s = all microlines / the frames' summed device time (re-rendered,
-resident). per_item_ms: device time of 32 sampled items rendered alone
-(p50 / p99).
+resident). per_scene_ms_in_batch: an item's latency inside the batch, the
+device time of the frame that renders it (p50 / p99 over items);
+per_item_alone_ms: 32 sampled items rendered alone, for comparison.
 """
 import argparse
 import json
@@ -276,28 +279,43 @@ def batch_leg(kind, steps, warmup, sample=32):
     frames = batch.build_frames(items, fw, fh, budget, align=align)
     pack_s = time.perf_counter() - t0
 
-    def timed(scene, w, h):
+    def timed(scene, w, h, tune=False):
+        """Device time of re-rendering the scene; with tune, the launch
+        shape and graph replay come from the autotuner, and the eager
+        default is timed beside it."""
         r = Renderer(RendererOptions(DestFramebuffer.full_window(w, h), bg))
         try:
             cmds = scene.build()
             r.begin_scene()
             r.render_commands(cmds)
             r.end_scene()
-            for _ in range(max(warmup, 1)):
-                r.rerender()
-            r.synchronize()
-            t = time.perf_counter()
-            for _ in range(steps):
-                r.rerender()
-            r.synchronize()
-            return (time.perf_counter() - t) / steps * 1e3, r.stats()["microline_count"]
+
+            def clock():
+                for _ in range(max(warmup, 1)):
+                    r.rerender()
+                r.synchronize()
+                t = time.perf_counter()
+                for _ in range(steps):
+                    r.rerender()
+                r.synchronize()
+                return (time.perf_counter() - t) / steps * 1e3
+
+            eager = clock()
+            choice = None
+            ms = eager
+            if tune:
+                choice = r.autotune(frames=5)
+                ms = clock()
+            return ms, eager, choice, r.stats()["microline_count"]
         finally:
             r.close()
 
-    frame_ms, ml = [], 0
+    frame_ms, eager_ms, choices, ml = [], [], [], 0
     for f in frames:
-        ms, m = timed(f.scene, f.width, f.height)
+        ms, eager, choice, m = timed(f.scene, f.width, f.height, tune=True)
         frame_ms.append(round(ms, 3))
+        eager_ms.append(round(eager, 3))
+        choices.append(choice)
         ml += m
     rng = np.random.default_rng(0)
     per = []
@@ -311,12 +329,20 @@ def batch_leg(kind, steps, warmup, sample=32):
                                      paints, a.fill_rules))
         per.append(timed(s, it.width, it.height)[0])
     total = sum(frame_ms)
+    # an item's latency inside the batch: the device time of the frame that
+    # renders it (every item of a frame completes with its last kernel)
+    lat = np.repeat(frame_ms, [len(f.cells) for f in frames])
     return {"workload": "%s-%d" % (kind, len(items)), "items": len(items),
             "frames": len(frames), "frame_size": [fw, fh], "microlines": ml,
             "frames_ms": round(total, 3), "aggregate_msegments_s": round(ml / total / 1e3, 1),
-            "frame_ms": frame_ms, "pack_s": round(pack_s, 2),
-            "per_item_ms_p50": round(float(np.percentile(per, 50)), 3),
-            "per_item_ms_p99": round(float(np.percentile(per, 99)), 3), "sampled": len(per)}
+            "frame_ms": frame_ms, "frame_ms_eager_default": eager_ms,
+            "aggregate_msegments_s_eager_default": round(ml / sum(eager_ms) / 1e3, 1),
+            "tuned": choices, "pack_s": round(pack_s, 2),
+            "per_scene_ms_in_batch_p50": round(float(np.percentile(lat, 50)), 3),
+            "per_scene_ms_in_batch_p99": round(float(np.percentile(lat, 99)), 3),
+            "per_item_alone_ms_p50": round(float(np.percentile(per, 50)), 3),
+            "per_item_alone_ms_p99": round(float(np.percentile(per, 99)), 3),
+            "sampled_alone": len(per)}
 
 
 def run_diff_leg(steps, warmup):

@@ -288,8 +288,11 @@ pfh_status pfh_get_launch_shape(pfh_context *ctx, pfh_launch_shape *shape);
  * (1024, 2048, 4096, 8192, 16384 workgroups), first for fill_grid, then for
  * grid_cap, timing whole frames with events on the context stream; keeps
  * the fastest shape in the context and returns it in `chosen` (may be
- * NULL). Results do not depend on the shape. */
+ * NULL). It then times eager frames against replays of a captured graph of
+ * the frame at that shape and keeps the faster for pfh_rerender_scene
+ * (pfh_get_graphs reports the choice). Results do not depend on either. */
 pfh_status pfh_autotune(pfh_context *ctx, uint32_t frames, pfh_launch_shape *chosen);
+pfh_status pfh_get_graphs(pfh_context *ctx, int32_t *enabled);
 pfh_status pfh_synchronize(pfh_context *ctx);
 
 /* Reads the destination framebuffer, RGBA8, top row = scene row 0

@@ -264,6 +264,7 @@ SIGNATURES = {
     "pfh_scene_push_paint": (U32, [P, C.POINTER(ColorU)]),
     "pfh_scene_set_draw_points": (I32, [P, P, U32]),
     "pfh_set_gradient_checkpoint": (I32, [P, I32]),
+    "pfh_get_graphs": (I32, [P, C.POINTER(I32)]),
     "pfh_scene_set_paint_colors": (I32, [P, P, U32]),
     "pfh_scene_push_gradient_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Gradient), I32]),
     "pfh_scene_push_pattern_paint": (U32, [P, C.POINTER(ColorU), C.POINTER(Pattern), I32]),

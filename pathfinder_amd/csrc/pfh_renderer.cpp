@@ -1607,6 +1607,28 @@ static pfh_status time_frames(pfh_context *ctx, uint32_t frames, float *ms) {
     return PFH_OK;
 }
 
+// The same through a captured graph of the frame (captured once, then
+// replayed: one warm-up and `frames` timed launches).
+static pfh_status time_graph_frames(pfh_context *ctx, uint32_t frames, float *ms) {
+    drop_graph(ctx);
+    pfh_status s = capture_frame(ctx);
+    if (s != PFH_OK) return s;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = hipGraphLaunch(ctx->graph_exec, ctx->stream);
+    if (err == hipSuccess) err = hipEventRecord(e0, ctx->stream);
+    for (uint32_t i = 0; err == hipSuccess && i < frames; i++)
+        err = hipGraphLaunch(ctx->graph_exec, ctx->stream);
+    if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    if (err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return fail(ctx, PFH_ERROR_HIP, "graph timing: %s", hipGetErrorString(err));
+    return PFH_OK;
+}
+
 pfh_status pfh_autotune(pfh_context *ctx, uint32_t frames, pfh_launch_shape *chosen) {
     if (!ctx) return PFH_ERROR_INVALID_ARGUMENT;
     if (ctx->in_scene) return fail(ctx, PFH_ERROR_STATE, "autotune inside a scene");
@@ -1638,9 +1660,26 @@ pfh_status pfh_autotune(pfh_context *ctx, uint32_t frames, pfh_launch_shape *cho
         (param == 0 ? best.fill_grid : best.grid_cap) = best_v;
     }
     ctx->shape = best;
+    // then eager launches against a replayed graph of the frame with that
+    // shape: the faster one is kept for pfh_rerender_scene
+    float eager_ms = 0.0f, graph_ms = 0.0f;
+    if (s == PFH_OK) s = time_frames(ctx, frames, &eager_ms);
+    if (s == PFH_OK) s = time_graph_frames(ctx, frames, &graph_ms);
+    if (s == PFH_OK) {
+        ctx->use_graphs = graph_ms < eager_ms;
+        if (!ctx->use_graphs) drop_graph(ctx);
+    } else {
+        drop_graph(ctx);
+    }
     ctx->prof.mode = saved_prof;
     if (s != PFH_OK) return s;
     if (chosen) *chosen = best;
+    return PFH_OK;
+}
+
+pfh_status pfh_get_graphs(pfh_context *ctx, int32_t *enabled) {
+    if (!ctx || !enabled) return PFH_ERROR_INVALID_ARGUMENT;
+    *enabled = ctx->use_graphs ? 1 : 0;
     return PFH_OK;
 }
 

@@ -881,10 +881,13 @@ class Renderer:
         self._check(self._lib.pfh_set_launch_shape(self._h, C.byref(s)))
 
     def autotune(self, frames=5):
-        """Picks the fastest launch shape for the last scene (pfh_autotune)."""
+        """Picks the fastest launch shape for the last scene, then eager
+        launches or graph replay (pfh_autotune)."""
         s = L.LaunchShape()
         self._check(self._lib.pfh_autotune(self._h, frames, C.byref(s)))
-        return {"grid_cap": s.grid_cap, "fill_grid": s.fill_grid}
+        g = C.c_int32()
+        self._check(self._lib.pfh_get_graphs(self._h, C.byref(g)))
+        return {"grid_cap": s.grid_cap, "fill_grid": s.fill_grid, "graphs": bool(g.value)}
 
     def set_graphs(self, enable=True):
         """Re-render frames through a captured HIP graph."""

@@ -63,6 +63,8 @@ def test_autotune_picks_a_candidate(renderer_factory):
     print("autotuned", shape, flush=True)
     assert shape["grid_cap"] in (1024, 2048, 4096, 8192, 16384)
     assert shape["fill_grid"] in (1024, 2048, 4096, 8192, 16384)
-    assert r.launch_shape() == shape
-    r.rerender()
-    assert np.array_equal(r.read_pixels(), px0)
+    assert r.launch_shape() == {k: shape[k] for k in ("grid_cap", "fill_grid")}
+    assert isinstance(shape["graphs"], bool)  # eager or graph replay, whichever was faster
+    for _ in range(3):
+        r.rerender()
+        assert np.array_equal(r.read_pixels(), px0)
