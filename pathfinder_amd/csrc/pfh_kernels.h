@@ -100,7 +100,8 @@ struct BatchArgs {
     uint32_t *mask;                // 64 words per alpha tile
     uint32_t mask_capacity;        // alpha tiles
     const float4 *lut;             // padded area LUT (259 rows of 260 texels, byte / 255)
-    float *coverage;               // optional: 256 floats per alpha tile, the signed
+    const uint32_t *lut_sat;       // its saturated cells per row (pfh_lut_saturation)
+    float *coverage;              // optional: 256 floats per alpha tile, the signed
                                    // coverage sum before abs/clamp (adjoint checkpoint)
     uint32_t *tile_paths;          // optional: the batch path of every tile (adjoint)
     uint8_t *cov_state;            // optional: per pixel 2 bits (the adjoint's gate

@@ -1921,50 +1921,53 @@ __device__ __forceinline__ uint32_t unorm8_unit(float c) { return (uint32_t)(c *
 // mask texel, unorm8 store of the four rows of this lane's mask texel. The
 // coverage sums are finite (sums of LUT texels in [0, 1] times window widths
 // in [-1, 1]), so clamp(|x|, 0, 1) is med3(|x|, 0, 1), and the stored values
-// need no second clamp.
+// need no second clamp. For the adjoint it also writes, as asked, the signed
+// sums (coverage) or their 2-bit states (cov_state: the gate 0 < |C| < 1
+// with the clip texel not below min(|C|, 1), and the sign), and in a
+// recompute pass (states_only) only the states.
 __device__ __forceinline__ void store_mask(const BatchArgs &a, uint32_t alpha_index,
                                            int32_t backdrop, int32_t clip, uint32_t lane,
                                            float4 acc) {
     float bd = (float)backdrop;
+    float C0 = bd + acc.x, C1 = bd + acc.y, C2 = bd + acc.z, C3 = bd + acc.w;
+    bool clipped = clip >= 0 && (uint32_t)clip < a.mask_capacity;
+    if (!a.states_only) {
+        float c0 = __builtin_amdgcn_fmed3f(fabsf(C0), 0.0f, 1.0f);
+        float c1 = __builtin_amdgcn_fmed3f(fabsf(C1), 0.0f, 1.0f);
+        float c2 = __builtin_amdgcn_fmed3f(fabsf(C2), 0.0f, 1.0f);
+        float c3 = __builtin_amdgcn_fmed3f(fabsf(C3), 0.0f, 1.0f);
+        if (clipped) {
+            uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
+            c0 = min_glsl(c0, byte_to_unit(ct & 0xffu));
+            c1 = min_glsl(c1, byte_to_unit((ct >> 8) & 0xffu));
+            c2 = min_glsl(c2, byte_to_unit((ct >> 16) & 0xffu));
+            c3 = min_glsl(c3, byte_to_unit(ct >> 24));
+        }
+        a.mask[(size_t)alpha_index * 64 + lane] = unorm8_unit(c0) | (unorm8_unit(c1) << 8) |
+                                                  (unorm8_unit(c2) << 16) |
+                                                  (unorm8_unit(c3) << 24);
+    }
     if (a.cov_state) {
-        // what the adjoint reads of C (k_tile_backward): the gate 0 < |C| < 1
-        // with the clip texel not below min(|C|, 1), and the sign
-        uint32_t ct = 0;
-        if (clip >= 0 && (uint32_t)clip < a.mask_capacity)
-            ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
-        const float C[4] = {bd + acc.x, bd + acc.y, bd + acc.z, bd + acc.w};
+        uint32_t ct = clipped ? a.mask[(size_t)(uint32_t)clip * 64 + lane] : 0u;
+        const float C[4] = {C0, C1, C2, C3};
         uint32_t st = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             float aC = fabsf(C[k]);
-            bool limited = clip >= 0 && byte_to_unit((ct >> (8 * k)) & 0xffu) < min(aC, 1.0f);
+            bool limited = clipped && byte_to_unit((ct >> (8 * k)) & 0xffu) < min(aC, 1.0f);
             bool open = aC > 0.0f && aC < 1.0f && !limited;
             st |= ((open ? 1u : 0u) | (C[k] < 0.0f ? 2u : 0u)) << (2 * k);
         }
         a.cov_state[(size_t)alpha_index * 64 + lane] = (uint8_t)st;
-        if (a.states_only) return;
     }
-    float c0 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.x), 0.0f, 1.0f);
-    float c1 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.y), 0.0f, 1.0f);
-    float c2 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.z), 0.0f, 1.0f);
-    float c3 = __builtin_amdgcn_fmed3f(fabsf(bd + acc.w), 0.0f, 1.0f);
-    if (clip >= 0 && (uint32_t)clip < a.mask_capacity) {
-        uint32_t ct = a.mask[(size_t)(uint32_t)clip * 64 + lane];
-        c0 = min_glsl(c0, byte_to_unit(ct & 0xffu));
-        c1 = min_glsl(c1, byte_to_unit((ct >> 8) & 0xffu));
-        c2 = min_glsl(c2, byte_to_unit((ct >> 16) & 0xffu));
-        c3 = min_glsl(c3, byte_to_unit(ct >> 24));
-    }
-    a.mask[(size_t)alpha_index * 64 + lane] = unorm8_unit(c0) | (unorm8_unit(c1) << 8) |
-                                              (unorm8_unit(c2) << 16) | (unorm8_unit(c3) << 24);
     // checkpoint for the adjoint: the signed sum before abs and clamp, pixel
     // (column lane & 15, row 4 (lane >> 4) + k) at [alpha][k][lane]
     if (a.coverage) {
         float *cv = a.coverage + (size_t)alpha_index * 256 + lane;
-        cv[0] = bd + acc.x;
-        cv[64] = bd + acc.y;
-        cv[128] = bd + acc.z;
-        cv[192] = bd + acc.w;
+        cv[0] = C0;
+        cv[64] = C1;
+        cv[128] = C2;
+        cv[192] = C3;
     }
 }
 
@@ -2005,38 +2008,67 @@ __device__ __forceinline__ void column_ballots(uint32_t pc, uint32_t &lo, uint32
 }
 
 constexpr uint32_t kFillGroup = 64;  // alpha tiles per wavefront group
-constexpr uint32_t kZeroSlot = 256;  // res entry that holds float4(0)
 
-// computeCoverage (fill_area.inc.glsl:11-27) of one fill for the pixel
-// column centred at x and all four row groups of the tile, written to
-// out[64 g] for row group g (rows 4g..4g+3). Every value is bit-identical to
-// coverage_record(lut, f, x, 4g + 0.5): the x window, the quotient, the LUT
-// row (u) and its weight depend on the column only and are computed once;
-// per row group only yy, the LUT column (s) and the bilinear blend remain.
-__device__ __forceinline__ void coverage_pair(LutRef lut, const FillRecord &f, float x,
-                                              float4 *out) {
+// computeCoverage (fill_area.inc.glsl:11-27) of one fill at the pixel column
+// centred at x, for the four row groups of the tile (rows 4g .. 4g + 3, the
+// four channels of a mask texel), in two halves: pair_column computes what
+// depends on the column only (the x window, the quotient, the rise, the LUT
+// row u and its weight), pair_group the rest for group g (yy, the LUT column
+// s, the bilinear blend). Every value is bit-identical to
+// coverage_record(lut, f, x, 4g + 0.5).
+struct PairColumn {
+    float ly, rise, wb, dX;  // fill's left y, rise to the column, LUT row weight, window
+};
+
+// pair_column also tells which row groups read saturated LUT cells. Group g
+// samples padded cell (jp, ip_g) with ip_g = clamp(floor(sc_g) + 1, 0, 257);
+// where that cell is all 255 (ip_g <= F[jp], lut_sat) the blend is exactly 1
+// and the value exactly dX, where it is all 0 (ip_g >= Z[jp]) exactly +-0.
+// sc_g equals sc_0 - 64 g to within 1.9e-4 (|ly|, |rise| <= 16, so |yy| <= 32
+// and |sc| < 640: half-ulp errors of 2^-19 in yy and in yy + 8, times 16, and
+// 2^-15 in sc, for both), so floor(sc_g) lies between floor(sc_0 - 4e-4) and
+// floor(sc_0 + 4e-4), less 64 g, and the groups are classified on those
+// bounds: groups [0, nz) certainly zero, [gf, 4) certainly dX, the groups
+// between to be sampled by pair_group. row0 is the byte offset of LUT row jp.
+__device__ __forceinline__ PairColumn pair_column(const FillRecord &f, float x,
+                                                  const uint32_t *sat, uint32_t &row0,
+                                                  uint32_t &nz, uint32_t &gf) {
+    PairColumn c;
     float wx = __builtin_amdgcn_fmed3f(f.fx - x, -0.5f, 0.5f);
     float wy = __builtin_amdgcn_fmed3f(f.tx - x, -0.5f, 0.5f);
     float offset = __builtin_fmaf(0.5f, wy - wx, wx) - (f.lx - x);
     float q0 = offset * f.rw;
     float t = __builtin_fmaf(__builtin_fmaf(-q0, f.w, offset), f.rw, q0);
-    float rise = t * f.dy;
-    float dX = wx - wy;
-    float u = __builtin_fmaf(fabsf(f.d * dX), 16.0f, -0.5f);
+    c.rise = t * f.dy;
+    c.dX = wx - wy;
+    c.ly = f.ly;
+    float u = __builtin_fmaf(fabsf(f.d * c.dX), 16.0f, -0.5f);
     float fu = floorf(u);
-    float wb = u - fu;
+    c.wb = u - fu;
     uint32_t jp = (uint32_t)min(max((int32_t)fu + 1, 0), 257);
-    uint32_t row0 = jp * kLutStride * kLutTexelBytes;
-#pragma unroll
-    for (uint32_t g = 0; g < 4; g++) {
-        float yy = (f.ly - ((float)(g * 4u) + 0.5f)) + rise;
-        float sc = __builtin_fmaf(yy + 8.0f, 16.0f, -0.5f);
-        float fs = floorf(sc);
-        float wa = sc - fs;
-        uint32_t ip = (uint32_t)min(max((int32_t)fs + 1, 0), 257);
-        float4 r = lut_blend(lut, row0 + ip * kLutTexelBytes, wa, wb);
-        out[64u * g] = make_float4(r.x * dX, r.y * dX, r.z * dX, r.w * dX);
-    }
+    row0 = jp * kLutStride * kLutTexelBytes;
+    float yy0 = (f.ly - 0.5f) + c.rise;
+    float s0 = __builtin_fmaf(yy0 + 8.0f, 16.0f, -0.5f);
+    int32_t lo = (int32_t)floorf(s0 - 4e-4f) + 1, hi = (int32_t)floorf(s0 + 4e-4f) + 1;
+    uint32_t w = sat[jp];
+    int32_t full = (int32_t)(int16_t)(w & 0xffffu), zero = (int32_t)(w >> 16);
+    // zero: lo - 64 g >= Z; dX: hi - 64 g <= F (the clamps to [0, 257] are
+    // monotone, so this never claims a group the clamped index would not)
+    nz = (uint32_t)min(max(((lo - zero) >> 6) + 1, 0), 4);
+    gf = (uint32_t)min(max((hi - full + 63) >> 6, 0), 4);
+    return c;
+}
+
+// The value of row group g of a pair from pair_column's results.
+__device__ __forceinline__ float4 pair_group(LutRef lut, const PairColumn &c, uint32_t row0,
+                                             uint32_t g) {
+    float yy = (c.ly - ((float)(g * 4u) + 0.5f)) + c.rise;
+    float sc = __builtin_fmaf(yy + 8.0f, 16.0f, -0.5f);
+    float fs = floorf(sc);
+    float wa = sc - fs;
+    uint32_t ip = (uint32_t)min(max((int32_t)fs + 1, 0), 257);
+    float4 r = lut_blend(lut, row0 + ip * kLutTexelBytes, wa, c.wb);
+    return make_float4(r.x * c.dX, r.y * c.dX, r.z * c.dX, r.w * c.dX);
 }
 
 // fill.cs.glsl:61-88 with fill_compute.inc.glsl:11-25. Lane (lx, ly) of a
@@ -2048,34 +2080,46 @@ __device__ __forceinline__ void coverage_pair(LutRef lut, const FillRecord &f, f
 // after tile, 64 at a time. Lane k turns fill k of a chunk into a record, and
 // the chunk's (fill, pixel column) pairs are numbered in stream order. Round
 // by round (64 pairs), lane i evaluates computeCoverage of pair i for the
-// four row groups into LDS; then the wavefront walks the round's fills in
-// stream order and every lane adds the value of its column, or exactly +0
-// where the fill's window does not see the column (dX = 0 there, and the
-// sum ends in abs(), so +-0 changes no bit), storing a tile's mask when the
-// stream moves on to the next tile. A pixel thus sums the same fills in the
-// same order as the reference's list walk.
+// four row groups into LDS (pair_column: the LUT is sampled only where its
+// cell is not saturated), the values stored column by column in pair order.
+// Then, tile by tile through the round, every lane adds its column's values
+// of that tile, in order, to its four sums; a fill whose window does not see
+// a column adds exactly +-0 there (dX = 0) and the sum ends in abs(), so it
+// is left out. A tile's mask is stored once the stream has passed its last
+// fill. A pixel thus sums the same fills in the same order as the
+// reference's list walk.
 // Tiles whose fills are split over several runs (rare) take the list walk
 // one chunk at a time instead (accumulate_chunk).
-__global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
+constexpr uint32_t kZeroEntry = 3 * 65 + 64;  // the value entry that holds +0
+
+// Five wavefronts per SIMD: at most 96 VGPRs and 32 KB of LDS per workgroup.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+k_fill(BatchArgs a) {
     __shared__ FillRecord s_rec[kBlock / 64][64];
-    __shared__ float4 s_res[kBlock / 64][4 * 64 + 1];
-    __shared__ int32_t s_cadj[kBlock / 64][64];
+    // values: column slot s (0..63) of row group g at s + 65 g; then +0
+    __shared__ float4 s_res[kBlock / 64][kZeroEntry + 1];
+    __shared__ uint32_t s_cadj[kBlock / 64][64];
     __shared__ uint8_t s_pair[kBlock / 64][64 * 16];
     __shared__ unsigned long long s_cmask[kBlock / 64][16];
-    __shared__ uint32_t s_cstart[kBlock / 64][17];
+    __shared__ uint8_t s_cstart[kBlock / 64][16];
+    __shared__ uint8_t s_quad[kBlock / 64][4 * 64];
+    __shared__ uint32_t sat[258];
+    for (uint32_t i = threadIdx.x; i < 258u; i += blockDim.x) sat[i] = a.lut_sat[i];
+    __syncthreads();
     uint32_t wv = threadIdx.x >> 6, lane = lane_id();
     FillRecord *rec = s_rec[wv];
     float4 *res = s_res[wv];
-    int32_t *cadj = s_cadj[wv];
+    uint32_t *cadj = s_cadj[wv];
     uint8_t *pair_fill = s_pair[wv];
     unsigned long long *cmask = s_cmask[wv];
-    uint32_t *cstart = s_cstart[wv];
+    uint8_t *cstart = s_cstart[wv];
+    uint8_t *quad = s_quad[wv];
+    if (lane == 0) res[kZeroEntry] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     LutRef lut = lut_ref(a.lut);
     uint32_t n_alpha = uniform(a.counters->alpha);
     uint32_t start = uniform(a.counters->alpha_start);
     uint32_t col = lane & 15u, grp = lane >> 4;
     float fragx = (float)col + 0.5f, fragy = (float)(grp * 4u) + 0.5f;
-    if (lane == 0) res[kZeroSlot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t stride = ((gridDim.x * blockDim.x) >> 6) * kFillGroup;
     for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kFillGroup);
          g0 < n_alpha; g0 += stride) {
@@ -2103,7 +2147,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
         uint32_t S = incl - n;
         uint32_t total = last_lane(incl);
 
-        int32_t cur = -1;
+        // this lane's sums for the tile the stream is in
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t ts;
         uint2 fw;
@@ -2122,35 +2166,35 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             uint32_t pe = wave_inclusive_scan(span);
             uint32_t pb = pe - span;
             uint32_t pairs = last_lane(pe);
-            // what the accumulation reads of fill k: first column, visible
-            // columns, first pair, tile slot
-            uint32_t info = (fc.c0 & 15u) | (nvis << 4) | (pb << 9) | (ts << 20);
             if (have) {
                 rec[lane] = f;
                 // first column - first pair (low 16 bits, signed), first column
-                // (4 bits), visible columns (5 bits)
-                cadj[lane] = (int32_t)((((uint32_t)fc.c0 - pb) & 0xffffu) |
-                                       ((fc.c0 & 15u) << 16) | (nvis << 20));
+                // (4 bits), visible columns (5 bits), tile slot (6 bits)
+                cadj[lane] = (((uint32_t)fc.c0 - pb) & 0xffffu) | ((fc.c0 & 15u) << 16) |
+                             (nvis << 20) | (ts << 25);
                 for (uint32_t j = 0; j < span; j++) pair_fill[pb + j] = (uint8_t)lane;
             }
+            // the fills that end their tile's stream
+            unsigned long long fin_m = __ballot(have && tlast);
             // the next chunk's fills load meanwhile
             uint32_t nts;
             uint2 nfw;
             bool ntlast;
             fill_lookup(a, pos + 64u, total, S, rx, n, lane, nts, nfw, ntlast);
-            (void)tlast;
             wave_lds_sync();
             for (uint32_t r0 = 0; r0 < pairs; r0 += 64u) {
                 uint32_t r1 = min(r0 + 64u, pairs);
-                // lane i: pair r0 + i, its fill k and pixel column pc (16 when
-                // the fill's window sees no column: its value is never read)
+                // lane i: pair r0 + i, its fill k, tile slot pt and pixel
+                // column pc (16 when the fill's window sees no column: no
+                // value)
                 uint32_t p = r0 + lane;
-                uint32_t k = 0, pc = 16u;
+                uint32_t k = 0, pc = 16u, pt = 64u;
                 if (p < r1) {
                     k = pair_fill[p];
-                    uint32_t ck = (uint32_t)cadj[k];
+                    uint32_t ck = cadj[k];
                     uint32_t c = p + (uint32_t)(int32_t)(int16_t)(ck & 0xffffu);
-                    if (c - ((ck >> 16) & 15u) < (ck >> 20)) pc = c;
+                    if (c - ((ck >> 16) & 15u) < ((ck >> 20) & 31u)) pc = c;
+                    pt = ck >> 25;
                 }
                 // The round's values are stored column by column: column q's
                 // pairs, in pair (= stream) order, from cstart[q] on. Lane
@@ -2163,54 +2207,82 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
                 uint32_t cs = wave_inclusive_scan(cnt) - cnt;
                 if (lane < 16u) {
                     cmask[lane] = lanes_q;
-                    cstart[lane] = cs;
+                    cstart[lane] = (uint8_t)cs;
                 }
+                // the lanes whose pair starts a tile's run in the round
+                uint32_t prev = (uint32_t)__shfl((int)pt, (int)lane - 1, 64);
+                unsigned long long seg_m = __ballot(p < r1 && (lane == 0u || pt != prev));
                 wave_lds_sync();
+                // saturated row groups directly; the others (mixed) as a list
+                // of (pair, group) items sampled 64 at a time
+                uint32_t nz = 0, gf = 0, slot = 0, row0 = 0;
+                PairColumn pcv = {0.0f, 0.0f, 0.0f, 0.0f};
                 if (pc < 16u) {
                     uint32_t rank = (uint32_t)__popcll(cmask[pc] & ((1ull << lane) - 1ull));
-                    float x = (float)pc + 0.5f;
-                    coverage_pair(lut, rec[k], x, res + cstart[pc] + rank);
+                    slot = cstart[pc] + rank;
+                    pcv = pair_column(rec[k], (float)pc + 0.5f, sat, row0, nz, gf);
+                    float4 full = make_float4(pcv.dX, pcv.dX, pcv.dX, pcv.dX);
+                    float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                    for (uint32_t g = 0; g < 4; g++) {
+                        if (g < nz) res[slot + 65u * g] = zero;
+                        if (g >= gf) res[slot + 65u * g] = full;
+                    }
+                }
+                uint32_t nmix = gf - nz;
+                uint32_t qe = wave_inclusive_scan(nmix);
+                uint32_t nq = last_lane(qe);
+                uint32_t qb = qe - nmix;
+#pragma unroll
+                for (uint32_t i = 0; i < 4u; i++)
+                    if (i < nmix) quad[qb + i] = (uint8_t)(lane | ((nz + i) << 6));
+                wave_lds_sync();
+                for (uint32_t q0 = 0; q0 < nq; q0 += 64u) {
+                    uint32_t q = q0 + lane;
+                    uint32_t d = quad[min(q, 255u)];
+                    uint32_t src = d & 63u, g = d >> 6;
+                    // the item's pair data from its pair's lane
+                    PairColumn qc;
+                    qc.ly = __shfl(pcv.ly, (int)src, 64);
+                    qc.rise = __shfl(pcv.rise, (int)src, 64);
+                    qc.wb = __shfl(pcv.wb, (int)src, 64);
+                    qc.dX = __shfl(pcv.dX, (int)src, 64);
+                    uint32_t qrs = (uint32_t)__shfl((int)(row0 | (slot << 24)), (int)src, 64);
+                    if (q < nq) res[(qrs >> 24) + 65u * g] = pair_group(lut, qc, qrs & 0xffffffu, g);
                 }
                 wave_lds_sync();
-                // this lane's column: which pairs of the round it holds and
-                // where its values start
+                // tile by tile: this lane's column values of the tile's run
+                // [L, E) of pair lanes are its values from e_prev to e
                 unsigned long long my_lanes = cmask[col];
-                const float4 *mine_v = res + 64u * grp + cstart[col];
-                // the fills with a pair in [r0, r1), tile by tile in stream
-                // order; a tile's pairs in the round are [pa, pz) and this
-                // lane's of them are its column's values [s, e)
-                uint32_t ka = (uint32_t)__popcll(__ballot(pe <= r0) & have_m);
-                uint32_t kb = (uint32_t)__popcll(__ballot(pb < r1) & have_m);
-                for (uint32_t kk = ka; kk < kb;) {
-                    int32_t t = (int32_t)__builtin_amdgcn_readlane((int)ts, (int)kk);
-                    if (t != cur) {
-                        if (cur >= 0)
-                            store_mask(a, start + g0 + (uint32_t)cur,
-                                       (int32_t)__builtin_amdgcn_readlane(t_w, cur) >> 24,
-                                       (int32_t)__builtin_amdgcn_readlane(t_clip, cur), lane,
-                                       acc);
-                        cur = t;
-                        acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    }
-                    // nonzero: lane kk is in it
-                    unsigned long long seg = __ballot(ts == (uint32_t)t) & have_m;
-                    uint32_t ke = min(64u - (uint32_t)__builtin_clzll(seg), kb);
-                    uint32_t pa = max((uint32_t)__builtin_amdgcn_readlane((int)pb, (int)kk), r0) - r0;
-                    uint32_t pz =
-                        min((uint32_t)__builtin_amdgcn_readlane((int)pe, (int)(ke - 1u)), r1) - r0;
-                    unsigned long long la = pa >= 64u ? ~0ull : (1ull << pa) - 1ull;
-                    unsigned long long lz = pz >= 64u ? ~0ull : (1ull << pz) - 1ull;
-                    uint32_t sj = (uint32_t)__popcll(my_lanes & la);
-                    uint32_t ej = (uint32_t)__popcll(my_lanes & lz);
-                    // two values per step, read together (the second one is
-                    // the zero slot past the end)
-                    for (uint32_t jj = sj; jj < ej; jj += 2u) {
-                        float4 v = mine_v[jj];
-                        float4 w = jj + 1u < ej ? mine_v[jj + 1u] : res[kZeroSlot];
+                const float4 *mine = res + 65u * grp + cstart[col];
+                uint32_t e_prev = 0;
+                // the round ends its last tile's stream if pair r1 - 1 is the
+                // last pair of a fill that ends its tile
+                uint32_t kz = (uint32_t)__popcll(__ballot(pe < r1) & have_m);
+                bool fin = ((fin_m >> kz) & 1ull) &&
+                           (uint32_t)__builtin_amdgcn_readlane((int)pe, (int)kz) == r1;
+                while (seg_m != 0ull) {
+                    uint32_t L = (uint32_t)__ffsll((long long)seg_m) - 1u;
+                    seg_m &= seg_m - 1ull;
+                    uint32_t E = seg_m ? (uint32_t)__ffsll((long long)seg_m) - 1u : r1 - r0;
+                    uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)pt, (int)L);
+                    unsigned long long below = E >= 64u ? ~0ull : (1ull << E) - 1ull;
+                    uint32_t e = (uint32_t)__popcll(my_lanes & below);
+                    // two values per step, read together (the second one
+                    // from the zero entry past the end)
+                    for (uint32_t j = e_prev; j < e; j += 2u) {
+                        float4 v = mine[j];
+                        float4 w = j + 1u < e ? mine[j + 1u] : res[kZeroEntry];
                         acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
                         acc.x += w.x; acc.y += w.y; acc.z += w.z; acc.w += w.w;
                     }
-                    kk = ke;
+                    e_prev = e;
+                    if (seg_m != 0ull || fin) {
+                        store_mask(a, start + g0 + t,
+                                   (int32_t)__builtin_amdgcn_readlane(t_w, t) >> 24,
+                                   (int32_t)__builtin_amdgcn_readlane(t_clip, t), lane, acc);
+                        acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
                 }
                 // every lane is done with this round's values
                 wave_lds_sync();
@@ -2219,10 +2291,6 @@ __global__ void __launch_bounds__(kBlock) k_fill(BatchArgs a) {
             fw = nfw;
             tlast = ntlast;
         }
-        if (cur >= 0)
-            store_mask(a, start + g0 + (uint32_t)cur,
-                       (int32_t)__builtin_amdgcn_readlane(t_w, cur) >> 24,
-                       (int32_t)__builtin_amdgcn_readlane(t_clip, cur), lane, acc);
 
         // tiles without fills: backdrop (and clip) only
         unsigned long long zero = __ballot(valid && !chained && n == 0u);

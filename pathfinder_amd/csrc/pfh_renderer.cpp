@@ -319,6 +319,7 @@ struct pfh_context {
     EventProfiler prof;
     // tables
     float4 *d_lut = nullptr;
+    uint32_t *d_lut_sat = nullptr;  // saturated LUT cells per row (lut_saturation)
     float4 *d_metadata = nullptr;  // 10 float4 per paint (TextureMetadataEntry texels)
     uint32_t paint_count = 0;
     bool paints_plain = true;      // every paint's composite control word is zero
@@ -406,6 +407,39 @@ pfh_status dalloc(pfh_context *ctx, T **out, size_t count) {
                         count * sizeof(T), hipGetErrorString(e));
     *out = static_cast<T *>(p);
     return PFH_OK;
+}
+
+// The saturated cells of the padded area LUT, for k_fill. Padded texel (j, i)
+// is byte texel (clamp(j - 1), clamp(i - 1)) of the 256 x 256 RGBA table
+// (clamp to [0, 255], row j); the bilinear sample at padded cell (jp, ip),
+// 0 <= jp, ip <= 257, blends texels (jp .. jp + 1, ip .. ip + 1). Where all
+// four hold 255 in every channel the blend is exactly 1 in every channel,
+// where all four hold 0 exactly 0. Word jp: low 16 bits the largest ip such
+// that every cell (jp, 0 .. ip) is all 255 (as int16; -0x7fff when cell
+// (jp, 0) is not), high 16 bits the smallest ip such that every cell
+// (jp, ip .. 257) is all 0 (0x7fff when cell (jp, 257) is not). The sentinels
+// lie far enough outside [0, 257] that no index compares past them.
+constexpr uint32_t kLutSatRows = 258;
+void lut_saturation(const uint8_t *rgba, uint32_t *out) {
+    auto texel = [&](int j, int i) {
+        j = std::min(std::max(j - 1, 0), 255);
+        i = std::min(std::max(i - 1, 0), 255);
+        const uint8_t *t = rgba + ((size_t)j * 256 + (size_t)i) * 4;
+        return (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16 | (uint32_t)t[3] << 24;
+    };
+    auto cell = [&](int jp, int ip, uint32_t v) {
+        return texel(jp, ip) == v && texel(jp + 1, ip) == v && texel(jp, ip + 1) == v &&
+               texel(jp + 1, ip + 1) == v;
+    };
+    for (int jp = 0; jp < (int)kLutSatRows; jp++) {
+        int full = -1;
+        while (full + 1 <= 257 && cell(jp, full + 1, 0xffffffffu)) full++;
+        int zero = 258;
+        while (zero - 1 >= 0 && cell(jp, zero - 1, 0u)) zero--;
+        if (full < 0) full = -0x7fff;
+        if (zero == 258) zero = 0x7fff;
+        out[jp] = ((uint32_t)full & 0xffffu) | ((uint32_t)zero << 16);
+    }
 }
 
 template <class T>
@@ -579,6 +613,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.mask = ctx->d_mask;
     a.mask_capacity = ctx->cap_a;
     a.lut = ctx->d_lut;
+    a.lut_sat = ctx->d_lut_sat;
     a.coverage = ctx->gradients && ctx->checkpoint == PFH_CHECKPOINT_COVERAGE ? ctx->d_coverage
                                                                              : nullptr;
     a.cov_state = ctx->gradients && ctx->checkpoint == PFH_CHECKPOINT_STATES ? ctx->d_cov_state
@@ -1108,9 +1143,18 @@ pfh_status pfh_context_create(const pfh_context_options *options, pfh_context **
     // 259 rows of 260 padded texels, each a texel and its difference to the next
     if (dalloc(ctx.get(), &ctx->d_lut, 2 * 259 * 260) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
     if (dalloc(ctx.get(), &ctx->d_frame, 1) != PFH_OK) return PFH_ERROR_OUT_OF_MEMORY;
+    if (dalloc(ctx.get(), &ctx->d_lut_sat, kLutSatRows) != PFH_OK)
+        return PFH_ERROR_OUT_OF_MEMORY;
     const uint8_t *lut = opt.area_lut_rgba ? opt.area_lut_rgba : pfh_area_lut_rgba;
     if (hipMemcpy(d_bytes, lut, 256 * 256 * 4, hipMemcpyHostToDevice) != hipSuccess)
         return PFH_ERROR_HIP;
+    {
+        std::vector<uint32_t> sat(kLutSatRows);
+        lut_saturation(lut, sat.data());
+        if (hipMemcpy(ctx->d_lut_sat, sat.data(), sat.size() * 4, hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return PFH_ERROR_HIP;
+    }
     pfh::launch_lut_expand(d_bytes, ctx->d_lut, ctx->stream);
     pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return PFH_ERROR_HIP;
