@@ -1992,19 +1992,20 @@ __device__ __forceinline__ void fill_lookup(const BatchArgs &a, uint32_t pos, ui
     if (q < total) fw = a.sorted_fills[trx + tn - 1u - (q - tS)];
 }
 
-// The compiler's writelane intrinsic (v_writelane_b32: `value` into lane
-// `lane` of `old`), which this clang has no builtin for.
-__device__ int lane_write(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
-
-// Lane Q of (lo, hi) for Q = Q0..15: the ballot of pc == Q (one compare and
-// two lane writes per column; a select on lane == Q needs two moves and two
-// selects more).
-template <uint32_t Q>
-__device__ __forceinline__ void column_ballots(uint32_t pc, uint32_t &lo, uint32_t &hi) {
-    unsigned long long m = __ballot(pc == Q);
-    lo = (uint32_t)lane_write((int)(uint32_t)m, (int)Q, (int)lo);
-    hi = (uint32_t)lane_write((int)(uint32_t)(m >> 32), (int)Q, (int)hi);
-    if constexpr (Q + 1u < 16u) column_ballots<Q + 1u>(pc, lo, hi);
+// The lanes whose pc equals this lane's index, for lanes 0..15 (0 in lanes
+// 16..63), pc in 0..16 (16: none): the AND over pc's four low bits of the
+// ballot of the bit or of its complement, as the lane index has the bit or
+// not, with the ballot of pc < 16. Five ballots and a few bit operations
+// per lane, in place of one ballot and two lane writes per column.
+__device__ __forceinline__ unsigned long long column_lanes(uint32_t pc, uint32_t lane) {
+    unsigned long long m = __ballot(pc < 16u);
+#pragma unroll
+    for (uint32_t b = 0; b < 4; b++) {
+        unsigned long long bits = __ballot((pc >> b) & 1u);
+        unsigned long long want = ((lane >> b) & 1u) ? ~0ull : 0ull;
+        m &= ~(bits ^ want);
+    }
+    return lane < 16u ? m : 0ull;
 }
 
 constexpr uint32_t kFillGroup = 64;  // alpha tiles per wavefront group
@@ -2200,9 +2201,7 @@ k_fill(BatchArgs a) {
                 // pairs, in pair (= stream) order, from cstart[q] on. Lane
                 // q < 16 gets the mask of the lanes whose pair is in column q
                 // (0 in lanes 16..63).
-                uint32_t lq_lo = 0, lq_hi = 0;
-                column_ballots<0>(pc, lq_lo, lq_hi);
-                unsigned long long lanes_q = (unsigned long long)lq_hi << 32 | lq_lo;
+                unsigned long long lanes_q = column_lanes(pc, lane);
                 uint32_t cnt = (uint32_t)__popcll(lanes_q);
                 uint32_t cs = wave_inclusive_scan(cnt) - cnt;
                 if (lane < 16u) {
