@@ -75,7 +75,9 @@ def algorithmic_bytes(st, fb_tiles, pixels, points, load_action):
         "bound": 16 * P + 16 * T,
         "bin_count": 16 * M + 48 * P + 4 * M,
         "bin_emit": 16 * M + 48 * P + 4 * M + 12 * F + 4 * F,
-        "fill_link": 4 * F + 4 * F + 4 * F,
+        # fill tiles and words read, sorted words and links written, and per
+        # tile with fills its head, run and flag (one run per alpha tile)
+        "fill_link": (4 + 8) * F + (8 + 4) * F + (4 + 8 + 1) * A,
         "propagate_count": 12 * C + 16 * T + 4 * C,
         "propagate_emit": 12 * C + 16 * T + 12 * T + 8 * A + 4 * C,
         "sort_tiles": 8 * Fb + 4 * Lp + 4 * L,

@@ -1578,28 +1578,47 @@ __global__ void __launch_bounds__(kBlock) k_row_paths(BatchArgs a) {
 // In sequential order each link pushes onto the list head, so before the sort
 // a tile's next is the previous linked tile (the next lower index); the sort
 // keeps tiles with index >= z in ascending order and leaves culled tiles with
-// their pre-sort next. One wavefront walks the covering paths from the top
+// their pre-sort next. A wavefront walks the covering paths from the top
 // down, 64 at a time, and writes exactly those words; of each covering tile
 // it reads the status byte propagate wrote beside it.
-// A workgroup takes kSortBand framebuffer tiles of one row (a wavefront each),
-// lists in LDS the row's paths that overlap the band, and stages their rects
-// in LDS 64 at a time for all of its wavefronts.
-constexpr uint32_t kSortBand = 16;
+// A workgroup takes kSortBand framebuffer tiles of one row, two adjacent
+// tiles per wavefront (their status bytes share a cache line, so the second
+// probe of a path is a cache hit), lists in LDS the row's paths that overlap
+// the band, and stages their rects in LDS 64 at a time for all of its
+// wavefronts. Workgroups go to the eight XCDs in turn (workgroup i to XCD
+// i mod 8), so workgroup i takes band (i mod 8) * ceil(n / 8) + i / 8 of the
+// n bands in row-major order: each XCD takes a run of whole rows, and a
+// path's status bytes, read by the bands of adjacent rows, stay in one L2.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kSortWaves = 16;
+constexpr uint32_t kSortPer = 2;  // framebuffer tiles per wavefront
+constexpr uint32_t kSortBand = kSortWaves * kSortPer;
 constexpr uint32_t kBandCap = 8192;  // band list entries held in LDS
 
-__global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
+// The walk of one framebuffer tile: the z-buffer search, the links and the
+// list statistics, carried from chunk to chunk of covering paths.
+struct SortWalk {
+    bool z_found;
+    int32_t z;
+    int32_t succ;             // lowest linked tile above the current chunk
+    int32_t pending;          // culled tile whose next is in a lower chunk
+    int32_t lowest_survivor;
+    uint32_t n_linked, n_survivors, n_survivor_alpha;
+};
+
+__global__ void __launch_bounds__(kSortWaves * 64) k_sort_tiles(BatchArgs a) {
     __shared__ int32_t s_miny[64], s_minx[64], s_maxx[64];
     __shared__ uint32_t s_first[64];
     __shared__ uint32_t s_band[kBandCap];
-    __shared__ uint32_t s_wave[kSortBand + 1];
+    __shared__ uint32_t s_wave[kSortWaves + 1];
     uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
-    int32_t y = (int32_t)(blockIdx.x / bands);
-    uint32_t band = blockIdx.x % bands;
+    uint32_t per_xcd = gridDim.x / kXcds;
+    uint32_t linear = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    int32_t y = (int32_t)(linear / bands);
+    uint32_t band = linear % bands;
     if (y >= a.fb_height_tiles) return;
     uint32_t lane = lane_id();
-    int32_t x = (int32_t)(band * kSortBand + (threadIdx.x >> 6));
-    bool active = x < a.fb_width_tiles;
-    uint32_t wave = (uint32_t)y * (uint32_t)a.fb_width_tiles + (uint32_t)x;
+    int32_t x0 = (int32_t)(band * kSortBand + (threadIdx.x >> 6) * kSortPer);
     const uint32_t *list = a.row_paths + (size_t)y * a.path_count;
     uint32_t len = a.row_counts[y];
     // The row's paths that overlap this band, in order, compacted into LDS.
@@ -1619,7 +1638,7 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
         if (lane == 0) s_wave[threadIdx.x >> 6] = __popcll(m);
         __syncthreads();
         uint32_t before = 0, total = 0;
-        for (uint32_t w = 0; w < kSortBand; w++) {
+        for (uint32_t w = 0; w < kSortWaves; w++) {
             uint32_t c = s_wave[w];
             before += w < (threadIdx.x >> 6) ? c : 0u;
             total += c;
@@ -1632,12 +1651,10 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
     bool in_lds = band_len <= kBandCap;
     if (in_lds) len = band_len;
 
-    bool z_found = false;
-    int32_t z = 0;
-    int32_t succ = -1;        // lowest linked tile above the current chunk
-    int32_t pending = -1;     // culled tile whose next is in a lower chunk
-    int32_t lowest_survivor = -1;
-    uint32_t n_linked = 0, n_survivors = 0, n_survivor_alpha = 0;
+    SortWalk wk[kSortPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kSortPer; u++)
+        wk[u] = SortWalk{false, 0, -1, -1, -1, 0u, 0u, 0u};
     for (uint32_t end = len; end > 0;) {
         uint32_t begin = end > 64 ? end - 64 : 0;
         uint32_t k = begin + lane;
@@ -1650,76 +1667,94 @@ __global__ void __launch_bounds__(kSortBand * 64) k_sort_tiles(BatchArgs a) {
             s_first[lane] = info.z;
         }
         __syncthreads();
-        bool valid = active && k < end;
-        int32_t t = -1;
-        bool linked = false, occl = false, alpha_tile = false;
-        if (valid) {
-            int32_t miny = s_miny[lane], minx = s_minx[lane], maxx = s_maxx[lane];
-            uint32_t ti = s_first[lane] + (uint32_t)(y - miny) * (uint32_t)(maxx - minx) +
-                          (uint32_t)(x - minx);
-            if (x >= minx && x < maxx && ti < a.tile_count) {
-                t = (int32_t)ti;
-                uint32_t st = a.tile_status[t];  // words 2-3 as propagate wrote them
-                linked = (st & kTileLinked) != 0;
-                alpha_tile = (st & kTileAlpha) != 0;
-                occl = (st & kTileOccluder) != 0;
-            }
+        int32_t miny = 0, minx = 0, maxx = 0;
+        uint32_t first = 0;
+        if (k < end) {
+            miny = s_miny[lane];
+            minx = s_minx[lane];
+            maxx = s_maxx[lane];
+            first = s_first[lane];
         }
         __syncthreads();
-        unsigned long long lmask = __ballot(linked);
-        unsigned long long omask = __ballot(occl);
-        unsigned long long amask = __ballot(alpha_tile);
-        int32_t zlane;  // lanes >= zlane survive
-        if (z_found) {
-            zlane = 64;
-        } else if (omask) {
-            zlane = 63 - __clzll((long long)omask);
-            z = __shfl(t, zlane, 64);
-            z_found = true;
-        } else {
-            zlane = 0;
-        }
-        if (lmask) {
-            // the pending culled tile's next is this chunk's highest linked tile
-            int32_t hi_lane = 63 - __clzll((long long)lmask);
-            int32_t hi_t = __shfl(t, hi_lane, 64);
-            if (pending >= 0 && lane == 0) a.tile_next[pending] = (uint32_t)hi_t;
-            if (pending >= 0) pending = -1;
-            int32_t lo_lane = __ffsll((long long)lmask) - 1;
-            int32_t lo_t = __shfl(t, lo_lane, 64);
-            if (linked) {
-                unsigned long long above = lane == 63 ? 0ull : (lmask & (~0ull << (lane + 1)));
-                unsigned long long below = lmask & ((1ull << lane) - 1ull);
-                int32_t up_lane = above ? __ffsll((long long)above) - 1 : 64;
-                int32_t down_lane = below ? 63 - __clzll((long long)below) : -1;
-                int32_t up_t = __shfl(t, up_lane & 63, 64);
-                int32_t down_t = __shfl(t, down_lane & 63, 64);
-                if ((int32_t)lane >= zlane) {
-                    a.tile_next[t] = (uint32_t)(up_lane < 64 ? up_t : succ);
-                } else if (down_lane >= 0) {
-                    a.tile_next[t] = (uint32_t)down_t;
+#pragma unroll
+        for (uint32_t u = 0; u < kSortPer; u++) {
+            SortWalk &w = wk[u];
+            int32_t x = x0 + (int32_t)u;
+            bool valid = x < a.fb_width_tiles && k < end;
+            int32_t t = -1;
+            bool linked = false, occl = false, alpha_tile = false;
+            if (valid) {
+                uint32_t ti = first + (uint32_t)(y - miny) * (uint32_t)(maxx - minx) +
+                              (uint32_t)(x - minx);
+                if (x >= minx && x < maxx && ti < a.tile_count) {
+                    t = (int32_t)ti;
+                    uint32_t st = a.tile_status[t];  // words 2-3 as propagate wrote them
+                    linked = (st & kTileLinked) != 0;
+                    alpha_tile = (st & kTileAlpha) != 0;
+                    occl = (st & kTileOccluder) != 0;
                 }
-            } else {
-                // keep the shuffles uniform across the wave
-                (void)__shfl(t, 0, 64);
-                (void)__shfl(t, 0, 64);
             }
-            // lowest linked lane culled -> its next lies in a lower chunk
-            if (lo_lane < zlane) pending = lo_t;
-            unsigned long long surv = lmask & (zlane >= 64 ? 0ull : (~0ull << zlane));
-            if (surv) lowest_survivor = __shfl(t, __ffsll((long long)surv) - 1, 64);
-            n_linked += __popcll(lmask);
-            n_survivors += __popcll(surv);
-            n_survivor_alpha += __popcll(surv & amask);
-            succ = lo_t;
+            unsigned long long lmask = __ballot(linked);
+            unsigned long long omask = __ballot(occl);
+            unsigned long long amask = __ballot(alpha_tile);
+            int32_t zlane;  // lanes >= zlane survive
+            if (w.z_found) {
+                zlane = 64;
+            } else if (omask) {
+                zlane = 63 - __clzll((long long)omask);
+                w.z = __shfl(t, zlane, 64);
+                w.z_found = true;
+            } else {
+                zlane = 0;
+            }
+            if (lmask) {
+                // the pending culled tile's next is this chunk's highest linked tile
+                int32_t hi_lane = 63 - __clzll((long long)lmask);
+                int32_t hi_t = __shfl(t, hi_lane, 64);
+                if (w.pending >= 0 && lane == 0) a.tile_next[w.pending] = (uint32_t)hi_t;
+                if (w.pending >= 0) w.pending = -1;
+                int32_t lo_lane = __ffsll((long long)lmask) - 1;
+                int32_t lo_t = __shfl(t, lo_lane, 64);
+                if (linked) {
+                    unsigned long long above = lane == 63 ? 0ull : (lmask & (~0ull << (lane + 1)));
+                    unsigned long long below = lmask & ((1ull << lane) - 1ull);
+                    int32_t up_lane = above ? __ffsll((long long)above) - 1 : 64;
+                    int32_t down_lane = below ? 63 - __clzll((long long)below) : -1;
+                    int32_t up_t = __shfl(t, up_lane & 63, 64);
+                    int32_t down_t = __shfl(t, down_lane & 63, 64);
+                    if ((int32_t)lane >= zlane) {
+                        a.tile_next[t] = (uint32_t)(up_lane < 64 ? up_t : w.succ);
+                    } else if (down_lane >= 0) {
+                        a.tile_next[t] = (uint32_t)down_t;
+                    }
+                } else {
+                    // keep the shuffles uniform across the wave
+                    (void)__shfl(t, 0, 64);
+                    (void)__shfl(t, 0, 64);
+                }
+                // lowest linked lane culled -> its next lies in a lower chunk
+                if (lo_lane < zlane) w.pending = lo_t;
+                unsigned long long surv = lmask & (zlane >= 64 ? 0ull : (~0ull << zlane));
+                if (surv) w.lowest_survivor = __shfl(t, __ffsll((long long)surv) - 1, 64);
+                w.n_linked += __popcll(lmask);
+                w.n_survivors += __popcll(surv);
+                w.n_survivor_alpha += __popcll(surv & amask);
+                w.succ = lo_t;
+            }
         }
         end = begin;
     }
-    if (lane == 0 && active) {
-        if (pending >= 0) a.tile_next[pending] = 0xffffffffu;
-        a.first_tile_map[wave] = lowest_survivor;
-        a.zbuffer[8 + wave] = z_found ? z : 0;
-        a.list_stats[wave] = make_uint4(n_linked, n_survivors, n_survivor_alpha, 0);
+#pragma unroll
+    for (uint32_t u = 0; u < kSortPer; u++) {
+        int32_t x = x0 + (int32_t)u;
+        if (lane == 0 && x < a.fb_width_tiles) {
+            const SortWalk &w = wk[u];
+            uint32_t wave = (uint32_t)y * (uint32_t)a.fb_width_tiles + (uint32_t)x;
+            if (w.pending >= 0) a.tile_next[w.pending] = 0xffffffffu;
+            a.first_tile_map[wave] = w.lowest_survivor;
+            a.zbuffer[8 + wave] = w.z_found ? w.z : 0;
+            a.list_stats[wave] = make_uint4(w.n_linked, w.n_survivors, w.n_survivor_alpha, 0);
+        }
     }
 }
 
@@ -3343,8 +3378,10 @@ void launch_sort(const BatchArgs &a, hipStream_t s, Profiler *p) {
     if (fbn == 0) return;
     Scope t(p, s, K_SORT_TILES);
     uint32_t bands = ((uint32_t)a.fb_width_tiles + kSortBand - 1) / kSortBand;
-    hipLaunchKernelGGL(k_sort_tiles, dim3((uint32_t)a.fb_height_tiles * bands),
-                       dim3(kSortBand * 64), 0, s, a);
+    // a whole number of workgroups per XCD; those past the last band return
+    uint32_t n = (uint32_t)a.fb_height_tiles * bands;
+    hipLaunchKernelGGL(k_sort_tiles, dim3((n + kXcds - 1) / kXcds * kXcds),
+                       dim3(kSortWaves * 64), 0, s, a);
 }
 
 void launch_fill(const BatchArgs &a, hipStream_t s, Profiler *p) {
