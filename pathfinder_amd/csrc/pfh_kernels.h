@@ -88,6 +88,8 @@ struct BatchArgs {
     int32_t *seg_delta;            // [segment][column]: backdrop delta sum of the segment
     uint32_t *seg_alpha;           // [segment][column]: alpha tiles the segment allocates
     uint2 *alpha_tiles;            // tile_count
+    uint2 *alpha_words;            // per alpha tile: its tile's words 2-3 (k_fill reads
+                                   // them here, beside alpha_tiles, not at the tile)
     int32_t *zbuffer;              // 8 + fb tiles
     int32_t *first_tile_map;       // fb tiles
     uint32_t *row_paths;           // fb_height_tiles * path_count

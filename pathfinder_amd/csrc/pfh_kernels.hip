@@ -1425,9 +1425,9 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
                     need_alpha = false;
                 }
             }
+            uint32_t ai = alpha_base + count;
             if (need_alpha) {
                 if (EMIT) {
-                    uint32_t ai = alpha_base + count;
                     a.alpha_tiles[ai] = make_uint2(t, (uint32_t)clip_alpha);
                     draw_alpha = (int32_t)(ai + first_alpha);
                 }
@@ -1447,6 +1447,7 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
                 // words 0 and 1 live in tile_next / tile_first; the tile's
                 // 16 bytes are written whole (no partial-sector writes)
                 a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, z, w);
+                if (need_alpha) a.alpha_words[ai] = make_uint2(z, w);
                 // k_sort_tiles links the linked tiles inside the framebuffer
                 // grid; the others keep an empty link
                 int32_t gx = (int32_t)(rx0 + (uint32_t)tile_x), gy = (int32_t)(ry0 + y);
@@ -2166,7 +2167,7 @@ k_fill(BatchArgs a) {
         if (valid) {
             uint2 at = a.alpha_tiles[ba];
             t_clip = at.y;
-            uint2 zw = ((const uint2 *)(a.tiles + at.x))[1];
+            uint2 zw = a.alpha_words[ba];  // the tile's words 2-3
             valid = sext24(zw.x) >= 0;
             t_w = zw.y;
             if (valid) {

@@ -293,6 +293,7 @@ struct Batch {
     uint32_t *d_tile_delta = nullptr;  // bin's backdrop deltas per tile
     uint32_t *d_col_alpha = nullptr;
     uint2 *d_alpha = nullptr;
+    uint2 *d_alpha_words = nullptr;  // per alpha tile: its tile words 2-3
     int32_t *d_zbuffer = nullptr;
     int32_t *d_ftm = nullptr;
     uint32_t *d_row_paths = nullptr;
@@ -465,7 +466,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
     dfree(ctx, b.d_tile_flags); dfree(ctx, b.d_tile_delta);
     dfree(ctx, b.d_tile_status);
-    dfree(ctx, b.d_alpha); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
+    dfree(ctx, b.d_alpha); dfree(ctx, b.d_alpha_words); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
     dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
 }
@@ -602,6 +603,7 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.seg_delta = b.d_seg_delta;
     a.seg_alpha = b.d_seg_alpha;
     a.alpha_tiles = b.d_alpha;
+    a.alpha_words = b.d_alpha_words;
     a.zbuffer = b.d_zbuffer;
     a.first_tile_map = b.d_ftm;
     a.row_paths = b.d_row_paths;
@@ -937,6 +939,7 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_alpha, h.tile_count)) != PFH_OK) return s;
+    if ((s = dalloc(ctx, &b.d_alpha_words, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_zbuffer, fbn + 8)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_ftm, fbn)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_row_paths, (size_t)ctx->fbh * std::max<uint32_t>(h.path_count, 1))) !=
