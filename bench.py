@@ -371,6 +371,9 @@ def main():
     ap.add_argument("--diff-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the glyph-atlas and multi-canvas legs")
+    ap.add_argument("--shape", default=None, metavar="GRID_CAP,FILL_GRID",
+                    help="render with this launch shape and eager launches instead of "
+                         "autotuning (counter runs: no graph capture under the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the frames, run a plain 256 MiB device-to-device copy so a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE")
@@ -415,8 +418,14 @@ def main():
     st = r.stats()
 
     # launch shapes for this workload (pfh_autotune; results do not depend on them)
-    shape = r.autotune(frames=5)
-    print("bench: autotuned launch shape %s" % shape, file=sys.stderr, flush=True)
+    if args.shape:
+        grid_cap, fill_grid = (int(v) for v in args.shape.split(","))
+        r.set_launch_shape(grid_cap, fill_grid)
+        shape = {"grid_cap": grid_cap, "fill_grid": fill_grid, "graphs": False}
+        print("bench: launch shape %s" % shape, file=sys.stderr, flush=True)
+    else:
+        shape = r.autotune(frames=5)
+        print("bench: autotuned launch shape %s" % shape, file=sys.stderr, flush=True)
 
     # warm-up with every kernel timed alone (stages serialized on one stream:
     # overlapping kernels would share the time) to find the dominant one

@@ -1122,16 +1122,14 @@ __device__ void link_path(const BatchArgs &a, LinkLds &L, uint32_t f0, uint32_t 
 }
 
 // Paths of at most kLinkWaveFills fills whose tile rect has at most 2^24
-// tiles are linked by k_fill_link_small, a wavefront each; the others by
-// k_fill_link, a workgroup each.
+// tiles are linked by link_small_paths, a wavefront each; the others by
+// link_large_path, a workgroup each.
 constexpr uint32_t kLinkWaveFills = 256;
 __device__ __forceinline__ bool link_by_wave(uint32_t n, uint64_t rect) {
     return n <= kLinkWaveFills && rect <= (1ull << 24);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
-    __shared__ LinkLds lds;
-    uint32_t path = blockIdx.x;
+__device__ void link_large_path(const BatchArgs &a, LinkLds &lds, uint32_t path) {
     if (path >= a.path_count) return;
     uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
     if (f0 == f1) return;
@@ -1165,11 +1163,10 @@ __device__ __forceinline__ void link_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
+__device__ void link_small_paths(const BatchArgs &a, LinkWaveLds *lds, uint32_t block) {
     constexpr uint32_t kQ = kLinkWaveFills / 64;
-    __shared__ LinkWaveLds lds[kBlock / 64];
     uint32_t wave = threadIdx.x >> 6, lane = lane_id();
-    uint32_t path = blockIdx.x * (kBlock / 64) + wave;
+    uint32_t path = block * (kBlock / 64) + wave;
     if (path >= a.path_count) return;
     uint32_t f0 = a.path_fills[path], f1 = max(a.path_fills[path + 1], f0);
     uint32_t n = f1 - f0;
@@ -1307,6 +1304,21 @@ __global__ void __launch_bounds__(kBlock) k_fill_link_small(BatchArgs a) {
             if (!(start_bits >> k & 1u)) a.fill_links[f0 + i] = f0 + i - 1;
         }
     }
+}
+
+// Every path's fill lists in one launch: workgroup p < path_count links path
+// p if it is too large for a wavefront (link_path), the workgroups after them
+// link the small paths four at a time (link_small_paths), so both kinds run
+// side by side.
+__global__ void __launch_bounds__(kBlock) k_fill_link(BatchArgs a) {
+    __shared__ union {
+        LinkLds large;
+        LinkWaveLds small[kBlock / 64];
+    } lds;
+    if (blockIdx.x < a.path_count)
+        link_large_path(a, lds.large, blockIdx.x);
+    else
+        link_small_paths(a, lds.small, blockIdx.x - a.path_count);
 }
 
 // ------------------------------------------------------------------ propagate
@@ -3340,8 +3352,8 @@ void launch_fill_link(const BatchArgs &a, hipStream_t s, Profiler *p) {
     Scope t(p, s, K_FILL_LINK);
     hipLaunchKernelGGL(k_path_fills, dim3((a.path_count + kBlock) / kBlock), dim3(kBlock), 0, s,
                        a);
-    hipLaunchKernelGGL(k_fill_link_small, dim3((a.path_count + 3) / 4), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_fill_link, dim3(a.path_count), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_fill_link, dim3(a.path_count + (a.path_count + 3) / 4), dim3(kBlock), 0,
+                       s, a);
 }
 
 void launch_propagate(const BatchArgs &a, hipStream_t s, Profiler *p) {
