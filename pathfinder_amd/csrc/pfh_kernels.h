@@ -135,6 +135,7 @@ struct TileArgs {
     int32_t fb_width_tiles, fb_height_tiles;
     int32_t load;                  // 1: LOAD_ACTION_LOAD, 0: clear
     float clear[4];
+    int32_t plain_paints;          // every paint's control word is zero (solid src-over)
 };
 
 // The adjoint of composite (k_tile) for one draw batch drawn over a clear

@@ -2641,8 +2641,12 @@ __device__ float4 paint_color(const TileArgs &a, const float4 *m, float fx, floa
 // ------------------------------------------------------------------ tile
 
 // tile.cs.glsl:71-163 with calculateColor/sampleMask (tile_fragment.inc.glsl:
-// 539-614) for solid colours and the Normal composite op. One wavefront per
-// framebuffer tile, lane (lx, ly) shades rows 4ly..4ly+3 of column lx.
+// 539-614). One wavefront per framebuffer tile, lane (lx, ly) shades rows
+// 4ly..4ly+3 of column lx. kPaints: some paint has a colour texture, a
+// filter or a composite op other than Normal (paint_color); without it every
+// paint is a solid colour composited src-over, and the kernel is built
+// without paint_color's registers.
+template <bool kPaints>
 __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
     uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
@@ -2694,7 +2698,8 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
         // the paint's metadata: base colour (texel 2) and control word (texel 8)
         const float4 *md = color < a.paint_count ? a.metadata + (size_t)color * 10 : nullptr;
         float4 base = md ? md[2] : make_float4(0, 0, 0, 0);
-        bool plain = !md || md[8].x == 0.0f;  // no colour texture, Normal composite
+        // no colour texture, Normal composite
+        bool plain = !kPaints || !md || md[8].x == 0.0f;
         int32_t mask_ctrl = ctrl & 0x3;
         uint32_t texel = 0;
         if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
@@ -2719,7 +2724,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
                 d[sub][1] = d[sub][1] * ia + base.y * al;
                 d[sub][2] = d[sub][2] * ia + base.z * al;
                 d[sub][3] = d[sub][3] * ia + al;
-            } else {
+            } else if constexpr (kPaints) {
                 float fx = (float)(gx * 16 + lx) + 0.5f;
                 float fy = (float)(gy * 16 + ly * 4 + (uint32_t)sub) + 0.5f;
                 float4 c = paint_color(a, md, fx, fy, mask_alpha);
@@ -3408,7 +3413,10 @@ void launch_tile(const TileArgs &a, hipStream_t s, Profiler *p) {
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     if (fbn == 0) return;
     Scope t(p, s, K_TILE);
-    hipLaunchKernelGGL(k_tile, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+    if (a.plain_paints)
+        hipLaunchKernelGGL(k_tile<false>, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_tile<true>, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
 }
 
 void launch_fill_zero(float *dst, size_t count, hipStream_t s) {

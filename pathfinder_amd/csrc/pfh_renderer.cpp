@@ -649,6 +649,7 @@ pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
     t.mask = ctx->d_mask;
     t.metadata = ctx->d_metadata;
     t.paint_count = ctx->paint_count;
+    t.plain_paints = ctx->paints_plain ? 1 : 0;
     t.color_texture = nullptr;
     t.color_width = t.color_height = 0;
     t.color_flags = b.color_flags;

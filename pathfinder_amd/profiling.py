@@ -21,7 +21,7 @@ import os
 import re
 import sys
 
-_KERNEL_RE = re.compile(r"(k_\w+)\(")
+_KERNEL_RE = re.compile(r"(k_\w+)(?:<[^>]*>)?\(")
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 FRAME_MARK = "k_frame_reset"
 
