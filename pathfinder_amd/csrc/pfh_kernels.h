@@ -57,7 +57,7 @@ struct BatchArgs {
     float xform[6];                // m00 m01 m10 m11 tx ty
     // clip batch (propagate / fill)
     const int32_t *clip_prop_md;
-    const uint4 *clip_tiles;
+    const uint2 *clip_tiles;       // the clip batch's tile words 2-3
     uint32_t clip_path_count, clip_tile_count;
     // working buffers
     int32_t *backdrops;            // working copy, 3 words per column
@@ -73,8 +73,8 @@ struct BatchArgs {
     uint2 *sorted_fills;           // fill words 0-1 sorted by (tile, fill index), fill_capacity
     uint2 *tile_runs;              // per tile with fills: its last run (first sorted index, count)
     uint2 *run_prev;               // at a run's first sorted index: the previous run or (0, 0)
-    uint4 *tiles;                  // tile_count; words 0 and 1 are not kept here
-                                   // (tile_next, tile_first)
+    uint2 *tiles;                  // tile_count: words 2-3 (words 0 and 1 are kept in
+                                   // tile_next and tile_first)
     uint32_t *tile_next;           // tile word 0 of linked tiles (k_sort_tiles; propagate for
                                    // linked tiles outside the framebuffer)
     uint32_t *tile_first;          // tile word 1 of tiles with fills (k_fill_link)
@@ -117,7 +117,7 @@ struct BatchArgs {
 };
 
 struct TileArgs {
-    const uint4 *tiles;
+    const uint2 *tiles;            // tile words 2-3
     const uint32_t *tile_next;     // tile word 0 (the sorted list link)
     uint32_t tile_count;
     uint32_t mask_capacity;        // alpha tiles held by `mask`

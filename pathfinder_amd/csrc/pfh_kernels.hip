@@ -1388,7 +1388,7 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
             if (cload) {
                 uint32_t ux = (uint32_t)tile_x + rx0, uy = y + ry0;
                 uint32_t ct = cofs + (uy - cry0) * (crx1 - crx0) + (ux - crx0);
-                cl[u] = ((const uint2 *)(a.clip_tiles + min(ct, a.clip_tile_count - 1u)))[1];
+                cl[u] = a.clip_tiles[min(ct, a.clip_tile_count - 1u)];  // words 2-3
             }
         }
         bool stop = false;
@@ -1456,9 +1456,9 @@ __device__ uint32_t propagate_segment(const BatchArgs &a, uint32_t col, uint32_t
                               (tile_backdrop != 0 && draw_alpha < 0 && zwrite ? kTileOccluder
                                                                                : 0) |
                               (draw_alpha >= 0 ? kTileAlpha : 0));
-                // words 0 and 1 live in tile_next / tile_first; the tile's
-                // 16 bytes are written whole (no partial-sector writes)
-                a.tiles[t] = make_uint4(0xffffffffu, 0xffffffffu, z, w);
+                // words 0 and 1 live in tile_next / tile_first; words 2-3
+                // are written whole (no partial-sector writes)
+                a.tiles[t] = make_uint2(z, w);
                 if (need_alpha) a.alpha_words[ai] = make_uint2(z, w);
                 // k_sort_tiles links the linked tiles inside the framebuffer
                 // grid; the others keep an empty link
@@ -2673,7 +2673,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
     // loop the load is unconditional, at index 0 past the list's end (in
     // bounds: the loop runs only when tile_count > 0)
     bool valid = t >= 0 && (uint32_t)t < a.tile_count;
-    uint4 w = make_uint4(0, 0, 0, 0);
+    uint2 w = make_uint2(0, 0);  // tile words 2-3
     uint32_t w_next = 0xffffffffu;
     if (valid) {
         w = a.tiles[t];
@@ -2683,16 +2683,16 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
         int32_t tn = (int32_t)w_next;
         bool next_valid = tn >= 0 && (uint32_t)tn < a.tile_count;
         uint32_t tc = next_valid ? (uint32_t)tn : 0u;
-        uint4 wn = a.tiles[tc];
+        uint2 wn = a.tiles[tc];
         uint32_t wn_next = a.tile_next[tc];
-        int32_t alpha = sext24(w.z);
-        uint32_t color = w.w & 0xffffu;
-        int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
+        int32_t alpha = sext24(w.x);
+        uint32_t color = w.y & 0xffffu;
+        int32_t ctrl = (int32_t)((w.y >> 16) & 0xffu);
         float backdrop;
         if (alpha >= 0) {
             backdrop = 0.0f;
         } else {
-            backdrop = (float)((int32_t)w.w >> 24);
+            backdrop = (float)((int32_t)w.y >> 24);
             ctrl &= ~0x3;
         }
         // the paint's metadata: base colour (texel 2) and control word (texel 8)
@@ -2808,15 +2808,15 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
 
     // the layer's paint, alpha tile and mask value per sub-row, as k_tile reads them
     auto layer = [&](int32_t t, float4 &base, int32_t &alpha, uint32_t &color, float ma[4]) {
-        uint4 w = a.tiles[t];
-        alpha = sext24(w.z);
-        color = w.w & 0xffffu;
-        int32_t ctrl = (int32_t)((w.w >> 16) & 0xffu);
+        uint2 w = a.tiles[t];  // tile words 2-3
+        alpha = sext24(w.x);
+        color = w.y & 0xffffu;
+        int32_t ctrl = (int32_t)((w.y >> 16) & 0xffu);
         float backdrop;
         if (alpha >= 0) {
             backdrop = 0.0f;
         } else {
-            backdrop = (float)((int32_t)w.w >> 24);
+            backdrop = (float)((int32_t)w.y >> 24);
             ctrl &= ~0x3;
         }
         base = color < a.paint_count ? a.metadata[(size_t)color * 10 + 2] : make_float4(0, 0, 0, 0);
@@ -2981,7 +2981,7 @@ __device__ float2 microline_pixel_weights(const MicrolineGradArgs &g, uint32_t p
             if (tx >= rx0 && tx < rx1 && ty >= ry0 && ty < ry1) {
                 uint32_t tile = tile_base + (uint32_t)((ty - ry0) * rw + (tx - rx0));
                 if (tile < g.b.tile_count) {
-                    int32_t alpha = sext24(g.b.tiles[tile].z);
+                    int32_t alpha = sext24(g.b.tiles[tile].x);
                     if (alpha >= 0 && (uint32_t)alpha < g.alpha_capacity &&
                         g.grad_stamp[(uint32_t)alpha] == g.epoch) {
                         uint32_t px = (uint32_t)(cx & 15), py = (uint32_t)(cy & 15);

@@ -284,7 +284,7 @@ struct Batch {
     uint2 *d_sorted_fills = nullptr;
     uint2 *d_run_prev = nullptr;
     uint2 *d_tile_runs = nullptr;
-    uint4 *d_tiles = nullptr;
+    uint2 *d_tiles = nullptr;  // tile words 2-3
     uint32_t *d_tile_next = nullptr;   // tile word 0 (linked tiles)
     uint32_t *d_tile_first = nullptr;  // tile word 1 (tiles with fills)
     uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
@@ -1815,7 +1815,8 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
         size_t n = b->host.tile_count;
         std::vector<uint32_t> next(n), first(n);
         std::vector<uint8_t> status(n), flags(n);
-        HIP_TRY(ctx, hipMemcpy(dst, b->d_tiles, n * 16, hipMemcpyDeviceToHost));
+        std::vector<uint2> words(n);
+        HIP_TRY(ctx, hipMemcpy(words.data(), b->d_tiles, n * 8, hipMemcpyDeviceToHost));
         HIP_TRY(ctx, hipMemcpy(next.data(), b->d_tile_next, n * 4, hipMemcpyDeviceToHost));
         HIP_TRY(ctx, hipMemcpy(first.data(), b->d_tile_first, n * 4, hipMemcpyDeviceToHost));
         HIP_TRY(ctx, hipMemcpy(status.data(), b->d_tile_status, n, hipMemcpyDeviceToHost));
@@ -1824,6 +1825,8 @@ pfh_status pfh_read_debug(pfh_context *ctx, uint32_t batch_id, pfh_debug_kind ki
         for (size_t i = 0; i < n; i++) {
             out[4 * i] = (status[i] & pfh::kTileLinked) ? next[i] : 0xffffffffu;
             out[4 * i + 1] = flags[i] ? first[i] : 0xffffffffu;
+            out[4 * i + 2] = words[i].x;
+            out[4 * i + 3] = words[i].y;
         }
     } else if (bytes) {
         HIP_TRY(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
