@@ -2767,10 +2767,113 @@ __global__ void k_list_lengths(const uint4 *list_stats, uint32_t *lengths, uint3
     if (i < fb_tiles) lengths[i] = list_stats[i].y;
 }
 
+// The sum of v over the 64 lanes of a wavefront (every lane active), in an
+// SGPR: within each row of 16 lanes v += v of the lane 1, 2, 4, 8 below
+// (row_shr), then lane 15's sum is added to rows 1 and 3 (row_bcast:15) and
+// lane 31's to rows 2 and 3 (row_bcast:31); lane 63 holds the total. Register
+// moves between lanes, no LDS round trips (a butterfly of lane shuffles made
+// the tile adjoint three times slower). The order of the additions is fixed.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_or_zero_f(float x) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_or_zero_f<0x111, 0xf>(v);
+    v += dpp_or_zero_f<0x112, 0xf>(v);
+    v += dpp_or_zero_f<0x114, 0xf>(v);
+    v += dpp_or_zero_f<0x118, 0xf>(v);
+    v += dpp_or_zero_f<0x142, 0xa>(v);
+    v += dpp_or_zero_f<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// One layer of a framebuffer tile's list as k_tile reads it: the tile's words
+// 2-3 and its paint's base colour (the same in every lane) and this lane's
+// mask texel; for the gradients also its alpha tile's clip texel, fill
+// state or signed sums, and its draw path.
+struct LayerIn {
+    uint2 w;
+    float4 base;
+    uint32_t texel;
+    int32_t clip;       // clip alpha tile of the layer's alpha tile, or -1
+    uint32_t ctexel;    // this lane's texel of it
+    uint32_t cstate;    // this lane's 2-bit fill states (cov_state)
+    float C;            // this lane's signed coverage sum in sub-row sub (coverage checkpoint)
+    float T;            // this lane's T_k in sub-row sub (walk 2's output)
+    uint32_t draw_path;
+};
+
+// Loads the layers of list slots k0 .. k0 + kLayerGroup - 1 (below n) in a
+// few rounds of loads for the whole group (the slots' tiles, their words,
+// then paints, mask texels and the rest) instead of dependent loads per
+// layer. Values the same in every lane are read as such (scalar loads), so
+// the vector stores and atomics of the layers before do not hold them up.
+constexpr uint32_t kLayerGroup = 4;
+template <bool kGrad>
+__device__ __forceinline__ void load_layers(const TileGradArgs &g, uint32_t off, uint32_t k0,
+                                            uint32_t n, uint32_t sub, uint32_t lane,
+                                            uint32_t tt[], LayerIn L[]) {
+    const TileArgs &a = g.f;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    for (uint32_t j = 0; j < kLayerGroup; j++)
+        tt[j] = k0 + j < n ? uniform(g.list_tiles[off + k0 + j]) : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < kLayerGroup; j++)
+        L[j].w = k0 + j < n ? a.tiles[tt[j]] : make_uint2(0xffffffffu, 0u);
+#pragma unroll
+    for (uint32_t j = 0; j < kLayerGroup; j++) {
+        uint32_t color = uniform(L[j].w.y & 0xffffu);
+        int32_t alpha = (int32_t)uniform((uint32_t)sext24(L[j].w.x));
+        int32_t ctrl = (int32_t)((L[j].w.y >> 16) & 0xffu);
+        if (alpha < 0) ctrl &= ~0x3;
+        L[j].base = color < a.paint_count ? a.metadata[(size_t)color * 10 + 2]
+                                          : make_float4(0, 0, 0, 0);
+        L[j].texel = ((ctrl & 0x3) != 0 && (uint32_t)alpha < a.mask_capacity)
+                         ? a.mask[(size_t)(uint32_t)alpha * 64 + lane] : 0u;
+        if (kGrad) {
+            bool alpha_tile = alpha >= 0 && (uint32_t)alpha >= g.alpha_start &&
+                              (uint32_t)alpha - g.alpha_start < g.alpha_count;
+            L[j].clip = alpha_tile
+                            ? (int32_t)g.alpha_tiles[(uint32_t)alpha - g.alpha_start].y : -1;
+            L[j].ctexel = (L[j].clip >= 0 && (uint32_t)L[j].clip < a.mask_capacity)
+                              ? a.mask[(size_t)(uint32_t)L[j].clip * 64 + lane] : 0u;
+            L[j].cstate = (alpha_tile && g.cov_state)
+                              ? g.cov_state[(size_t)(uint32_t)alpha * 64 + lane] : 0u;
+            L[j].C = (alpha_tile && !g.cov_state)
+                         ? g.coverage[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane] : 0.0f;
+            uint32_t bp = k0 + j < n ? g.tile_paths[tt[j]] : 0xffffffffu;
+            L[j].draw_path = bp < g.path_count ? g.dice_md[bp].x : 0xffffffffu;
+            L[j].T = g.trans[(size_t)(off + min(k0 + j, n - 1u)) * 256 + sub * 64 + lane];
+        }
+    }
+}
+
+// The layer's alpha tile and its mask value in sub-row `sub` of this lane's
+// texel, as k_tile computes them.
+__device__ __forceinline__ int32_t layer_mask(const LayerIn &L, uint32_t sub, float &m) {
+    int32_t alpha = sext24(L.w.x);
+    int32_t ctrl = (int32_t)((L.w.y >> 16) & 0xffu);
+    float backdrop;
+    if (alpha >= 0) {
+        backdrop = 0.0f;
+    } else {
+        backdrop = (float)((int32_t)L.w.y >> 24);
+        ctrl &= ~0x3;
+    }
+    int32_t mask_ctrl = ctrl & 0x3;
+    m = 1.0f;
+    if (mask_ctrl != 0) {
+        float cov = (float)((L.texel >> (8 * sub)) & 0xffu) / 255.0f + backdrop;
+        if (mask_ctrl & 0x1) {
+            cov = fabsf(cov);
+        } else {
+            float q = cov - 2.0f * floorf(cov / 2.0f);
+            cov = 1.0f - fabsf(1.0f - q);
+        }
+        m = min_glsl(m, cov);
+    }
+    return alpha;
 }
 
 // The composite of k_tile, D_k = D_(k-1) (1 - a_k) + (rgb_k a_k, a_k) with
@@ -2780,171 +2883,160 @@ __device__ __forceinline__ float wave_sum(float v) {
 //   dL/drgb_k = g_rgb T_k a_k,   dL/dalpha_k += dL/da_k m_k   (per draw path),
 //   dL/dm_k   = dL/da_k alpha_k,
 // and for an alpha tile dL/dC = dL/dm sign(C) when 0 < |C| < 1 and the clip
-// texel does not limit m (0 otherwise). One wavefront per framebuffer tile,
-// lane (lx, ly) owns rows 4ly..4ly+3 of column lx as in k_tile. The list is
-// followed once, storing each slot's tile and a_k; T_k (top down) and the
-// gradients (bottom up, D recomputed) then iterate the slots, whose loads do
-// not depend on each other. Each alpha tile written gets this pass's epoch in
-// grad_stamp, so alpha tiles no list composited read as zero without
-// clearing grad_cov.
+// texel does not limit m (0 otherwise). A workgroup per framebuffer tile, a
+// wavefront per sub-row: lane (lx, ly) of wavefront s owns pixel (lx, 4 ly +
+// s), channel s of mask texel (lx, ly). Wavefront 0 follows the list once,
+// storing each slot's tile; then every wavefront runs T_k (top down, with
+// a_k) and the gradients (bottom up, D recomputed) over the slots
+// kLayerGroup at a time, loading a group's layers together since their
+// loads do not depend on each other. Four wavefronts per tile each carry a
+// quarter of the per-layer work, so a long list (the slowest tile decides
+// the launch) takes a quarter of the time. Each alpha tile written gets this
+// pass's epoch in grad_stamp, so alpha tiles no list composited read as zero
+// without clearing grad_cov. A layer's colour gradient (a quarter per
+// wavefront) goes to its slot, for k_slot_colors.
 __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
+    __shared__ uint32_t s_n;
     const TileArgs &a = g.f;
-    uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t wave = blockIdx.x;  // the framebuffer tile
     uint32_t fbn = (uint32_t)(a.fb_width_tiles * a.fb_height_tiles);
     if (wave >= fbn) return;
-    uint32_t lane = lane_id(), lx = lane & 15u, ly = lane >> 4;
+    uint32_t sub = threadIdx.x >> 6, lane = lane_id(), lx = lane & 15u, ly = lane >> 4;
     uint32_t off = g.list_offsets[wave], end = g.list_offsets[wave + 1];
     if (end > g.slot_capacity) end = g.slot_capacity;
     uint32_t gx = wave % (uint32_t)a.fb_width_tiles, gy = wave / (uint32_t)a.fb_width_tiles;
-    uint32_t px = gx * 16 + lx, py0 = gy * 16 + ly * 4;
-    float4 gp[4];
-#pragma unroll
-    for (int sub = 0; sub < 4; sub++) {
-        uint32_t py = py0 + (uint32_t)sub;
-        gp[sub] = (px < g.width && py < g.height)
-                      ? ((const float4 *)g.grad_image)[(size_t)py * g.width + px]
-                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
+    uint32_t px = gx * 16 + lx, py = gy * 16 + ly * 4 + sub;
+    float4 q = (px < g.width && py < g.height)
+                   ? ((const float4 *)g.grad_image)[(size_t)py * g.width + px]
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-    // the layer's paint, alpha tile and mask value per sub-row, as k_tile reads them
-    auto layer = [&](int32_t t, float4 &base, int32_t &alpha, uint32_t &color, float ma[4]) {
-        uint2 w = a.tiles[t];  // tile words 2-3
-        alpha = sext24(w.x);
-        color = w.y & 0xffffu;
-        int32_t ctrl = (int32_t)((w.y >> 16) & 0xffu);
-        float backdrop;
-        if (alpha >= 0) {
-            backdrop = 0.0f;
-        } else {
-            backdrop = (float)((int32_t)w.y >> 24);
-            ctrl &= ~0x3;
+    // walk 1 (wavefront 0): each slot's tile; one dependent load per layer
+    if (sub == 0) {
+        uint32_t n = 0;
+        int32_t t = (int32_t)uniform((uint32_t)a.first_tile_map[wave]);
+        for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count &&
+                                 off + n < end;
+             guard++) {
+            g.list_tiles[off + n] = (uint32_t)t;
+            n++;
+            t = (int32_t)uniform(a.tile_next[t]);
         }
-        base = color < a.paint_count ? a.metadata[(size_t)color * 10 + 2] : make_float4(0, 0, 0, 0);
-        int32_t mask_ctrl = ctrl & 0x3;
-        uint32_t texel = 0;
-        if (mask_ctrl != 0 && (uint32_t)alpha < a.mask_capacity)
-            texel = a.mask[(size_t)(uint32_t)alpha * 64 + lane];
-#pragma unroll
-        for (int sub = 0; sub < 4; sub++) {
-            float m = 1.0f;
-            if (mask_ctrl != 0) {
-                float cov = (float)((texel >> (8 * sub)) & 0xffu) / 255.0f + backdrop;
-                if (mask_ctrl & 0x1) {
-                    cov = fabsf(cov);
-                } else {
-                    float q = cov - 2.0f * floorf(cov / 2.0f);
-                    cov = 1.0f - fabsf(1.0f - q);
-                }
-                m = min_glsl(m, cov);
-            }
-            ma[sub] = m;
-        }
-        return (int32_t)a.tile_next[t];
-    };
-
-    // walk 1: each slot's tile (stored by every lane, so each lane later reads
-    // back its own store) and a_k
-    uint32_t n = 0;
-    int32_t t = a.first_tile_map[wave];
-    for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count &&
-                             off + n < end;
-         guard++) {
-        float4 base;
-        int32_t alpha;
-        uint32_t color;
-        float ma[4];
-        int32_t next = layer(t, base, alpha, color, ma);
-        g.list_tiles[off + n] = (uint32_t)t;
-        float *slot = g.trans + (size_t)(off + n) * 256 + lane;
-#pragma unroll
-        for (int sub = 0; sub < 4; sub++) slot[sub * 64] = base.w * ma[sub];
-        n++;
-        t = next;
+        // slots the chase did not reach (a truncated list) hold no layer
+        for (uint32_t k = off + n; k < end; k++)
+            if (lane == 4u) g.trans[(size_t)k * 256 + 4] = __uint_as_float(0xffffffffu);
+        if (lane == 0) s_n = n;
     }
-    // walk 2: T_k, top down (each lane reads back only what it wrote)
-    float T[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-#pragma unroll 4
-    for (uint32_t k = n; k-- > 0;) {
-        float *slot = g.trans + (size_t)(off + k) * 256 + lane;
+    __syncthreads();
+    uint32_t n = s_n;
+    float *tr = g.trans + sub * 64 + lane;  // this lane's T_k of slot k at tr[256 (off + k)]
+
+    // walk 2: a_k and T_k, top down (each lane reads back only what it wrote)
+    float T = 1.0f;
+    for (uint32_t kg = n; kg > 0;) {
+        uint32_t k0 = kg >= kLayerGroup ? kg - kLayerGroup : 0u;
+        uint32_t tt[kLayerGroup];
+        LayerIn L[kLayerGroup];
+        load_layers<false>(g, off, k0, kg, sub, lane, tt, L);
 #pragma unroll
-        for (int sub = 0; sub < 4; sub++) {
-            float al = slot[sub * 64];
-            slot[sub * 64] = T[sub];
-            T[sub] *= 1.0f - al;
+        for (uint32_t jj = kLayerGroup; jj-- > 0;) {
+            uint32_t k = k0 + jj;
+            if (k >= kg) continue;
+            float m;
+            (void)layer_mask(L[jj], sub, m);
+            tr[(size_t)(off + k) * 256] = T;
+            T *= 1.0f - L[jj].base.w * m;
         }
+        kg = k0;
     }
     // walk 3: gradients, bottom up
-    float D[4][4];
+    float D[4] = {a.clear[0], a.clear[1], a.clear[2], a.clear[3]};
+    for (uint32_t k0 = 0; k0 < n; k0 += kLayerGroup) {
+        uint32_t tt[kLayerGroup];
+        LayerIn L[kLayerGroup];
+        load_layers<true>(g, off, k0, n, sub, lane, tt, L);
 #pragma unroll
-    for (int sub = 0; sub < 4; sub++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) D[sub][c] = a.clear[c];
-    for (uint32_t k = 0; k < n; k++) {
-        float4 base;
-        int32_t alpha;
-        uint32_t color;
-        float ma[4];
-        t = (int32_t)g.list_tiles[off + k];
-        (void)layer(t, base, alpha, color, ma);
-        const float *slot = g.trans + (size_t)(off + k) * 256 + lane;
-        bool alpha_tile = alpha >= 0 && (uint32_t)alpha >= g.alpha_start &&
-                          (uint32_t)alpha - g.alpha_start < g.alpha_count;
-        int32_t clip = -1;
-        if (alpha_tile) clip = (int32_t)g.alpha_tiles[(uint32_t)alpha - g.alpha_start].y;
-        uint32_t ctexel = 0;
-        if (clip >= 0 && (uint32_t)clip < a.mask_capacity)
-            ctexel = a.mask[(size_t)(uint32_t)clip * 64 + lane];
-        uint32_t cstate = (alpha_tile && g.cov_state)
-                              ? g.cov_state[(size_t)(uint32_t)alpha * 64 + lane] : 0u;
-        float gr = 0.0f, gg = 0.0f, gb = 0.0f, ga = 0.0f;
-#pragma unroll
-        for (int sub = 0; sub < 4; sub++) {
-            float Tk = slot[sub * 64];
-            float al = base.w * ma[sub];
-            float4 q = gp[sub];
-            float dal = Tk * (q.x * (base.x - D[sub][0]) + q.y * (base.y - D[sub][1]) +
-                              q.z * (base.z - D[sub][2]) + q.w * (1.0f - D[sub][3]));
-            gr += q.x * Tk * al;
-            gg += q.y * Tk * al;
-            gb += q.z * Tk * al;
-            ga += dal * ma[sub];
+        for (uint32_t jj = 0; jj < kLayerGroup; jj++) {
+            uint32_t k = k0 + jj;
+            if (k >= n) break;
+            float m;
+            int32_t alpha = layer_mask(L[jj], sub, m);
+            float4 base = L[jj].base;
+            bool alpha_tile = alpha >= 0 && (uint32_t)alpha >= g.alpha_start &&
+                              (uint32_t)alpha - g.alpha_start < g.alpha_count;
+            float Tk = L[jj].T;
+            float al = base.w * m;
+            float dal = Tk * (q.x * (base.x - D[0]) + q.y * (base.y - D[1]) +
+                              q.z * (base.z - D[2]) + q.w * (1.0f - D[3]));
+            float gr = q.x * Tk * al, gg = q.y * Tk * al, gb = q.z * Tk * al;
+            float ga = dal * m;
             if (alpha_tile) {
                 float dC = 0.0f;
                 if (g.cov_state) {
-                    uint32_t st = cstate >> (2 * sub);
+                    uint32_t st = L[jj].cstate >> (2 * sub);
                     if (st & 1u) dC = dal * base.w * ((st & 2u) ? -1.0f : 1.0f);
                 } else {
-                    float C = g.coverage[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane];
+                    float C = L[jj].C;
                     float aC = fabsf(C);
                     bool limited = false;
-                    if (clip >= 0) {
-                        float cv = (float)((ctexel >> (8 * sub)) & 0xffu) / 255.0f;
+                    if (L[jj].clip >= 0) {
+                        float cv = (float)((L[jj].ctexel >> (8 * sub)) & 0xffu) / 255.0f;
                         limited = cv < min(aC, 1.0f);
                     }
                     if (aC > 0.0f && aC < 1.0f && !limited)
                         dC = dal * base.w * (C < 0.0f ? -1.0f : 1.0f);
                 }
                 g.grad_cov[(size_t)(uint32_t)alpha * 256 + sub * 64 + lane] = dC;
+                if (lane == 0) g.grad_stamp[(uint32_t)alpha] = g.epoch;
             }
             float ia = 1.0f - al;
-            D[sub][0] = D[sub][0] * ia + base.x * al;
-            D[sub][1] = D[sub][1] * ia + base.y * al;
-            D[sub][2] = D[sub][2] * ia + base.z * al;
-            D[sub][3] = D[sub][3] * ia + al;
+            D[0] = D[0] * ia + base.x * al;
+            D[1] = D[1] * ia + base.y * al;
+            D[2] = D[2] * ia + base.z * al;
+            D[3] = D[3] * ia + al;
+            gr = wave_sum(gr);
+            gg = wave_sum(gg);
+            gb = wave_sum(gb);
+            ga = wave_sum(ga);
+            // this wavefront's quarter of the layer's colour gradient, and the
+            // draw path (of the tile's path, which k_bound recorded), to words
+            // 0-4 of the slot's sub-row, which walk 3 has read
+            float *out = g.trans + (size_t)(off + k) * 256 + sub * 64;
+            if (lane < 5u)
+                out[lane] = lane == 0u ? gr : lane == 1u ? gg : lane == 2u ? gb
+                          : lane == 3u ? ga : __uint_as_float(L[jj].draw_path);
         }
-        if (alpha_tile) g.grad_stamp[(uint32_t)alpha] = g.epoch;
-        gr = wave_sum(gr);
-        gg = wave_sum(gg);
-        gb = wave_sum(gb);
-        ga = wave_sum(ga);
-        // the tile's path (k_bound recorded it) and that path's draw path id
-        uint32_t bp = g.tile_paths[t];
-        uint32_t draw_path = bp < g.path_count ? g.dice_md[bp].x : 0xffffffffu;
-        if (lane == 0 && draw_path < g.color_count) {
-            atomicAdd(&g.grad_colors[draw_path * 4 + 0], gr);
-            atomicAdd(&g.grad_colors[draw_path * 4 + 1], gg);
-            atomicAdd(&g.grad_colors[draw_path * 4 + 2], gb);
-            atomicAdd(&g.grad_colors[draw_path * 4 + 3], ga);
+    }
+}
+
+// The list slots' colour gradients (k_tile_backward's four sub-row records
+// of each slot) into their draw paths' RGBA in grad_colors, a thread per
+// slot. (Summing them in one workgroup's LDS first was four times slower.)
+__device__ __forceinline__ bool slot_colors(const TileGradArgs &g, uint32_t s, uint32_t &dp,
+                                            float4 &c) {
+    const float *w = g.trans + (size_t)s * 256;
+    dp = __float_as_uint(w[4]);
+    c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t sub = 0; sub < 4; sub++) {
+        c.x += w[sub * 64 + 0];
+        c.y += w[sub * 64 + 1];
+        c.z += w[sub * 64 + 2];
+        c.w += w[sub * 64 + 3];
+    }
+    return dp < g.color_count;
+}
+__global__ void __launch_bounds__(kBlock) k_slot_colors(TileGradArgs g) {
+    uint32_t fbn = (uint32_t)(g.f.fb_width_tiles * g.f.fb_height_tiles);
+    uint32_t total = min(g.list_offsets[fbn], g.slot_capacity);
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < total;
+         s += gridDim.x * blockDim.x) {
+        uint32_t dp;
+        float4 c;
+        if (slot_colors(g, s, dp, c)) {
+            atomicAdd(&g.grad_colors[dp * 4 + 0], c.x);
+            atomicAdd(&g.grad_colors[dp * 4 + 1], c.y);
+            atomicAdd(&g.grad_colors[dp * 4 + 2], c.z);
+            atomicAdd(&g.grad_colors[dp * 4 + 3], c.w);
         }
     }
 }
@@ -3439,7 +3531,10 @@ void launch_scan(uint32_t *data, uint32_t n, uint32_t *partials, hipStream_t s) 
 void launch_tile_backward(const TileGradArgs &a, hipStream_t s) {
     uint32_t fbn = (uint32_t)(a.f.fb_width_tiles * a.f.fb_height_tiles);
     if (fbn == 0) return;
-    hipLaunchKernelGGL(k_tile_backward, dim3((fbn + 3) / 4), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_tile_backward, dim3(fbn), dim3(kBlock), 0, s, a);
+    if (a.slot_capacity && a.color_count)
+        hipLaunchKernelGGL(k_slot_colors, dim3(grid_for(a.slot_capacity, kBlock)),
+                           dim3(kBlock), 0, s, a);
 }
 
 void launch_microline_backward(const MicrolineGradArgs &a, hipStream_t s) {
