@@ -102,3 +102,38 @@ def test_4k_workload_bit_exact(renderer_factory):
     report = compare_frame(r, o, cmds)
     _say(report)
     assert_bit_exact(report)
+
+
+def _luts():
+    """Area LUTs whose saturated cells differ from the shipped one: none at
+    all (every 0 raised to 1, every 255 lowered to 254), and wider ones, not
+    shaped as a prefix and a suffix of each row (bytes below 40 cleared and
+    above 215 set, and a band of rows cleared)."""
+    import oracle
+    base = oracle.area_lut().reshape(256, 256, 4)
+    none = np.clip(base, 1, 254).astype(np.uint8)
+    wide = base.copy()
+    wide[wide < 40] = 0
+    wide[wide > 215] = 255
+    wide[100:110] = 0
+    return {"no saturated cells": none, "wide saturated cells": wide}
+
+
+@pytest.mark.parametrize("which", sorted(_luts()))
+def test_custom_area_lut_bit_exact(which, renderer_factory):
+    """The fill samples the LUT only where its cell is not all 0 or all 255,
+    from a table built from the LUT in use: the frame must equal the oracle's
+    with the same LUT whatever its saturated cells are."""
+    import oracle
+    lut = _luts()[which]
+    scene, size, bg = synth.small_scene(paths=96)
+    _say("case custom LUT", which)
+    from oracle import packing
+    cmds = packing.build(scene, (1, 0, 0, 1, 0, 0))
+    o = oracle.Oracle(lut=lut.reshape(-1), threads=8)
+    o.render(cmds, size[0], size[1], bg)
+    r = renderer_factory(size[0], size[1], bg, area_lut=lut.reshape(-1))
+    product_frame(r, scene)
+    report = compare_frame(r, o, cmds)
+    _say(report)
+    assert_bit_exact(report)
