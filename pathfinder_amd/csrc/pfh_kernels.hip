@@ -2056,7 +2056,12 @@ __device__ __forceinline__ unsigned long long column_lanes(uint32_t pc, uint32_t
     return lane < 16u ? m : 0ull;
 }
 
-constexpr uint32_t kFillGroup = 64;  // alpha tiles per wavefront group
+// Alpha tiles per wavefront group: 64, halved (down to 8) while the groups
+// would number fewer than kFillWaves, so that batches of few alpha tiles with
+// long fill runs (glyph pages) still spread over every SIMD.
+constexpr uint32_t kFillGroup = 64;
+constexpr uint32_t kFillGroupMin = 8;
+constexpr uint32_t kFillWaves = 4096;
 
 // computeCoverage (fill_area.inc.glsl:11-27) of one fill at the pixel column
 // centred at x, for the four row groups of the tile (rows 4g .. 4g + 3, the
@@ -2124,9 +2129,10 @@ __device__ __forceinline__ float4 pair_group(LutRef lut, const PairColumn &c, ui
 // wavefront owns the four pixels of column lx, rows 4ly..4ly+3 of an alpha
 // tile (the four channels of mask texel (lx, ly)).
 //
-// A wavefront takes 64 consecutive alpha tiles and streams their fills: each
-// tile's fills from the highest index down (its run in sorted_fills), tile
-// after tile, 64 at a time. Lane k turns fill k of a chunk into a record, and
+// A wavefront takes a group of consecutive alpha tiles (64, fewer when the
+// batch has few) and streams their fills: each tile's fills from the highest
+// index down (its run in sorted_fills), tile after tile, 64 at a time. Lane k
+// turns fill k of a chunk into a record, and
 // the chunk's (fill, pixel column) pairs are numbered in stream order. Round
 // by round (64 pairs), lane i evaluates computeCoverage of pair i for the
 // four row groups into LDS (pair_column: the LUT is sampled only where its
@@ -2169,12 +2175,16 @@ k_fill(BatchArgs a) {
     uint32_t start = uniform(a.counters->alpha_start);
     uint32_t col = lane & 15u, grp = lane >> 4;
     float fragx = (float)col + 0.5f, fragy = (float)(grp * 4u) + 0.5f;
-    uint32_t stride = ((gridDim.x * blockDim.x) >> 6) * kFillGroup;
-    for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kFillGroup);
+    uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    uint32_t group = kFillGroup;
+    while (group > kFillGroupMin && (n_alpha + group - 1u) / group < min(waves, kFillWaves))
+        group >>= 1;
+    uint32_t stride = waves * group;
+    for (uint32_t g0 = uniform(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * group);
          g0 < n_alpha; g0 += stride) {
-        // lane i: alpha tile g0 + i
+        // lane i < group: alpha tile g0 + i
         uint32_t ba = g0 + lane;
-        bool valid = ba < n_alpha && start + ba < a.mask_capacity;
+        bool valid = lane < group && ba < n_alpha && start + ba < a.mask_capacity;
         uint32_t t_clip = 0xffffffffu, t_w = 0, rx = 0, ry = 0;
         if (valid) {
             uint2 at = a.alpha_tiles[ba];
