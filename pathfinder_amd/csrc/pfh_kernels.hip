@@ -2131,10 +2131,11 @@ __device__ __forceinline__ float4 pair_group(LutRef lut, const PairColumn &c, ui
 //
 // A wavefront takes a group of consecutive alpha tiles (64, fewer when the
 // batch has few) and streams their fills: each tile's fills from the highest
-// index down (its run in sorted_fills), tile after tile, 64 at a time. Lane k
-// turns fill k of a chunk into a record, and
-// the chunk's (fill, pixel column) pairs are numbered in stream order. Round
-// by round (64 pairs), lane i evaluates computeCoverage of pair i for the
+// index down (its run in sorted_fills), tile after tile, in chunks of at most
+// 64. Lane k turns fill k of a chunk into a record, and the chunk's (fill,
+// pixel column) pairs are numbered in stream order; a chunk ends where its
+// pairs fill whole rounds (but for one straddling fill), not after 64 fills.
+// Round by round (64 pairs), lane i evaluates computeCoverage of pair i for the
 // four row groups into LDS (pair_column: the LUT is sampled only where its
 // cell is not saturated), the values stored column by column in pair order.
 // Then, tile by tile through the round, every lane adds its column's values
@@ -2212,9 +2213,8 @@ k_fill(BatchArgs a) {
         uint2 fw;
         bool tlast;
         fill_lookup(a, 0u, total, S, rx, n, lane, ts, fw, tlast);
-        for (uint32_t pos = 0; pos < total; pos += 64u) {
+        for (uint32_t pos = 0, K = 0; pos < total; pos += K) {
             bool have = pos + lane < total;
-            unsigned long long have_m = __ballot(have);
             FillRecord f;
             FillColumns fc = make_record(fw, f);
             // every fill takes at least one pair (a fill that sees no column
@@ -2224,7 +2224,20 @@ k_fill(BatchArgs a) {
             uint32_t span = have ? max(nvis, 1u) : 0u;
             uint32_t pe = wave_inclusive_scan(span);
             uint32_t pb = pe - span;
-            uint32_t pairs = last_lane(pe);
+            // The chunk: all the fills left when they fit in the 64 lanes,
+            // else the first K fills whose pairs end at or before the last
+            // multiple of 64 of the 64 fills' pairs, so that only the round
+            // of a straddling fill is not full (spans are 1..16 pairs, so K
+            // >= 1); the next chunk starts at fill K
+            if (pos + 64u >= total) {
+                K = total - pos;
+            } else {
+                uint32_t whole = last_lane(pe) & ~63u;
+                K = (uint32_t)__popcll(__ballot(pe <= whole));
+            }
+            have = lane < K;
+            unsigned long long have_m = __ballot(have);
+            uint32_t pairs = (uint32_t)__builtin_amdgcn_readlane((int)pe, (int)(K - 1u));
             if (have) {
                 rec[lane] = f;
                 // first column - first pair (low 16 bits, signed), first column
@@ -2239,7 +2252,7 @@ k_fill(BatchArgs a) {
             uint32_t nts;
             uint2 nfw;
             bool ntlast;
-            fill_lookup(a, pos + 64u, total, S, rx, n, lane, nts, nfw, ntlast);
+            fill_lookup(a, pos + K, total, S, rx, n, lane, nts, nfw, ntlast);
             wave_lds_sync();
             for (uint32_t r0 = 0; r0 < pairs; r0 += 64u) {
                 uint32_t r1 = min(r0 + 64u, pairs);
