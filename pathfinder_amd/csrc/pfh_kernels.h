@@ -96,13 +96,6 @@ struct BatchArgs {
     uint32_t *row_counts;          // fb_height_tiles
     uint4 *list_stats;             // per fb tile: linked, survivors, survivor alpha tiles
     uint32_t *scan_partials;
-    // one-launch scans of dice and bin: the count pass stores each block's
-    // total (kBlock items) and adds it to its kScanTile-item tile's partial;
-    // only the partials are scanned; the emit pass takes a block's first
-    // output from its tile's partial and the block totals before it
-    uint32_t *block_sums;          // per block of kBlock items (dice's, then bin's)
-    uint32_t *dice_partials, *bin_partials;
-    uint32_t dice_partial_count, bin_partial_count;  // zeroed by k_batch_init
     BatchCounters *counters;
     FrameState *frame;
     // frame
@@ -260,8 +253,6 @@ void launch_d3d9_tile(const D3D9TileArgs &a, hipStream_t s);
 
 // Scratch words needed by the scans of a batch.
 size_t scan_partials_words(uint32_t max_elements);
-// Block totals of the count passes for that many items.
-size_t block_sum_words(uint32_t max_elements);
 
 }  // namespace pfh
 

@@ -249,29 +249,6 @@ void scan_exclusive(uint32_t *data, const uint32_t *n_ptr, uint32_t n_host, uint
                        cap, partials);
 }
 
-// A count pass's block total: block b (items b * kBlock ..) stores it and
-// adds it to the partial of its tile of kScanTile items (its 4096-item tile
-// of the three-launch scan), which k_scan_partials then scans.
-constexpr uint32_t kBlocksPerTile = kScanTile / kBlock;
-
-__device__ __forceinline__ void publish_block_sum(uint32_t sum, uint32_t b, uint32_t *block_sums,
-                                                  uint32_t *partials) {
-    if (threadIdx.x == 0) {
-        block_sums[b] = sum;
-        if (sum) atomicAdd(&partials[b / kBlocksPerTile], sum);
-    }
-}
-
-// The emit pass's first output of block b: its tile's scanned partial plus
-// the totals of the blocks before it in the tile (at most 15: one load per
-// lane, summed across the wavefront), in every thread.
-__device__ __forceinline__ uint32_t block_base(uint32_t b, const uint32_t *block_sums,
-                                               const uint32_t *partials) {
-    uint32_t first = b / kBlocksPerTile * kBlocksPerTile, lane = lane_id();
-    uint32_t v = first + lane < b ? block_sums[first + lane] : 0u;
-    return partials[b / kBlocksPerTile] + last_lane(wave_inclusive_scan(v));
-}
-
 // ------------------------------------------------------------------ batch init
 
 __global__ void __launch_bounds__(kBlock) k_batch_init(BatchArgs a) {
@@ -285,8 +262,6 @@ __global__ void __launch_bounds__(kBlock) k_batch_init(BatchArgs a) {
     for (uint32_t i = gid; i < fbn; i += stride) a.first_tile_map[i] = -1;
     for (uint32_t i = gid; i < a.column_count * 3; i += stride) a.backdrops[i] = a.backdrops_init[i];
     for (uint32_t i = gid; i <= a.column_count; i += stride) a.col_alpha_offsets[i] = 0;
-    for (uint32_t i = gid; i < a.dice_partial_count; i += stride) a.dice_partials[i] = 0;
-    for (uint32_t i = gid; i < a.bin_partial_count; i += stride) a.bin_partials[i] = 0;
 }
 
 __global__ void k_frame_reset(FrameState *f) {
@@ -440,20 +415,25 @@ __device__ bool dice_setup(const BatchArgs &a, uint32_t seg, uint32_t path, Dice
 __global__ void __launch_bounds__(kBlock) k_dice_count(BatchArgs a) {
     __shared__ uint32_t zs[kBlock];
     __shared__ uint32_t qrange[2];
-    __shared__ uint32_t lds4[4];
     for (uint32_t s0 = blockIdx.x * kBlock; s0 < a.segment_count; s0 += gridDim.x * kBlock) {
-        uint32_t seg = s0 + threadIdx.x, n = 0;
+        uint32_t seg = s0 + threadIdx.x;
         if (a.path_count == 0) {
-            if (seg < a.segment_count) atomicOr(&a.counters->error, 1u);
-        } else {
-            uint32_t path = block_segment_path(a, s0, zs, qrange);
-            DiceSeg d;
             if (seg < a.segment_count) {
-                if (dice_setup(a, seg, path, d)) n = (uint32_t)d.n;
-                else atomicOr(&a.counters->error, 1u);
+                atomicOr(&a.counters->error, 1u);
+                a.seg_offsets[seg] = 0;
             }
+            continue;
         }
-        publish_block_sum(block_reduce_sum(n, lds4), s0 / kBlock, a.block_sums, a.dice_partials);
+        uint32_t path = block_segment_path(a, s0, zs, qrange);
+        if (seg >= a.segment_count) continue;
+        DiceSeg d;
+        uint32_t n = 0;
+        if (dice_setup(a, seg, path, d)) {
+            n = (uint32_t)d.n;
+        } else {
+            atomicOr(&a.counters->error, 1u);
+        }
+        a.seg_offsets[seg] = n;
     }
 }
 
@@ -504,26 +484,16 @@ __global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
     __shared__ uint32_t s_first[kBlock];           // block-relative first microline
     __shared__ uint32_t s_n[kBlock];               // n | curve << 31
     __shared__ uint32_t s_path[kBlock];
-    __shared__ uint32_t lds4[4];
-    if (a.path_count == 0) {
-        // no segment has a path: every segment counted 0 microlines
-        for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a.segment_count;
-             i += gridDim.x * kBlock)
-            a.seg_offsets[i] = 0;
-        return;
-    }
+    if (a.path_count == 0) return;
     for (uint32_t s0 = blockIdx.x * kBlock; s0 < a.segment_count; s0 += gridDim.x * kBlock) {
         uint32_t seg = s0 + threadIdx.x;
         uint32_t path = block_segment_path(a, s0, zs, qrange);
         uint32_t s_end = min(s0 + kBlock, a.segment_count);
+        uint32_t lo = a.seg_offsets[s0], hi = a.seg_offsets[s_end];
         DiceSeg d;
         bool ok = seg < a.segment_count && dice_setup(a, seg, path, d);
         // a segment that failed the setup counted 0 microlines: an empty range
-        uint32_t total;
-        uint32_t first = block_exclusive_scan(ok ? (uint32_t)d.n : 0u, lds4, &total);
-        uint32_t lo = block_base(s0 / kBlock, a.block_sums, a.dice_partials);
-        if (seg < a.segment_count) a.seg_offsets[seg] = lo + first;
-        s_first[threadIdx.x] = first;
+        s_first[threadIdx.x] = (seg < a.segment_count ? a.seg_offsets[seg] : hi) - lo;
         s_n[threadIdx.x] = ok ? ((uint32_t)d.n | (d.curve ? 0x80000000u : 0u)) : 0u;
         s_pc[threadIdx.x] = ok ? make_float4(d.p0.x, d.p0.y, d.c0.x, d.c0.y)
                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -531,7 +501,7 @@ __global__ void __launch_bounds__(kBlock) k_dice_emit(BatchArgs a) {
                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         s_path[threadIdx.x] = ok ? d.path : 0u;
         __syncthreads();
-        uint32_t last = s_end - s0 - 1;
+        uint32_t total = hi - lo, last = s_end - s0 - 1;
         for (uint32_t m = threadIdx.x; m < total; m += kBlock) {
             // the last segment of the block starting at or before m (an empty
             // segment starts where the next one does, so this is never one)
@@ -774,35 +744,18 @@ __device__ uint32_t bin_microline(const BatchArgs &a, uint4 ml, uint32_t fill_ba
     return count;
 }
 
-// The count pass leaves each microline's fill count in ml_fill_offsets and
-// the block totals; the emit pass turns the counts into offsets (block scan
-// from the block's first fill) and writes the fills.
 __global__ void __launch_bounds__(kBlock) k_bin_count(BatchArgs a) {
-    __shared__ uint32_t lds4[4];
     uint32_t n = a.counters->microline_count;
-    for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
-        uint32_t m = b0 + threadIdx.x, c = 0;
-        if (m < n) {
-            c = bin_microline<false>(a, a.microlines[m], 0);
-            a.ml_fill_offsets[m] = c;
-        }
-        publish_block_sum(block_reduce_sum(c, lds4), b0 / kBlock, a.block_sums, a.bin_partials);
-    }
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n;
+         m += gridDim.x * blockDim.x)
+        a.ml_fill_offsets[m] = bin_microline<false>(a, a.microlines[m], 0);
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin_emit(BatchArgs a) {
-    __shared__ uint32_t lds4[4];
     uint32_t n = a.counters->microline_count;
-    for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
-        uint32_t m = b0 + threadIdx.x;
-        uint32_t c = m < n ? a.ml_fill_offsets[m] : 0u, total;
-        uint32_t f0 = block_base(b0 / kBlock, a.block_sums, a.bin_partials) +
-                      block_exclusive_scan(c, lds4, &total);
-        if (m < n) {
-            a.ml_fill_offsets[m] = f0;
-            bin_microline<true>(a, a.microlines[m], f0);
-        }
-    }
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < n;
+         m += gridDim.x * blockDim.x)
+        bin_microline<true>(a, a.microlines[m], a.ml_fill_offsets[m]);
 }
 
 // First fill of each path (path_count + 1 entries, the last one the fill
@@ -3458,10 +3411,6 @@ size_t scan_partials_words(uint32_t max_elements) {
     return (size_t)(max_elements + kScanTile - 1) / kScanTile + 1;
 }
 
-size_t block_sum_words(uint32_t max_elements) {
-    return (size_t)(max_elements + kBlock - 1) / kBlock + 1;
-}
-
 void launch_frame_reset(FrameState *frame, hipStream_t s) {
     hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(64), 0, s, frame);
 }
@@ -3469,8 +3418,6 @@ void launch_frame_reset(FrameState *frame, hipStream_t s) {
 void launch_batch_init(const BatchArgs &a, hipStream_t s, Profiler *p) {
     uint64_t n = (uint64_t)a.fb_width_tiles * a.fb_height_tiles + 8;
     if ((uint64_t)a.column_count * 3 + 1 > n) n = (uint64_t)a.column_count * 3 + 1;
-    if (a.bin_partial_count > n) n = a.bin_partial_count;
-    if (a.dice_partial_count > n) n = a.dice_partial_count;
     Scope t(p, s, K_BATCH_INIT);
     hipLaunchKernelGGL(k_batch_init, dim3(grid_for(n, kBlock, a.grid_cap)), dim3(kBlock), 0, s, a);
 }
@@ -3482,13 +3429,11 @@ void launch_dice(const BatchArgs &a, hipStream_t s, Profiler *p) {
         hipLaunchKernelGGL(k_dice_count, dim3(g), dim3(kBlock), 0, s, a);
     }
     {
-        // the tiles' partials only; k_dice_emit adds the rest
         Scope t(p, s, K_DICE_SCAN);
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, a.dice_partials,
-                           nullptr, a.segment_count, a.segment_count, &a.counters->microlines,
-                           a.seg_offsets,
-                           ScanCount{&a.counters->microline_count, a.microline_capacity,
-                                     &a.counters->overflow, 1u});
+        scan_exclusive(a.seg_offsets, nullptr, a.segment_count, a.segment_count, a.scan_partials,
+                       &a.counters->microlines, s,
+                       ScanCount{&a.counters->microline_count, a.microline_capacity,
+                                 &a.counters->overflow, 1u});
     }
     Scope t(p, s, K_DICE_EMIT);
     hipLaunchKernelGGL(k_dice_emit, dim3(g), dim3(kBlock), 0, s, a);
@@ -3508,13 +3453,10 @@ void launch_bin_count(const BatchArgs &a, hipStream_t s, Profiler *p) {
         Scope t(p, s, K_BIN_COUNT);
         hipLaunchKernelGGL(k_bin_count, dim3(g), dim3(kBlock), 0, s, a);
     }
-    // the tiles' partials only; k_bin_emit adds the rest
     Scope t(p, s, K_BIN_SCAN);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, a.bin_partials,
-                       &a.counters->microline_count, 0u, a.microline_capacity,
-                       &a.counters->fills, a.ml_fill_offsets,
-                       ScanCount{&a.counters->fill_count, a.fill_capacity, &a.counters->overflow,
-                                 2u});
+    scan_exclusive(a.ml_fill_offsets, &a.counters->microline_count, 0, a.microline_capacity,
+                   a.scan_partials, &a.counters->fills, s,
+                   ScanCount{&a.counters->fill_count, a.fill_capacity, &a.counters->overflow, 2u});
 }
 
 void launch_bin_emit(const BatchArgs &a, hipStream_t s, Profiler *p) {

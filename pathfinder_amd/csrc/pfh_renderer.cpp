@@ -300,7 +300,6 @@ struct Batch {
     uint32_t *d_row_counts = nullptr;
     uint4 *d_list_stats = nullptr;
     uint32_t *d_scan = nullptr;
-    uint32_t *d_block_sums = nullptr;  // block totals, then dice's and bin's tile partials
     pfh::BatchCounters *d_counters = nullptr;
     uint32_t cap_m = 0, cap_f = 0;
     pfh::BatchCounters counters = {};
@@ -469,7 +468,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_tile_status);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_alpha_words); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
     dfree(ctx, b.d_row_paths); dfree(ctx, b.d_row_counts); dfree(ctx, b.d_list_stats);
-    dfree(ctx, b.d_scan); dfree(ctx, b.d_block_sums); dfree(ctx, b.d_counters);
+    dfree(ctx, b.d_scan); dfree(ctx, b.d_counters);
 }
 
 void free_batches(pfh_context *ctx) {
@@ -494,7 +493,7 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_microlines); dfree(ctx, b.d_ml_fill_offsets);
     dfree(ctx, b.d_fill_words); dfree(ctx, b.d_fill_links); dfree(ctx, b.d_fill_tiles);
     dfree(ctx, b.d_sorted_fills);
-    dfree(ctx, b.d_run_prev); dfree(ctx, b.d_scan); dfree(ctx, b.d_block_sums);
+    dfree(ctx, b.d_run_prev); dfree(ctx, b.d_scan);
     b.cap_m = ctx->cap_m;
     b.cap_f = ctx->cap_f;
     pfh_status s;
@@ -507,10 +506,6 @@ pfh_status size_capacity_buffers(pfh_context *ctx, Batch &b) {
     if ((s = dalloc(ctx, &b.d_run_prev, b.cap_f)) != PFH_OK) return s;
     uint32_t scan_n = std::max({b.host.segment_count, b.cap_m, (uint32_t)b.host.backdrops.size()});
     if ((s = dalloc(ctx, &b.d_scan, pfh::scan_partials_words(scan_n))) != PFH_OK) return s;
-    size_t sums = pfh::block_sum_words(std::max(b.host.segment_count, b.cap_m)) +
-                  pfh::scan_partials_words(b.host.segment_count) +
-                  pfh::scan_partials_words(b.cap_m);
-    if ((s = dalloc(ctx, &b.d_block_sums, sums)) != PFH_OK) return s;
     return PFH_OK;
 }
 
@@ -615,13 +610,6 @@ pfh::BatchArgs batch_args(pfh_context *ctx, Batch &b) {
     a.row_counts = b.d_row_counts;
     a.list_stats = b.d_list_stats;
     a.scan_partials = b.d_scan;
-    a.block_sums = b.d_block_sums;
-    a.dice_partial_count = (uint32_t)pfh::scan_partials_words(b.host.segment_count);
-    a.bin_partial_count = (uint32_t)pfh::scan_partials_words(b.cap_m);
-    a.dice_partials = b.d_block_sums
-                          ? b.d_block_sums + pfh::block_sum_words(std::max(b.host.segment_count, b.cap_m))
-                          : nullptr;
-    a.bin_partials = a.dice_partials ? a.dice_partials + a.dice_partial_count : nullptr;
     a.counters = b.d_counters;
     a.frame = ctx->d_frame;
     a.mask = ctx->d_mask;
