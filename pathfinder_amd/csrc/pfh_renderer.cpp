@@ -749,12 +749,14 @@ pfh_status size_all(pfh_context *ctx) {
 
 // The frame's launches for batches whose buffers are sized (capturable).
 pfh_status enqueue_frame(pfh_context *ctx) {
+    // the side streams start as soon as the previous frame is done, beside
+    // the frame state reset (only propagate, on the context stream, reads it)
+    pfh_status s = start_frame_on_side_streams(ctx);
+    if (s != PFH_OK) return s;
     pfh::launch_frame_reset(ctx->d_frame, ctx->stream);
     if (ctx->d_fb_saved)
         HIP_TRY(ctx, hipMemcpyAsync(ctx->d_fb, ctx->d_fb_saved, ctx->fb_words * 4,
                                     hipMemcpyDeviceToDevice, ctx->stream));
-    pfh_status s = start_frame_on_side_streams(ctx);
-    if (s != PFH_OK) return s;
     if (ctx->prof.mode == PFH_PROFILE_SERIAL) {
         // every stage of every batch in order on the context stream, so that
         // each kernel's events time it alone
