@@ -2056,10 +2056,11 @@ __device__ __forceinline__ unsigned long long column_lanes(uint32_t pc, uint32_t
     return lane < 16u ? m : 0ull;
 }
 
-// Alpha tiles per wavefront group: 64, halved (down to 8) while the groups
-// would number fewer than kFillWaves, so that batches of few alpha tiles with
-// long fill runs (glyph pages) still spread over every SIMD.
-constexpr uint32_t kFillGroup = 64;
+// Alpha tiles per wavefront group: 32 (64 made the 4K fill 1.5 % slower: a
+// group is the unit of work balance across waves), halved (down to 8) while
+// the groups would number fewer than kFillWaves, so that batches of few
+// alpha tiles with long fill runs (glyph pages) still spread over every SIMD.
+constexpr uint32_t kFillGroup = 32;
 constexpr uint32_t kFillGroupMin = 8;
 constexpr uint32_t kFillWaves = 4096;
 
@@ -2129,7 +2130,7 @@ __device__ __forceinline__ float4 pair_group(LutRef lut, const PairColumn &c, ui
 // wavefront owns the four pixels of column lx, rows 4ly..4ly+3 of an alpha
 // tile (the four channels of mask texel (lx, ly)).
 //
-// A wavefront takes a group of consecutive alpha tiles (64, fewer when the
+// A wavefront takes a group of consecutive alpha tiles (32, fewer when the
 // batch has few) and streams their fills: each tile's fills from the highest
 // index down (its run in sorted_fills), tile after tile, in chunks of at most
 // 64. Lane k turns fill k of a chunk into a record, and the chunk's (fill,
