@@ -136,6 +136,13 @@ struct TileArgs {
     int32_t load;                  // 1: LOAD_ACTION_LOAD, 0: clear
     float clear[4];
     int32_t plain_paints;          // every paint's control word is zero (solid src-over)
+    // with gradients on: the tile of every list slot, recorded as the
+    // composite walks the lists (slot offsets: exclusive scan of the lists'
+    // lengths), so that the adjoint reads them instead of walking again;
+    // slots a walk does not reach hold ~0. Null otherwise.
+    uint32_t *list_tiles;
+    const uint32_t *list_offsets;  // fb tiles + 1
+    uint32_t list_capacity;
 };
 
 // The adjoint of composite (k_tile) for one draw batch drawn over a clear
@@ -152,6 +159,7 @@ struct TileGradArgs {
     const uint32_t *list_offsets;  // per fb tile: first slot of its list (exclusive scan)
     uint32_t slot_capacity;
     uint32_t *list_tiles;          // scratch: the tile of each list slot
+    int32_t lists_recorded;        // list_tiles holds the forward composite's slots
     float *trans;                  // scratch: 256 floats per list slot
     const uint32_t *tile_paths;    // the batch path of every tile (k_bound)
     const uint4 *dice_md;          // the batch's paths: draw path id in .x

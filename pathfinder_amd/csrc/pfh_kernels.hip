@@ -2703,7 +2703,17 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
         w = a.tiles[t];
         w_next = a.tile_next[t];
     }
+    // the slots of this list, for the adjoint
+    uint32_t slot = 0, slot_end = 0;
+    if (a.list_tiles) {
+        slot = min(a.list_offsets[wave], a.list_capacity);
+        slot_end = min(a.list_offsets[wave + 1], a.list_capacity);
+    }
     for (uint32_t guard = 0; valid && guard < a.tile_count; guard++) {
+        if (slot < slot_end) {
+            if (lane == 0) a.list_tiles[slot] = (uint32_t)t;
+            slot++;
+        }
         int32_t tn = (int32_t)w_next;
         bool next_valid = tn >= 0 && (uint32_t)tn < a.tile_count;
         uint32_t tc = next_valid ? (uint32_t)tn : 0u;
@@ -2762,7 +2772,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(TileArgs a) {
         w = wn;
         w_next = wn_next;
         valid = next_valid;
+        t = tn;
     }
+    for (; slot < slot_end; slot++)  // a truncated list: no layer
+        if (lane == 0) a.list_tiles[slot] = 0xffffffffu;
 #pragma unroll
     for (int sub = 0; sub < 4; sub++)
         px[sub * pitch] = unorm8(d[sub][0]) | (unorm8(d[sub][1]) << 8) |
@@ -2934,16 +2947,32 @@ __global__ void __launch_bounds__(kBlock) k_tile_backward(TileGradArgs g) {
                    ? ((const float4 *)g.grad_image)[(size_t)py * g.width + px]
                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-    // walk 1 (wavefront 0): each slot's tile; one dependent load per layer
+    // walk 1 (wavefront 0): each slot's tile; one dependent load per layer,
+    // unless the forward composite recorded them (then 64 slots per load,
+    // up to the first that holds no layer)
     if (sub == 0) {
         uint32_t n = 0;
-        int32_t t = (int32_t)uniform((uint32_t)a.first_tile_map[wave]);
-        for (uint32_t guard = 0; t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count &&
-                                 off + n < end;
-             guard++) {
-            g.list_tiles[off + n] = (uint32_t)t;
-            n++;
-            t = (int32_t)uniform(a.tile_next[t]);
+        if (g.lists_recorded) {
+            for (uint32_t k0 = off; k0 < end; k0 += 64u) {
+                uint32_t k = k0 + lane;
+                uint32_t t = k < end ? g.list_tiles[k] : 0xffffffffu;
+                unsigned long long none = __ballot(t >= a.tile_count);
+                if (none) {
+                    n = k0 - off + (uint32_t)__ffsll((long long)none) - 1u;
+                    break;
+                }
+                n = k0 + 64u - off;
+            }
+            n = min(n, end - off);
+        } else {
+            int32_t t = (int32_t)uniform((uint32_t)a.first_tile_map[wave]);
+            for (uint32_t guard = 0;
+                 t >= 0 && (uint32_t)t < a.tile_count && guard < a.tile_count && off + n < end;
+                 guard++) {
+                g.list_tiles[off + n] = (uint32_t)t;
+                n++;
+                t = (int32_t)uniform(a.tile_next[t]);
+            }
         }
         // slots the chase did not reach (a truncated list) hold no layer
         for (uint32_t k = off + n; k < end; k++)

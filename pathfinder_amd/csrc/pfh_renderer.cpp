@@ -288,6 +288,9 @@ struct Batch {
     uint32_t *d_tile_next = nullptr;   // tile word 0 (linked tiles)
     uint32_t *d_tile_first = nullptr;  // tile word 1 (tiles with fills)
     uint32_t *d_tile_paths = nullptr;  // with gradients: the batch path of every tile
+    // with gradients, draw batches: the composite's record of every list slot's
+    // tile, at the offsets of an exclusive scan of the lists' lengths
+    uint32_t *d_slot_off = nullptr, *d_slot_scan = nullptr, *d_slot_tiles = nullptr;
     uint8_t *d_tile_flags = nullptr;   // 1 per tile bin gave a fill
     uint8_t *d_tile_status = nullptr;  // per tile: linked / solid / alpha bits
     uint32_t *d_tile_delta = nullptr;  // bin's backdrop deltas per tile
@@ -464,6 +467,7 @@ void free_batch(pfh_context *ctx, Batch &b) {
     dfree(ctx, b.d_tile_runs);
     dfree(ctx, b.d_tiles); dfree(ctx, b.d_tile_next); dfree(ctx, b.d_tile_first);
     dfree(ctx, b.d_tile_paths); dfree(ctx, b.d_col_alpha);
+    dfree(ctx, b.d_slot_off); dfree(ctx, b.d_slot_scan); dfree(ctx, b.d_slot_tiles);
     dfree(ctx, b.d_tile_flags); dfree(ctx, b.d_tile_delta);
     dfree(ctx, b.d_tile_status);
     dfree(ctx, b.d_alpha); dfree(ctx, b.d_alpha_words); dfree(ctx, b.d_zbuffer); dfree(ctx, b.d_ftm);
@@ -650,6 +654,10 @@ pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
     t.metadata = ctx->d_metadata;
     t.paint_count = ctx->paint_count;
     t.plain_paints = ctx->paints_plain ? 1 : 0;
+    bool slots = ctx->gradients && b.d_slot_tiles;
+    t.list_tiles = slots ? b.d_slot_tiles : nullptr;
+    t.list_offsets = slots ? b.d_slot_off : nullptr;
+    t.list_capacity = slots ? b.host.tile_count + 1 : 0;
     t.color_texture = nullptr;
     t.color_width = t.color_height = 0;
     t.color_flags = b.color_flags;
@@ -667,6 +675,16 @@ pfh::TileArgs tile_args(pfh_context *ctx, const Batch &b) {
     t.load = b.load;
     for (int i = 0; i < 4; i++) t.clear[i] = b.clear[i];
     return t;
+}
+
+// With gradients, after a draw batch's sort: the slot offsets of its lists
+// (exclusive scan of their lengths), where the composite records each
+// slot's tile for the adjoint.
+void list_slots(pfh_context *ctx, Batch &b, hipStream_t s) {
+    if (!ctx->gradients || !b.d_slot_off) return;
+    uint32_t fbn = (uint32_t)(ctx->fbw * ctx->fbh);
+    pfh::launch_list_lengths(b.d_list_stats, b.d_slot_off, fbn, s);
+    pfh::launch_scan(b.d_slot_off, fbn, b.d_slot_scan, s);
 }
 
 pfh_status run_front(pfh_context *ctx, Batch &b, hipStream_t front, bool bound_launched) {
@@ -697,6 +715,7 @@ pfh_status run_back(pfh_context *ctx, Batch &b, hipStream_t front) {
     pfh::launch_row_paths(a, front, p);
     HIP_TRY(ctx, hipStreamWaitEvent(front, b.prop_done, 0));
     pfh::launch_sort(a, front, p);
+    list_slots(ctx, b, front);
     HIP_TRY(ctx, hipEventRecord(b.sort_done, front));
     HIP_TRY(ctx, hipStreamWaitEvent(s, b.link_done, 0));
     pfh::launch_fill(a, s, p);
@@ -817,6 +836,7 @@ pfh_status enqueue_frame(pfh_context *ctx) {
         HIP_TRY(ctx, hipStreamWaitEvent(side, ctx->batches[n - 1]->prop_done, 0));
         pfh::launch_row_paths(a, side, p);
         pfh::launch_sort(a, side, p);
+        list_slots(ctx, b, side);
         HIP_TRY(ctx, hipEventRecord(b.sort_done, side));
     }
     if (n) {
@@ -939,6 +959,13 @@ pfh_status add_batch(pfh_context *ctx, const pfh_tile_batch *in, bool draw) {
     if ((s = dalloc(ctx, &b.d_tile_status, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_tile_delta, h.tile_count)) != PFH_OK) return s;
     if (ctx->gradients && (s = dalloc(ctx, &b.d_tile_paths, h.tile_count)) != PFH_OK) return s;
+    if (ctx->gradients && draw) {
+        uint32_t fbn_slots = (uint32_t)(ctx->fbw * ctx->fbh);
+        if ((s = dalloc(ctx, &b.d_slot_off, (size_t)fbn_slots + 1)) != PFH_OK) return s;
+        if ((s = dalloc(ctx, &b.d_slot_scan, pfh::scan_partials_words(fbn_slots))) != PFH_OK)
+            return s;
+        if ((s = dalloc(ctx, &b.d_slot_tiles, (size_t)h.tile_count + 1)) != PFH_OK) return s;
+    }
     if ((s = dalloc(ctx, &b.d_tile_runs, h.tile_count)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_col_alpha, h.backdrops.size() + 1)) != PFH_OK) return s;
     if ((s = dalloc(ctx, &b.d_alpha, h.tile_count)) != PFH_OK) return s;
@@ -1046,7 +1073,10 @@ pfh_status backward_launch(pfh_context *ctx, Batch &b, const float *grad_image, 
     tg.list_offsets = sc.list_off;
     tg.slot_capacity = slots;
     tg.trans = sc.trans;
-    tg.list_tiles = sc.list_tiles;
+    // the forward composite recorded every slot's tile (same offsets: the
+    // same scan of the same lengths)
+    tg.lists_recorded = b.d_slot_tiles ? 1 : 0;
+    tg.list_tiles = b.d_slot_tiles ? b.d_slot_tiles : sc.list_tiles;
     tg.grad_stamp = sc.grad_stamp;
     tg.epoch = sc.epoch;
     tg.tile_paths = b.d_tile_paths;
