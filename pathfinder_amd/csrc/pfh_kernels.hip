@@ -3152,23 +3152,23 @@ __device__ float2 microline_pixel_weights(const MicrolineGradArgs &g, uint32_t p
     return make_float2(w0, w1);
 }
 
-__device__ __forceinline__ void add_point_grad(const MicrolineGradArgs &g, uint32_t index,
-                                               float2 gq) {
-    // dL/dp = M^T dL/dq for q = M p + t (get_point)
-    const float *m = g.b.xform;
-    if (index >= g.b.scene.point_count) return;
-    atomicAdd(&g.grad_points[2 * index + 0], m[0] * gq.x + m[2] * gq.y);
-    atomicAdd(&g.grad_points[2 * index + 1], m[1] * gq.x + m[3] * gq.y);
-}
 
 // One microline per thread: its segment (the last with seg_offsets <= m), the
 // unquantised end points as k_dice_emit computes them, the pixel weights, and
 // the chain rule through de Casteljau (Bernstein weights at t = i / n and
 // (i + 1) / n), the quadratic-to-cubic elevation and the transform.
-__global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs g) {
+// The gradients of microline m's end points carried to up to four scene
+// points (index ~0: none), for k_microline_backward to add.
+struct PointGrads {
+    uint32_t i0, i1, i2, i3;  // scene point indices, ~0: none
+    float2 g0, g1, g2, g3;
+};
+
+__device__ __forceinline__ PointGrads microline_point_grads(const MicrolineGradArgs &g,
+                                                            uint32_t m) {
     const BatchArgs &a = g.b;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < g.microline_count; m += stride) {
+    PointGrads pgr = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, {}, {}, {}, {}};
+    {
         uint32_t lo = 0, hi = a.segment_count + 1;  // seg_offsets[lo] <= m < seg_offsets[hi]
         while (lo + 1 < hi) {
             uint32_t mid = (lo + hi) >> 1;
@@ -3176,13 +3176,13 @@ __global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs
             else hi = mid;
         }
         uint32_t seg = lo;
-        if (seg >= a.segment_count) continue;
+        if (seg >= a.segment_count) return pgr;
         uint32_t path = a.microlines[m].w;
-        if (path >= a.path_count) continue;
+        if (path >= a.path_count) return pgr;
         DiceSeg d;
-        if (!dice_setup(a, seg, path, d)) continue;
+        if (!dice_setup(a, seg, path, d)) return pgr;
         int32_t i = (int32_t)(m - a.seg_offsets[seg]);
-        if (i >= d.n) continue;
+        if (i >= d.n) return pgr;
         auto point_at = [&](int32_t k) {
             if (k == 0) return d.p0;
             float t = (float)k / (float)d.n;
@@ -3198,7 +3198,7 @@ __global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs
         };
         float2 q0 = point_at(i), q1 = point_at(i + 1);
         float2 w = microline_pixel_weights(g, path, q0.x, q0.y, q1.x, q1.y);
-        if (w.x == 0.0f && w.y == 0.0f) continue;
+        if (w.x == 0.0f && w.y == 0.0f) return pgr;
         float2 nrm = make_float2(-(q1.y - q0.y), q1.x - q0.x);
         float2 g0 = make_float2(nrm.x * w.x, nrm.y * w.x);  // dL/dq0
         float2 g1 = make_float2(nrm.x * w.y, nrm.y * w.y);  // dL/dq1
@@ -3218,22 +3218,69 @@ __global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs
             if (flags & kFlagQuad) {
                 // c0 = (p0 + 2k) / 3, c1 = (p3 + 2k) / 3
                 const float third = 1.0f / 3.0f, two_thirds = 2.0f / 3.0f;
-                add_point_grad(g, from, make_float2(gc[0].x + gc[1].x * third,
-                                                    gc[0].y + gc[1].y * third));
-                add_point_grad(g, from + 1, make_float2((gc[1].x + gc[2].x) * two_thirds,
-                                                        (gc[1].y + gc[2].y) * two_thirds));
-                add_point_grad(g, from + 2, make_float2(gc[3].x + gc[2].x * third,
-                                                        gc[3].y + gc[2].y * third));
+                pgr.i0 = from;
+                pgr.g0 = make_float2(gc[0].x + gc[1].x * third, gc[0].y + gc[1].y * third);
+                pgr.i1 = from + 1;
+                pgr.g1 = make_float2((gc[1].x + gc[2].x) * two_thirds,
+                                     (gc[1].y + gc[2].y) * two_thirds);
+                pgr.i2 = from + 2;
+                pgr.g2 = make_float2(gc[3].x + gc[2].x * third, gc[3].y + gc[2].y * third);
             } else {
-#pragma unroll
-                for (int j = 0; j < 4; j++) add_point_grad(g, from + (uint32_t)j, gc[j]);
+                pgr.i0 = from;
+                pgr.g0 = gc[0];
+                pgr.i1 = from + 1;
+                pgr.g1 = gc[1];
+                pgr.i2 = from + 2;
+                pgr.g2 = gc[2];
+                pgr.i3 = from + 3;
+                pgr.g3 = gc[3];
             }
         } else {
-            add_point_grad(g, from, make_float2((1.0f - ta) * g0.x + (1.0f - tb) * g1.x,
-                                                (1.0f - ta) * g0.y + (1.0f - tb) * g1.y));
-            add_point_grad(g, from + 1,
-                           make_float2(ta * g0.x + tb * g1.x, ta * g0.y + tb * g1.y));
+            pgr.i0 = from;
+            pgr.g0 = make_float2((1.0f - ta) * g0.x + (1.0f - tb) * g1.x,
+                                 (1.0f - ta) * g0.y + (1.0f - tb) * g1.y);
+            pgr.i1 = from + 1;
+            pgr.g1 = make_float2(ta * g0.x + tb * g1.x, ta * g0.y + tb * g1.y);
         }
+    }
+    return pgr;
+}
+
+// One microline per thread. The microlines of a segment are consecutive and
+// send their gradients to the same few points, so each wavefront first sums
+// the contributions it has for one point (a ballot per distinct point) and
+// adds that sum with one atomic per coordinate, instead of one per lane:
+// far fewer atomics on the same addresses. (Float sums in another order: the
+// gradient is compared at a tolerance, not bit for bit.)
+__global__ void __launch_bounds__(kBlock) k_microline_backward(MicrolineGradArgs g) {
+    uint32_t stride = gridDim.x * blockDim.x, lane = lane_id();
+    const float *xf = g.b.xform;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < g.microline_count; base += stride) {
+        uint32_t m = base + threadIdx.x;
+        PointGrads pgr = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu, {}, {}, {}, {}};
+        if (m < g.microline_count) pgr = microline_point_grads(g, m);
+        auto add = [&](uint32_t index, float2 gq) {
+            // dL/dp = M^T dL/dq for q = M p + t (get_point)
+            bool valid = index < g.b.scene.point_count;
+            float gx = valid ? xf[0] * gq.x + xf[2] * gq.y : 0.0f;
+            float gy = valid ? xf[1] * gq.x + xf[3] * gq.y : 0.0f;
+            unsigned long long left = __ballot(valid);
+            while (left) {
+                uint32_t leader = (uint32_t)__ffsll((long long)left) - 1u;
+                uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)index, (int)leader);
+                bool mine = valid && index == key;
+                left &= ~__ballot(mine);
+                float sx = wave_sum(mine ? gx : 0.0f), sy = wave_sum(mine ? gy : 0.0f);
+                if (lane == leader) {
+                    atomicAdd(&g.grad_points[2 * key + 0], sx);
+                    atomicAdd(&g.grad_points[2 * key + 1], sy);
+                }
+            }
+        };
+        add(pgr.i0, pgr.g0);
+        add(pgr.i1, pgr.g1);
+        add(pgr.i2, pgr.g2);
+        add(pgr.i3, pgr.g3);
     }
 }
 
