@@ -2136,7 +2136,9 @@ __device__ __forceinline__ float4 pair_group(LutRef lut, const PairColumn &c, ui
 // 64. Lane k turns fill k of a chunk into a record, and the chunk's (fill,
 // pixel column) pairs are numbered in stream order; a chunk ends where its
 // pairs fill whole rounds (but for one straddling fill), not after 64 fills.
-// Round by round (64 pairs), lane i evaluates computeCoverage of pair i for the
+// Each fill sets the bit of its first pair in a 1024-bit start map, so pair
+// p's fill is the number of set bits at or before p, less one (one LDS OR per
+// fill instead of a byte per pair). Round by round (64 pairs), lane i evaluates computeCoverage of pair i for the
 // four row groups into LDS (pair_column: the LUT is sampled only where its
 // cell is not saturated), the values stored column by column in pair order.
 // Then, tile by tile through the round, every lane adds its column's values
@@ -2156,7 +2158,7 @@ k_fill(BatchArgs a) {
     // values: column slot s (0..63) of row group g at s + 65 g; then +0
     __shared__ float4 s_res[kBlock / 64][kZeroEntry + 1];
     __shared__ uint32_t s_cadj[kBlock / 64][64];
-    __shared__ uint8_t s_pair[kBlock / 64][64 * 16];
+    __shared__ unsigned long long s_pstart[kBlock / 64][16];
     __shared__ unsigned long long s_cmask[kBlock / 64][16];
     __shared__ uint8_t s_cstart[kBlock / 64][16];
     __shared__ uint8_t s_quad[kBlock / 64][4 * 64];
@@ -2167,7 +2169,7 @@ k_fill(BatchArgs a) {
     FillRecord *rec = s_rec[wv];
     float4 *res = s_res[wv];
     uint32_t *cadj = s_cadj[wv];
-    uint8_t *pair_fill = s_pair[wv];
+    unsigned long long *pstart = s_pstart[wv];
     unsigned long long *cmask = s_cmask[wv];
     uint8_t *cstart = s_cstart[wv];
     uint8_t *quad = s_quad[wv];
@@ -2239,13 +2241,16 @@ k_fill(BatchArgs a) {
             have = lane < K;
             unsigned long long have_m = __ballot(have);
             uint32_t pairs = (uint32_t)__builtin_amdgcn_readlane((int)pe, (int)(K - 1u));
+            // the chunk's start map: bit pb for each fill's first pair (the
+            // clear and the ORs are this wavefront's LDS operations, in order)
+            if (lane < 16u) pstart[lane] = 0ull;
             if (have) {
                 rec[lane] = f;
                 // first column - first pair (low 16 bits, signed), first column
                 // (4 bits), visible columns (5 bits), tile slot (6 bits)
                 cadj[lane] = (((uint32_t)fc.c0 - pb) & 0xffffu) | ((fc.c0 & 15u) << 16) |
                              (nvis << 20) | (ts << 25);
-                for (uint32_t j = 0; j < span; j++) pair_fill[pb + j] = (uint8_t)lane;
+                atomicOr(&pstart[pb >> 6], 1ull << (pb & 63u));
             }
             // the fills that end their tile's stream
             unsigned long long fin_m = __ballot(have && tlast);
@@ -2262,8 +2267,11 @@ k_fill(BatchArgs a) {
                 // value)
                 uint32_t p = r0 + lane;
                 uint32_t k = 0, pc = 16u, pt = 64u;
+                // pair p's fill: the last whose first pair is at or before p
+                unsigned long long sb = pstart[r0 >> 6];
+                uint32_t before = (uint32_t)__popcll(__ballot(have && pb < r0));
                 if (p < r1) {
-                    k = pair_fill[p];
+                    k = before + (uint32_t)__popcll(sb & ((2ull << lane) - 1ull)) - 1u;
                     uint32_t ck = cadj[k];
                     uint32_t c = p + (uint32_t)(int32_t)(int16_t)(ck & 0xffffu);
                     if (c - ((ck >> 16) & 15u) < ((ck >> 20) & 31u)) pc = c;
